@@ -1,0 +1,407 @@
+// C ABI of the HIP ray casters: handle lifetime, inputs, parameters, dispatch.
+// Declarations and the reference interface each entry point replaces: include/cvr.h.
+#include "cvr_ctx.hpp"
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace cvr {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+VolGeom geom_of(const cvr_ctx* c) {
+  VolGeom g{};
+  g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.dtype = c->dtype;
+  const float s[3] = {c->sx, c->sy, c->sz};
+  const int n[3] = {c->nx, c->ny, c->nz};
+  for (int i = 0; i < 3; ++i) {
+    g.scale[i] = s[i];
+    // VolumeGridSize = resolution * voxel size (rc1prenderer.cpp:233-241), float
+    g.G[i] = float(n[i]) * s[i];
+    g.tex_mul[i] = float(double(n[i]) / double(g.G[i]));
+  }
+  g.vox = c->d_vol;
+  return g;
+}
+
+// rc1prenderer.cpp:62-63: step = (0.5/sqrt(3)) * |voxel scale|, evaluated as the reference does.
+float default_step(const cvr_ctx* c) {
+  const double sx = c->sx, sy = c->sy, sz = c->sz;
+  return float((0.5f / std::sqrt(3.0f)) * std::sqrt(sx * sx + sy * sy + sz * sz));
+}
+
+// StructuredGridVolume::GetDiagonal in the float form ebsrenderer.cpp:99-105 uses.
+float volume_diagonal(const cvr_ctx* c) {
+  const float w = float(c->nx) * c->sx, h = float(c->ny) * c->sy, d = float(c->nz) * c->sz;
+  return std::sqrt(w * w + h * h + d * d);
+}
+
+// glm::lookAt basis in double (vis_utils/camera.cpp:281-284), rounded once to float.
+RayFrame frame_of(const cvr_camera* cam, int W, int H) {
+  RayFrame rf{};
+  double e[3], c[3], u[3];
+  for (int i = 0; i < 3; ++i) { e[i] = cam->eye[i]; c[i] = cam->center[i]; u[i] = cam->up[i]; }
+  double f[3] = {c[0] - e[0], c[1] - e[1], c[2] - e[2]};
+  double fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+  for (double& x : f) x /= fl;
+  double s[3] = {f[1] * u[2] - f[2] * u[1], f[2] * u[0] - f[0] * u[2], f[0] * u[1] - f[1] * u[0]};
+  double sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  for (double& x : s) x /= sl;
+  double uu[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2], s[0] * f[1] - s[1] * f[0]};
+  for (int i = 0; i < 3; ++i) {
+    rf.eye[i] = cam->eye[i];
+    rf.s[i] = float(s[i]);
+    rf.u[i] = float(uu[i]);
+    rf.f[i] = float(f[i]);
+  }
+  rf.tanfov = float(std::tan((double(cam->fovy_deg) * (3.14159265358979323846 / 180.0)) / 2.0));
+  rf.aspect = cam->aspect > 0 ? cam->aspect : float(W) / float(H);
+  rf.W = W;
+  rf.H = H;
+  return rf;
+}
+
+static size_t params_size(int technique) {
+  switch (technique) {
+    case CVR_RC1P: return sizeof(cvr_rc1p_params);
+    case CVR_EBS: return sizeof(cvr_ebs_params);
+    default: return 0;
+  }
+}
+
+static void defaults(int technique, void* p) {
+  if (technique == CVR_RC1P) {
+    cvr_rc1p_params* r = static_cast<cvr_rc1p_params*>(p);
+    r->step_size = 0.f;
+    r->et_threshold = 0.99f;
+  } else if (technique == CVR_EBS) {
+    cvr_ebs_params* e = static_cast<cvr_ebs_params*>(p);
+    e->step_size = 0.f;
+    e->et_threshold = 0.99f;
+    e->apply_occlusion = 1;
+    e->apply_shadow = 1;
+    e->ao_shells = 15;
+    e->ao_radius = 1.0f;
+    e->sdw_angle_deg = 1.0f;
+    e->sdw_interval = 2.0f;
+    e->sdw_initial = 2.0f;
+    e->sdw_weight = 1.0f;
+    e->sdw_max_dist = 0.f;
+    e->shadow_type = 0;
+    e->ka = 0.5f;
+    e->kd = 0.5f;
+  }
+}
+
+static void free_all(cvr_ctx* c) {
+  if (c->d_vol) (void)hipFree(c->d_vol);
+  if (c->d_tf_rgbt) (void)hipFree(c->d_tf_rgbt);
+  if (c->d_tf_rgba) (void)hipFree(c->d_tf_rgba);
+  if (c->d_sat) (void)hipFree(c->d_sat);
+  if (c->d_ext_lut) (void)hipFree(c->d_ext_lut);
+  if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->d_out) (void)hipFree(c->d_out);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace cvr
+
+using namespace cvr;
+
+extern "C" {
+
+const char* cvr_last_error(void) { return g_last_error.c_str(); }
+
+const char* cvr_technique_name(int technique) {
+  switch (technique) {
+    case CVR_RC1P: return "1-Pass - Ray Casting";
+    case CVR_EBS: return "1-Pass - Ray Casting - Extinction-based";
+    default: return nullptr;
+  }
+}
+
+int cvr_create(int technique, int device, cvr_handle* out) {
+  if (!out) { set_error("cvr_create: out is null"); return CVR_ERR_INVALID; }
+  *out = nullptr;
+  if (params_size(technique) == 0) { set_error("cvr_create: unknown technique"); return CVR_ERR_UNSUPPORTED; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    set_error("cvr_create: no HIP device available");
+    return CVR_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) { set_error("cvr_create: device index out of range"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(device);
+  cvr_ctx* c = new cvr_ctx();
+  c->technique = technique;
+  c->device = device;
+  defaults(CVR_RC1P, &c->rc1p);
+  defaults(CVR_EBS, &c->ebs);
+  if (hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    free_all(c);
+    delete c;
+    set_error("cvr_create: HIP allocation failed");
+    return CVR_ERR_HIP;
+  }
+  *out = c;
+  return CVR_OK;
+}
+
+void cvr_destroy(cvr_handle h) {
+  if (!h) return;
+  DeviceGuard dg(h->device);
+  (void)hipDeviceSynchronize();
+  free_all(h);
+  delete h;
+}
+
+int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, float sy, float sz,
+                   const void* voxels, int is_device) {
+  if (!h || !voxels) { set_error("cvr_set_volume: null handle or voxels"); return CVR_ERR_INVALID; }
+  if (nx < 1 || ny < 1 || nz < 1 || (dtype != CVR_U8 && dtype != CVR_U16)) {
+    set_error("cvr_set_volume: sizes must be >= 1 and dtype CVR_U8 or CVR_U16");
+    return CVR_ERR_INVALID;
+  }
+  if (!(sx > 0) || !(sy > 0) || !(sz > 0)) { set_error("cvr_set_volume: voxel scale must be > 0"); return CVR_ERR_INVALID; }
+  const size_t cells = size_t(nx) * ny * nz;
+  if (cells > (size_t(1) << 34)) { set_error("cvr_set_volume: volume too large"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  const size_t bytes = cells * (dtype == CVR_U8 ? 1 : 2);
+  const bool resize = !(h->d_vol && h->nx == nx && h->ny == ny && h->nz == nz && h->dtype == dtype);
+  if (resize) {
+    if (h->d_vol) CVR_HIP_CHECK(hipFree(h->d_vol));
+    h->d_vol = nullptr;
+    if (h->d_sat) CVR_HIP_CHECK(hipFree(h->d_sat));
+    h->d_sat = nullptr;
+    if (hipMalloc(&h->d_vol, bytes) != hipSuccess) {
+      h->d_vol = nullptr;
+      set_error("cvr_set_volume: out of device memory");
+      return CVR_ERR_NOMEM;
+    }
+  }
+  CVR_HIP_CHECK(hipMemcpy(h->d_vol, voxels, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+  h->nx = nx; h->ny = ny; h->nz = nz; h->dtype = dtype;
+  h->sx = sx; h->sy = sy; h->sz = sz;
+  h->vol_version++;
+  return CVR_OK;
+}
+
+// TransferFunction1D::BuildLinear (transferfunction1d.cpp:319-358) with the float
+// control-point colours of TransferControlPoint (transferfunction.cpp:8-24), then the
+// RGBt / RGBA upload arrays (:58-118).
+int cvr_set_transfer_function(cvr_handle h, int max_density, int extinction_input, int n_rgb,
+                              const double* rgb_iso, int n_alpha, const double* alpha_iso) {
+  if (!h) { set_error("cvr_set_transfer_function: null handle"); return CVR_ERR_INVALID; }
+  if (max_density < 1 || max_density > (1 << 20)) {
+    set_error("cvr_set_transfer_function: max_density out of range");
+    return CVR_ERR_INVALID;
+  }
+  if (n_rgb < 0 || n_alpha < 0 || (n_rgb > 0 && !rgb_iso) || (n_alpha > 0 && !alpha_iso)) {
+    set_error("cvr_set_transfer_function: bad control point arrays");
+    return CVR_ERR_INVALID;
+  }
+  const int T = max_density + 1;
+  std::vector<double> tab(size_t(T) * 4, 0.0);
+  for (int i = 0; i + 1 < n_rgb; ++i) {
+    const int i0 = int(rgb_iso[i * 4 + 3]), i1 = int(rgb_iso[(i + 1) * 4 + 3]);
+    if (i0 == i1 || i0 < 0 || i1 < 0 || i0 >= T || i1 >= T) {
+      set_error("cvr_set_transfer_function: rgb isovalues must be distinct and within [0, max_density]");
+      return CVR_ERR_INVALID;
+    }
+    const float c0[3] = {float(rgb_iso[i * 4]), float(rgb_iso[i * 4 + 1]), float(rgb_iso[i * 4 + 2])};
+    const float c1[3] = {float(rgb_iso[i * 4 + 4]), float(rgb_iso[i * 4 + 5]), float(rgb_iso[i * 4 + 6])};
+    const double diff[3] = {double(c1[0] - c0[0]), double(c1[1] - c0[1]), double(c1[2] - c0[2])};
+    for (int x = i0; x <= i1; ++x) {
+      const double k = double(x - i0) / double(i1 - i0);
+      for (int c = 0; c < 3; ++c) tab[size_t(x) * 4 + c] = double(c0[c]) + diff[c] * k;
+    }
+  }
+  for (int i = 0; i + 1 < n_alpha; ++i) {
+    const int i0 = int(alpha_iso[i * 2 + 1]), i1 = int(alpha_iso[(i + 1) * 2 + 1]);
+    if (i0 == i1 || i0 < 0 || i1 < 0 || i0 >= T || i1 >= T) {
+      set_error("cvr_set_transfer_function: alpha isovalues must be distinct and within [0, max_density]");
+      return CVR_ERR_INVALID;
+    }
+    const float a0 = float(alpha_iso[i * 2]), a1 = float(alpha_iso[i * 2 + 2]);
+    const double diff = double(a1 - a0);
+    for (int x = i0; x <= i1; ++x) {
+      const double k = double(x - i0) / double(i1 - i0);
+      tab[size_t(x) * 4 + 3] = double(a0) + diff * k;
+    }
+  }
+  std::vector<float4> rgbt(T), rgba(T);
+  for (int i = 0; i < T; ++i) {
+    const float r = float(tab[i * 4]), g = float(tab[i * 4 + 1]), b = float(tab[i * 4 + 2]);
+    const float v4 = float(tab[i * 4 + 3]);
+    const float ext = extinction_input ? v4 : float(std::log(1.0 / (1.0 - double(v4))));
+    const float opc = extinction_input ? float(1.0 - std::exp(-double(v4))) : v4;
+    rgbt[i] = make_float4(r, g, b, ext);
+    rgba[i] = make_float4(r, g, b, opc);
+  }
+  DeviceGuard dg(h->device);
+  if (h->tf_size != T) {
+    if (h->d_tf_rgbt) CVR_HIP_CHECK(hipFree(h->d_tf_rgbt));
+    if (h->d_tf_rgba) CVR_HIP_CHECK(hipFree(h->d_tf_rgba));
+    h->d_tf_rgbt = h->d_tf_rgba = nullptr;
+    if (hipMalloc(&h->d_tf_rgbt, T * sizeof(float4)) != hipSuccess ||
+        hipMalloc(&h->d_tf_rgba, T * sizeof(float4)) != hipSuccess) {
+      set_error("cvr_set_transfer_function: out of device memory");
+      return CVR_ERR_NOMEM;
+    }
+  }
+  CVR_HIP_CHECK(hipMemcpy(h->d_tf_rgbt, rgbt.data(), T * sizeof(float4), hipMemcpyHostToDevice));
+  CVR_HIP_CHECK(hipMemcpy(h->d_tf_rgba, rgba.data(), T * sizeof(float4), hipMemcpyHostToDevice));
+  h->tf_size = T;
+  h->tf_max_density = max_density;
+  h->tf_ext_input = extinction_input != 0;
+  h->tf_tab.swap(tab);
+  h->tf_version++;
+  return CVR_OK;
+}
+
+int cvr_default_params(int technique, void* params, size_t bytes) {
+  const size_t n = params_size(technique);
+  if (n == 0) { set_error("cvr_default_params: unknown technique"); return CVR_ERR_UNSUPPORTED; }
+  if (!params || bytes != n) { set_error("cvr_default_params: wrong params size"); return CVR_ERR_INVALID; }
+  defaults(technique, params);
+  return CVR_OK;
+}
+
+static void* live_params(cvr_ctx* c) {
+  switch (c->technique) {
+    case CVR_RC1P: return &c->rc1p;
+    case CVR_EBS: return &c->ebs;
+    default: return nullptr;
+  }
+}
+
+int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
+  if (!h || !params) { set_error("cvr_set_params: null argument"); return CVR_ERR_INVALID; }
+  if (bytes != params_size(h->technique)) { set_error("cvr_set_params: wrong params size for technique"); return CVR_ERR_INVALID; }
+  if (h->technique == CVR_RC1P) {
+    const cvr_rc1p_params* p = static_cast<const cvr_rc1p_params*>(params);
+    if (!(p->et_threshold > 0) || !std::isfinite(p->step_size)) {
+      set_error("cvr_set_params: et_threshold must be > 0");
+      return CVR_ERR_INVALID;
+    }
+  }
+  std::memcpy(live_params(h), params, bytes);
+  return CVR_OK;
+}
+
+int cvr_get_params(cvr_handle h, void* params, size_t bytes) {
+  if (!h || !params) { set_error("cvr_get_params: null argument"); return CVR_ERR_INVALID; }
+  if (bytes != params_size(h->technique)) { set_error("cvr_get_params: wrong params size for technique"); return CVR_ERR_INVALID; }
+  std::memcpy(params, live_params(h), bytes);
+  return CVR_OK;
+}
+
+static int check_inputs(cvr_ctx* c) {
+  if (!c->d_vol) { set_error("no volume set (cvr_set_volume)"); return CVR_ERR_STATE; }
+  if (!c->d_tf_rgbt) { set_error("no transfer function set (cvr_set_transfer_function)"); return CVR_ERR_STATE; }
+  return CVR_OK;
+}
+
+int cvr_prepare(cvr_handle h, void* hip_stream) {
+  if (!h) { set_error("cvr_prepare: null handle"); return CVR_ERR_INVALID; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  if (h->technique == CVR_EBS) return build_sat(h, st);
+  return CVR_OK;
+}
+
+int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
+               float* out_rgba, int out_is_device, void* hip_stream) {
+  if (!h || !cam || !out_rgba) { set_error("cvr_render: null argument"); return CVR_ERR_INVALID; }
+  if (width < 1 || height < 1 || size_t(width) * height > (size_t(1) << 28)) {
+    set_error("cvr_render: image size out of range");
+    return CVR_ERR_INVALID;
+  }
+  if (!(cam->fovy_deg > 0 && cam->fovy_deg < 180)) { set_error("cvr_render: fovy must be in (0, 180)"); return CVR_ERR_INVALID; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  const size_t bytes = size_t(width) * height * 4 * sizeof(float);
+  float* dst = out_rgba;
+  if (!out_is_device) {
+    if (h->d_out_bytes < bytes) {
+      if (h->d_out) CVR_HIP_CHECK(hipFree(h->d_out));
+      h->d_out = nullptr;
+      h->d_out_bytes = 0;
+      if (hipMalloc(&h->d_out, bytes) != hipSuccess) { set_error("cvr_render: out of device memory"); return CVR_ERR_NOMEM; }
+      h->d_out_bytes = bytes;
+    }
+    dst = h->d_out;
+  }
+  if (h->collect_stats) CVR_HIP_CHECK(hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), st));
+  RayFrame rf = frame_of(cam, width, height);
+  CVR_HIP_CHECK(hipEventRecord(h->ev0, st));
+  switch (h->technique) {
+    case CVR_RC1P: rc = launch_rc1p(h, rf, dst, st); break;
+    case CVR_EBS: rc = launch_ebs(h, rf, light, dst, st); break;
+    default: set_error("cvr_render: technique not available"); rc = CVR_ERR_UNSUPPORTED;
+  }
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipEventRecord(h->ev1, st));
+  if (!out_is_device) {
+    CVR_HIP_CHECK(hipMemcpyAsync(out_rgba, dst, bytes, hipMemcpyDeviceToHost, st));
+    CVR_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  if (h->collect_stats) {
+    unsigned long long cnt[4];
+    CVR_HIP_CHECK(hipMemcpyAsync(cnt, h->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    CVR_HIP_CHECK(hipStreamSynchronize(st));
+    h->stats.samples = cnt[0];
+    h->stats.shaded_samples = cnt[1];
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_ms = ms;
+  }
+  return CVR_OK;
+}
+
+int cvr_get_sat(cvr_handle h, double* out_host, size_t count) {
+  if (!h || !out_host) { set_error("cvr_get_sat: null argument"); return CVR_ERR_INVALID; }
+  if (h->technique != CVR_EBS) { set_error("cvr_get_sat: technique has no SAT"); return CVR_ERR_UNSUPPORTED; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  const size_t cells = size_t(h->nx + 2) * (h->ny + 2) * (h->nz + 2);
+  if (count != cells) { set_error("cvr_get_sat: count must be (nx+2)(ny+2)(nz+2)"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  rc = build_sat(h, nullptr);
+  if (rc) return rc;
+  std::vector<float2> tmp(cells);
+  CVR_HIP_CHECK(hipMemcpy(tmp.data(), h->d_sat, cells * sizeof(float2), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < cells; ++i) out_host[i] = double(tmp[i].x) + double(tmp[i].y);
+  return CVR_OK;
+}
+
+int cvr_get_stats(cvr_handle h, cvr_stats* out) {
+  if (!h || !out) { set_error("cvr_get_stats: null argument"); return CVR_ERR_INVALID; }
+  *out = h->stats;
+  return CVR_OK;
+}
+
+int cvr_set_collect_stats(cvr_handle h, int enable) {
+  if (!h) { set_error("cvr_set_collect_stats: null handle"); return CVR_ERR_INVALID; }
+  h->collect_stats = enable != 0;
+  return CVR_OK;
+}
+
+}  // extern "C"

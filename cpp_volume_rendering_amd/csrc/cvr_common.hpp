@@ -1,0 +1,140 @@
+// Shared host/device definitions of the HIP ray casters (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+namespace cvr {
+
+// Error reporting: the C ABI sets a thread-local message and returns a status.
+void set_error(const std::string& msg);
+
+#define CVR_HIP_CHECK(expr)                                                                   \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      ::cvr::set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr);    \
+      return -2;                                                                              \
+    }                                                                                         \
+  } while (0)
+
+// --------------------------------------------------------------------------- camera
+// Per-frame ray set-up, computed on the host in double and rounded once.
+// dir(px,py) = normalize(a*s + b*u + f) is the expansion of
+// `vec3(Vx*tan*aspect, Vy*tan, -1) * mat3(View)` (ray_marching_1p.comp:170) for
+// View = glm::lookAt(eye, center, up) = rows (s, u, -f).
+struct RayFrame {
+  float eye[3];
+  float s[3], u[3], f[3];
+  float tanfov, aspect;
+  int W, H;
+};
+
+// Volume geometry shared by all marchers.
+struct VolGeom {
+  int nx, ny, nz, dtype;
+  float G[3];        // box size, world units
+  float scale[3];    // voxel size
+  float tex_mul[3];  // n / G: world -> texel units
+  const void* vox;   // raw voxels, x fastest
+};
+
+// --------------------------------------------------------------------------- device helpers
+#if defined(__HIPCC__)
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float a, float b, float c) { return f3{a, b, c}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 norm3(f3 v) {
+  float r = 1.0f / sqrtf(dot3(v, v));
+  return v * r;
+}
+__device__ __forceinline__ f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
+
+// Camera ray of pixel (px, py), ray_marching_1p.comp:164-170.
+__device__ __forceinline__ f3 camera_ray(const RayFrame& rf, int px, int py) {
+  float vx = ((float(px) + 0.5f) / float(rf.W)) * 2.0f - 1.0f;
+  float vy = ((float(py) + 0.5f) / float(rf.H)) * 2.0f - 1.0f;
+  float a = vx * rf.tanfov * rf.aspect, b = vy * rf.tanfov;
+  f3 d = ld3(rf.s) * a + ld3(rf.u) * b + ld3(rf.f);
+  return norm3(d);
+}
+
+// Slab test against [-G/2, G/2], ray_bbox_intersection.comp:18-52.
+__device__ __forceinline__ bool ray_box(f3 o, f3 d, const float* G, float& tnear, float& tfar) {
+  float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+  float ax = ix * (-0.5f * G[0] - o.x), bx = ix * (0.5f * G[0] - o.x);
+  float ay = iy * (-0.5f * G[1] - o.y), by = iy * (0.5f * G[1] - o.y);
+  float az = iz * (-0.5f * G[2] - o.z), bz = iz * (0.5f * G[2] - o.z);
+  tnear = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+  tfar = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+  bool hit = tfar > tnear;
+  tnear = fmaxf(tnear, 0.0f);
+  return hit;
+}
+
+// GL_LINEAR / CLAMP_TO_EDGE along one axis: texel coordinate x (already u*n - 1/2).
+__device__ __forceinline__ void lin_axis(float x, int n, int& i0, int& i1, float& f) {
+  float fl = floorf(x);
+  f = x - fl;
+  int i = int(fl);
+  i0 = min(max(i, 0), n - 1);
+  i1 = min(max(i + 1, 0), n - 1);
+}
+
+// Trilinear sample of the raw volume at world-in-box position p (texture space
+// [0, G]); returns the normalised density v/255 or v/65535
+// (structuredgridvolume.cpp:121-151 + GL_LINEAR, utils.cpp:8-9).
+template <typename T>
+__device__ __forceinline__ float sample_volume(const T* __restrict__ v, const VolGeom& g, f3 p) {
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  lin_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, x1, fx);
+  lin_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, y1, fy);
+  lin_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, z1, fz);
+  const size_t sy = size_t(g.nx), sz = size_t(g.nx) * g.ny;
+  const size_t r00 = y0 * sy + z0 * sz, r10 = y1 * sy + z0 * sz, r01 = y0 * sy + z1 * sz, r11 = y1 * sy + z1 * sz;
+  float c000 = float(v[r00 + x0]), c100 = float(v[r00 + x1]);
+  float c010 = float(v[r10 + x0]), c110 = float(v[r10 + x1]);
+  float c001 = float(v[r01 + x0]), c101 = float(v[r01 + x1]);
+  float c011 = float(v[r11 + x0]), c111 = float(v[r11 + x1]);
+  float c00 = c000 + fx * (c100 - c000), c10 = c010 + fx * (c110 - c010);
+  float c01 = c001 + fx * (c101 - c001), c11 = c011 + fx * (c111 - c011);
+  float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+  float c = c0 + fz * (c1 - c0);
+  return c * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
+}
+
+// texture(TexTransferFunc, d) over a float4 table of T entries (GL_LINEAR, CLAMP_TO_EDGE).
+__device__ __forceinline__ float4 sample_tf(const float4* __restrict__ tf, int T, float d) {
+  int i0, i1;
+  float f;
+  lin_axis(d * float(T) - 0.5f, T, i0, i1, f);
+  float4 a = tf[i0], b = tf[i1];
+  return make_float4(a.x + f * (b.x - a.x), a.y + f * (b.y - a.y), a.z + f * (b.z - a.z), a.w + f * (b.w - a.w));
+}
+
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin to
+// the 8 XCDs; remap so each XCD works on one contiguous run of image tiles and
+// its private L2 sees one coherent region of the volume.
+__device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
+  const int nx = 8;
+  int per = nblocks / nx;
+  int rem = nblocks - per * nx;
+  int xcd = bid % nx, idx = bid / nx;
+  if (per == 0) return bid;
+  if (bid >= per * nx) return bid;  // tail blocks keep their id
+  (void)rem;
+  return xcd * per + idx;
+}
+#endif
+
+}  // namespace cvr

@@ -1,0 +1,408 @@
+// Extinction-based shading (Schlegel et al. 2011), gfx950.
+//
+// Semantics follow cppvolrend/structured/rc1pextbsd/ebs_ray_bbox_marching.comp:
+//   ambient occlusion from nested SAT boxes (:102-146), a dominant-axis cone
+//   shadow built from a chain of SAT boxes (:148-456), shading :500-551, the
+//   march :553-627.
+// The SAT is sampled with GL_LINEAR semantics like the reference's R32F texture,
+// but every tap and every 8-tap box sum is evaluated in FP64 on the two-float
+// (hi + lo) cells: the prefix values reach ~1e7 while small AO boxes hold ~10,
+// and an fp32 box sum would keep no correct digit there.
+// For a directional light the cone geometry is the same for every sample: each
+// workgroup builds the chain of boxes once in FP64 (offsets, next-position cut
+// coordinate, box volume) into LDS.  For a point light each sample builds its own
+// cone in fp32 with the same formulas.
+#include "cvr_ctx.hpp"
+#include <cmath>
+#include <vector>
+
+namespace cvr {
+
+// ----------------------------------------------------------------- cone geometry
+// Per-axis literal forms of ConeZAxis / ConeYAxis / ConeXAxis (:190-430).
+template <class R> struct ConeGeom {
+  int axis, sa, sb;
+  R interval, apos0;
+  R pj[4][3];
+  R scale[3];
+};
+
+// float / double overloads usable from host and device code
+__host__ __device__ inline float g_abs(float x) { return fabsf(x); }
+__host__ __device__ inline double g_abs(double x) { return fabs(x); }
+__host__ __device__ inline float g_cos(float x) { return cosf(x); }
+__host__ __device__ inline double g_cos(double x) { return cos(x); }
+__host__ __device__ inline float g_sin(float x) { return sinf(x); }
+__host__ __device__ inline double g_sin(double x) { return sin(x); }
+__host__ __device__ inline float g_sqrt(float x) { return sqrtf(x); }
+__host__ __device__ inline double g_sqrt(double x) { return sqrt(x); }
+__host__ __device__ inline float g_ceil(float x) { return ceilf(x); }
+__host__ __device__ inline double g_ceil(double x) { return ceil(x); }
+__host__ __device__ inline float g_min(float a, float b) { return fminf(a, b); }
+__host__ __device__ inline double g_min(double a, double b) { return fmin(a, b); }
+__host__ __device__ inline float g_max(float a, float b) { return fmaxf(a, b); }
+__host__ __device__ inline double g_max(double a, double b) { return fmax(a, b); }
+
+template <class R> __host__ __device__ inline void nrm(R v[3]) {
+  R l = g_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= l; v[1] /= l; v[2] /= l;
+}
+
+template <class R>
+__host__ __device__ inline void cone_setup(const R cv[3], R ang, R interval_vox, R initial_vox, const R scale[3],
+                                           ConeGeom<R>& g) {
+  R ac[3] = {g_abs(cv[0]), g_abs(cv[1]), g_abs(cv[2])};
+  int axis = (ac[2] > ac[0] && ac[2] > ac[1]) ? 2 : (ac[1] > ac[0] ? 1 : 0);  // :446-453
+  R pc = g_cos(ang), ps = g_sin(ang), nc = g_cos(-ang), ns = g_sin(-ang);
+  g.axis = axis;
+  for (int i = 0; i < 3; ++i) g.scale[i] = scale[i];
+  R sgn = cv[axis] < 0 ? R(-1) : R(1);
+  g.interval = interval_vox * sgn * scale[axis];
+  g.apos0 = initial_vox * sgn * scale[axis];
+  R(*pj)[3] = g.pj;
+  if (axis == 2) {  // :198-216
+    g.sa = 0; g.sb = 1;
+    R py[3] = {0, cv[1], cv[2]}; nrm(py);
+    R px[3] = {cv[0], 0, cv[2]}; nrm(px);
+    R a0[3] = {px[0] * nc - px[2] * ns, 0, px[0] * ns + px[2] * nc};
+    R a1[3] = {px[0] * pc - px[2] * ps, 0, px[0] * ps + px[2] * pc};
+    R b0[3] = {0, py[1] * nc - py[2] * ns, py[1] * ns + py[2] * nc};
+    R b1[3] = {0, py[1] * pc - py[2] * ps, py[1] * ps + py[2] * pc};
+    nrm(a0); nrm(a1); nrm(b0); nrm(b1);
+    for (int i = 0; i < 3; ++i) { pj[0][i] = a0[i]; pj[1][i] = a1[i]; pj[2][i] = b0[i]; pj[3][i] = b1[i]; }
+  } else if (axis == 1) {  // :280-298
+    g.sa = 0; g.sb = 2;
+    R px[3] = {cv[0], cv[1], 0}; nrm(px);
+    R pz[3] = {0, cv[1], cv[2]}; nrm(pz);
+    R a0[3] = {px[0] * nc - px[1] * ns, px[0] * ns + px[1] * nc, 0};
+    R a1[3] = {px[0] * pc - px[1] * ps, px[0] * ps + px[1] * pc, 0};
+    R b0[3] = {0, pz[2] * ns + pz[1] * nc, pz[2] * nc - pz[1] * ns};
+    R b1[3] = {0, pz[2] * ps + pz[1] * pc, pz[2] * pc - pz[1] * ps};
+    nrm(a0); nrm(a1); nrm(b0); nrm(b1);
+    for (int i = 0; i < 3; ++i) { pj[0][i] = a0[i]; pj[1][i] = a1[i]; pj[2][i] = b0[i]; pj[3][i] = b1[i]; }
+  } else {  // :360-378
+    g.sa = 1; g.sb = 2;
+    R py[3] = {cv[0], cv[1], 0}; nrm(py);
+    R pz[3] = {cv[0], 0, cv[2]}; nrm(pz);
+    R a0[3] = {py[1] * ns + py[0] * nc, py[1] * nc - py[0] * ns, 0};
+    R a1[3] = {py[1] * ps + py[0] * pc, py[1] * pc - py[0] * ps, 0};
+    R b0[3] = {pz[2] * ns + pz[0] * nc, 0, pz[2] * nc - pz[0] * ns};
+    R b1[3] = {pz[2] * ps + pz[0] * pc, 0, pz[2] * pc - pz[0] * ps};
+    nrm(a0); nrm(a1); nrm(b0); nrm(b1);
+    for (int i = 0; i < 3; ++i) { pj[0][i] = a0[i]; pj[1][i] = a1[i]; pj[2][i] = b0[i]; pj[3][i] = b1[i]; }
+  }
+}
+
+// Box of one cone step relative to the sample position (:229-261).
+template <class R>
+__host__ __device__ inline void cone_box(const ConeGeom<R>& g, R apos, R o1[3], R o2[3]) {
+  const int a = g.axis, sa = g.sa, sb = g.sb;
+  R mean = g_abs(apos + g.interval * R(0.5));
+  R pa1 = g.pj[0][sa] * (mean / g_abs(g.pj[0][a]));
+  R pa2 = g.pj[1][sa] * (mean / g_abs(g.pj[1][a]));
+  R pb1 = g.pj[2][sb] * (mean / g_abs(g.pj[2][a]));
+  R pb2 = g.pj[3][sb] * (mean / g_abs(g.pj[3][a]));
+  R a1 = g_min(pa1, pa2), a2 = g_max(pa1, pa2);
+  R b1 = g_min(pb1, pb2), b2 = g_max(pb1, pb2);
+  R qa = g_abs(a2 - a1) / g.scale[sa], qb = g_abs(b2 - b1) / g.scale[sb];
+  R as = (g_ceil(qa) - qa) * R(0.5), bs = (g_ceil(qb) - qb) * R(0.5);
+  a1 -= as * g.scale[sa]; a2 += as * g.scale[sa];
+  b1 -= bs * g.scale[sb]; b2 += bs * g.scale[sb];
+  o1[a] = g_min(apos, apos + g.interval); o2[a] = g_max(apos, apos + g.interval);
+  o1[sa] = a1; o2[sa] = a2;
+  o1[sb] = b1; o2[sb] = b2;
+}
+
+// One box of the directional-light cone chain.
+struct ShadowRow {
+  float o1[3], o2[3];
+  float next;     // apos + interval (the CUT_WHEN_AWAY_FROM_VOLUME coordinate)
+  float inv_vq;   // 1 / box volume in voxels
+};
+
+struct EbsK {
+  VolGeom g;
+  int w, h, d;           // SAT dims
+  double sat_tex_mul[3]; // SAT texel per world unit: w / (G + 2 s)
+  float scale[3], minSAT[3], maxSAT[3], minVol[3], maxVol[3];
+  int apply_occ, apply_sdw, shells, shadow_type, nrows;
+  float ao_radius, ka, kd, sdw_weight, sdw_angle, sdw_interval, sdw_initial, sdw_max_dist;
+  float light_pos[3], light_fwd[3];
+  int axis;
+};
+
+__device__ __forceinline__ double sat_cell(const float2* __restrict__ sat, size_t i) {
+  float2 v = sat[i];
+  return double(v.x) + double(v.y);
+}
+// GL_LINEAR tap of the SAT at texel coordinate (x, y, z), :74-83, in FP64.
+__device__ __forceinline__ double sat_tap(const float2* __restrict__ sat, const EbsK& k, double x, double y,
+                                          double z) {
+  double fx0 = floor(x), fy0 = floor(y), fz0 = floor(z);
+  double fx = x - fx0, fy = y - fy0, fz = z - fz0;
+  int x0 = min(max(int(fx0), 0), k.w - 1), x1 = min(max(int(fx0) + 1, 0), k.w - 1);
+  int y0 = min(max(int(fy0), 0), k.h - 1), y1 = min(max(int(fy0) + 1, 0), k.h - 1);
+  int z0 = min(max(int(fz0), 0), k.d - 1), z1 = min(max(int(fz0) + 1, 0), k.d - 1);
+  const size_t sy = size_t(k.w), sz = size_t(k.w) * k.h;
+  size_t r00 = y0 * sy + z0 * sz, r10 = y1 * sy + z0 * sz, r01 = y0 * sy + z1 * sz, r11 = y1 * sy + z1 * sz;
+  double c000 = sat_cell(sat, r00 + x0), c100 = sat_cell(sat, r00 + x1);
+  double c010 = sat_cell(sat, r10 + x0), c110 = sat_cell(sat, r10 + x1);
+  double c001 = sat_cell(sat, r01 + x0), c101 = sat_cell(sat, r01 + x1);
+  double c011 = sat_cell(sat, r11 + x0), c111 = sat_cell(sat, r11 + x1);
+  double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
+  double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
+  double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
+  return c0 * (1 - fz) + c1 * fz;
+}
+
+// EvaluateSAT3D (:87-100) after the border offset and clamp (:102-110 / :181-183).
+__device__ __forceinline__ double box_sum(const float2* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
+  p1 = mk3(fminf(fmaxf(p1.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
+           fminf(fmaxf(p1.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
+           fminf(fmaxf(p1.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
+  p2 = mk3(fminf(fmaxf(p2.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
+           fminf(fmaxf(p2.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
+           fminf(fmaxf(p2.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
+  const double x1 = double(p1.x) * k.sat_tex_mul[0] - 0.5, x2 = double(p2.x) * k.sat_tex_mul[0] - 0.5;
+  const double y1 = double(p1.y) * k.sat_tex_mul[1] - 0.5, y2 = double(p2.y) * k.sat_tex_mul[1] - 0.5;
+  const double z1 = double(p1.z) * k.sat_tex_mul[2] - 0.5, z2 = double(p2.z) * k.sat_tex_mul[2] - 0.5;
+  double V1 = sat_tap(sat, k, x2, y2, z2), V2 = sat_tap(sat, k, x1, y2, z2);
+  double V3 = sat_tap(sat, k, x2, y2, z1), V4 = sat_tap(sat, k, x1, y2, z1);
+  double V5 = sat_tap(sat, k, x2, y1, z2), V6 = sat_tap(sat, k, x1, y1, z2);
+  double V7 = sat_tap(sat, k, x2, y1, z1), V8 = sat_tap(sat, k, x1, y1, z1);
+  return V1 - V2 - V3 + V4 - V5 + V6 + V7 - V8;
+}
+
+// ExtinctionAmbientOcclusion (:112-146).
+__device__ float ebs_ao(const float2* __restrict__ sat, const EbsK& k, f3 tx) {
+  const f3 sc = ld3(k.scale);
+  const double rad = k.ao_radius;
+  double S0 = box_sum(sat, k, tx - sc * k.ao_radius, tx + sc * k.ao_radius);
+  double t = S0 * (1.0 / (rad * rad));
+  double Si = S0;
+  for (int i = 1; i < k.shells; ++i) {
+    const float r1f = k.ao_radius * float(i + 1);
+    const double r1 = rad * double(i + 1);
+    double S1 = box_sum(sat, k, tx - sc * r1f, tx + sc * r1f);
+    t = t + (S1 - Si) * (1.0 / (r1 * r1));
+    Si = S1;
+  }
+  const double rn = rad * double(k.shells);
+  return __expf(-float((1.0 / (rn * rn)) * t));
+}
+
+__device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// EvaluateShadowSAT3D (:148-188) summed along the cone; directional light from the table.
+__device__ float ebs_shadow_table(const float2* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
+                                  f3 tx) {
+  const float pa = comp(tx, k.axis), lo = k.minVol[k.axis], hi = k.maxVol[k.axis];
+  double st = 0.0;
+  for (int r = 0; r < nrows; ++r) {
+    const ShadowRow& R = rows[r];
+    const float nxt = pa + R.next;
+    if (!(nxt > lo && nxt < hi)) break;
+    f3 p1 = tx + ld3(R.o1), p2 = tx + ld3(R.o2);
+    st += box_sum(sat, k, p1, p2) * double(R.inv_vq) * double(k.sdw_weight);
+  }
+  return __expf(-float(st));
+}
+
+// Point light: the cone axis depends on the sample (:437-438).
+__device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k, f3 tx) {
+  f3 real = tx - mk3(0.5f * k.g.G[0], 0.5f * k.g.G[1], 0.5f * k.g.G[2]);
+  f3 cvv = norm3(ld3(k.light_pos) - real);
+  float cv[3] = {cvv.x, cvv.y, cvv.z};
+  ConeGeom<float> cg;
+  cone_setup<float>(cv, k.sdw_angle, k.sdw_interval, k.sdw_initial, k.scale, cg);
+  const int a = cg.axis;
+  const float pa = comp(tx, a), lo = k.minVol[a], hi = k.maxVol[a];
+  float apos = cg.apos0;
+  double st = 0.0;
+  while ((apos / cv[a]) < k.sdw_max_dist) {
+    const float nxt = pa + (apos + cg.interval);
+    if (!(nxt > lo && nxt < hi)) break;
+    float o1[3], o2[3];
+    cone_box<float>(cg, apos, o1, o2);
+    f3 p1 = tx + ld3(o1), p2 = tx + ld3(o2);
+    const float vq = (fabsf(p1.x - p2.x) / k.scale[0]) * (fabsf(p1.y - p2.y) / k.scale[1]) *
+                     (fabsf(p1.z - p2.z) / k.scale[2]);
+    st += (box_sum(sat, k, p1, p2) / double(vq)) * double(k.sdw_weight);
+    apos = apos + cg.interval;
+  }
+  return __expf(-float(st));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
+                                              const float2* __restrict__ sat, float step, float et, float4* __restrict__ out,
+                                              unsigned long long* __restrict__ counters) {
+  extern __shared__ float4 smem[];
+  __shared__ int s_nrows;
+  float4* s_tf = smem;
+  ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem + ntf);
+  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+  if (threadIdx.x == 0) s_nrows = k.nrows;
+  __syncthreads();
+  if (k.apply_sdw && k.shadow_type == 1) {
+    // Directional light: the cone chain is the same for every sample.  Each block
+    // builds it in FP64 (the decisions in ceil() and the distance test then match a
+    // double evaluation of :190-430 exactly) into LDS; row r is the r-th box.
+    double cv[3] = {double(k.light_fwd[0]), double(k.light_fwd[1]), double(k.light_fwd[2])};
+    nrm(cv);
+    const double scd[3] = {double(k.scale[0]), double(k.scale[1]), double(k.scale[2])};
+    ConeGeom<double> cg;
+    cone_setup<double>(cv, double(k.sdw_angle), double(k.sdw_interval), double(k.sdw_initial), scd, cg);
+    for (int r = threadIdx.x; r < k.nrows; r += blockDim.x) {
+      double apos = cg.apos0;
+      for (int i = 0; i < r; ++i) apos = apos + cg.interval;
+      if (!((apos / cv[cg.axis]) < double(k.sdw_max_dist))) {
+        atomicMin(&s_nrows, r);
+        continue;
+      }
+      double o1[3], o2[3];
+      cone_box<double>(cg, apos, o1, o2);
+      ShadowRow row;
+      double vq = 1.0;
+      for (int i = 0; i < 3; ++i) {
+        row.o1[i] = float(o1[i]);
+        row.o2[i] = float(o2[i]);
+        vq *= fabs(o1[i] - o2[i]) / scd[i];
+      }
+      row.next = float(apos + cg.interval);
+      row.inv_vq = float(1.0 / vq);
+      s_rows[r] = row;
+    }
+    __syncthreads();
+  }
+  const int nrows = s_nrows;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_x = (rf.W + 15) >> 4;
+  const int tx_ = bid % tiles_x, ty_ = bid / tiles_x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx_ * 16 + (w & 1) * 8 + (lane & 7);
+  const int py = ty_ * 16 + (w >> 1) * 8 + (lane >> 3);
+  if (px >= rf.W || py >= rf.H) return;
+
+  const T* vox = static_cast<const T*>(k.g.vox);
+  f3 dir = camera_ray(rf, px, py);
+  f3 eye = ld3(rf.eye);
+  float tn, tf_;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned int nsamp = 0, nshade = 0;
+  const float ka = k.apply_occ ? k.ka : 0.f, kd = k.apply_sdw ? k.kd : 0.f;
+  const float inv_k = 1.0f / (ka + kd);
+  if (ray_box(eye, dir, k.g.G, tn, tf_)) {
+    const float D = fabsf(tf_ - tn);
+    const f3 tp = eye + dir * tn + mk3(0.5f * k.g.G[0], 0.5f * k.g.G[1], 0.5f * k.g.G[2]);
+    for (int j = 0;; ++j) {
+      const float s = float(j) * step;
+      if (!(s < D)) break;
+      const float h = fminf(step, D - s);
+      const f3 p = tp + dir * (s + 0.5f * h);
+      const float dens = sample_volume<T>(vox, k.g, p);
+      float4 src = sample_tf(s_tf, ntf, dens);
+      ++nsamp;
+      if (src.w > 0.0f) {
+        ++nshade;
+        float Io = 0.f, Is = 0.f;
+        if (k.apply_occ) Io = ebs_ao(sat, k, p);
+        if (k.apply_sdw) Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, p) : ebs_shadow_point(sat, k, p);
+        // (1/(ka+kd)) * (rgb*Io*ka + rgb*Is*kd), :547
+        src.x = inv_k * (src.x * Io * ka + src.x * Is * kd);
+        src.y = inv_k * (src.y * Io * ka + src.y * Is * kd);
+        src.z = inv_k * (src.z * Io * ka + src.z * Is * kd);
+        const float a = 1.0f - __expf(-src.w * h);
+        const float wgt = (1.0f - dst.w) * a;
+        dst.x += wgt * src.x;
+        dst.y += wgt * src.y;
+        dst.z += wgt * src.z;
+        dst.w += wgt;
+        if (dst.w > et) break;
+      }
+    }
+  }
+  out[size_t(py) * rf.W + px] = dst;
+  if (counters) {
+    atomicAdd(&counters[0], (unsigned long long)nsamp);
+    atomicAdd(&counters[1], (unsigned long long)nshade);
+  }
+}
+
+int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
+  const cvr_ebs_params& P = c->ebs;
+  if (P.apply_occlusion == 0 && P.apply_shadow == 0) {
+    set_error("cvr_render: EBS needs occlusion or shadow enabled (the shading divides by ka+kd)");
+    return -1;
+  }
+  if (P.ao_shells < 1 || !(P.ao_radius > 0) || !(P.sdw_interval > 0)) {
+    set_error("cvr_render: EBS ao_shells >= 1, ao_radius > 0 and sdw_interval > 0 required");
+    return -1;
+  }
+  int rc = build_sat(c, st);
+  if (rc) return rc;
+  EbsK k{};
+  k.g = geom_of(c);
+  k.w = c->nx + 2; k.h = c->ny + 2; k.d = c->nz + 2;
+  const double sc[3] = {double(c->sx), double(c->sy), double(c->sz)};
+  const int dims[3] = {c->nx, c->ny, c->nz};
+  const int sdim[3] = {k.w, k.h, k.d};
+  for (int i = 0; i < 3; ++i) {
+    const float s = float(sc[i]);
+    const float G = k.g.G[i];
+    k.scale[i] = s;
+    k.minSAT[i] = s * 0.5f;
+    k.maxSAT[i] = G + s * 1.5f;
+    k.minVol[i] = s * 0.5f;
+    k.maxVol[i] = G - s * 0.5f;
+    // texel = p / (G + 2s) * (n+2) - 1/2  (GL_LINEAR over the SAT texture)
+    k.sat_tex_mul[i] = double(sdim[i]) / (double(G) + 2.0 * double(s));
+    (void)dims;
+  }
+  k.apply_occ = P.apply_occlusion; k.apply_sdw = P.apply_shadow;
+  k.shells = P.ao_shells; k.shadow_type = P.shadow_type;
+  k.ao_radius = P.ao_radius; k.ka = P.ka; k.kd = P.kd;
+  k.sdw_weight = P.sdw_weight;
+  k.sdw_angle = float(double(P.sdw_angle_deg) * 3.14159265358979323846 / 180.0);
+  k.sdw_interval = P.sdw_interval; k.sdw_initial = P.sdw_initial;
+  k.sdw_max_dist = P.sdw_max_dist > 0 ? P.sdw_max_dist : 0.75f * volume_diagonal(c);
+  if (light) {
+    for (int i = 0; i < 3; ++i) { k.light_pos[i] = light->position[i]; k.light_fwd[i] = light->forward[i]; }
+  } else {
+    for (int i = 0; i < 3; ++i) { k.light_pos[i] = rf.eye[i]; k.light_fwd[i] = -rf.f[i]; }
+  }
+  // directional light: LDS rows for the cone chain; |apos| grows by interval*scale per
+  // row and the chain stops once |apos| / |cv_axis| >= max distance, so this bounds it
+  k.nrows = 0;
+  if (P.apply_shadow && P.shadow_type == 1) {
+    double cv[3] = {k.light_fwd[0], k.light_fwd[1], k.light_fwd[2]};
+    nrm(cv);
+    const double ac[3] = {std::fabs(cv[0]), std::fabs(cv[1]), std::fabs(cv[2])};
+    k.axis = (ac[2] > ac[0] && ac[2] > ac[1]) ? 2 : (ac[1] > ac[0] ? 1 : 0);
+    const double per_row = double(k.sdw_interval) * double(k.scale[k.axis]);
+    const double n = double(k.sdw_max_dist) / per_row + std::fabs(double(k.sdw_initial)) / double(k.sdw_interval) + 2;
+    if (!(n < 4096)) {
+      set_error("cvr_render: EBS shadow chain longer than 4096 boxes");
+      return -4;
+    }
+    k.nrows = int(n);
+  }
+  const float step = P.step_size > 0 ? P.step_size : default_step(c);
+  const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
+  const size_t shm = size_t(c->tf_size) * sizeof(float4) + size_t(k.nrows) * sizeof(ShadowRow);
+  if (shm > 150 * 1024) {
+    set_error("cvr_render: EBS transfer function + shadow table exceed LDS");
+    return -4;
+  }
+  unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
+  if (c->dtype == CVR_U8)
+    hipLaunchKernelGGL(k_ebs<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
+                       step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  else
+    hipLaunchKernelGGL(k_ebs<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
+                       step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace cvr

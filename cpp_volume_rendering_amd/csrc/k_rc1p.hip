@@ -1,0 +1,98 @@
+// Plain single-pass ray casting (front-to-back emission-absorption), gfx950.
+//
+// Semantics follow cppvolrend/structured/rc1pass/ray_marching_1p.comp:156-250
+// (USE_TRANSPARENCY off): midpoint samples of the segment [s, s+h], h = min(step,
+// D - s); the TF holds (r, g, b, extinction); alpha = 1 - exp(-ext*h); composite
+// front to back; stop once accumulated alpha exceeds the threshold.
+//
+// Mapping: one lane per pixel, a wavefront covers an 8x8 pixel tile, a 256-lane
+// workgroup a 16x16 tile; workgroups are re-ordered so each XCD marches one
+// contiguous band of tiles.  The 1D transfer function is staged in LDS.  The
+// sample position is recomputed from the sample index each step (no running
+// sum), so the step schedule does not drift in fp32.
+#include "cvr_ctx.hpp"
+
+namespace cvr {
+
+template <typename T, bool TF_LDS>
+__global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const float4* __restrict__ tf, int ntf,
+                                               float step, float et, float4* __restrict__ out,
+                                               unsigned long long* __restrict__ counters) {
+  extern __shared__ float4 s_tf[];
+  if (TF_LDS) {
+    for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+    __syncthreads();
+  }
+  const float4* tfp = TF_LDS ? s_tf : tf;
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_x = (rf.W + 15) >> 4;
+  const int tx = bid % tiles_x, ty = bid / tiles_x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
+  const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+  if (px >= rf.W || py >= rf.H) return;
+
+  const T* vox = static_cast<const T*>(g.vox);
+  f3 dir = camera_ray(rf, px, py);
+  f3 eye = ld3(rf.eye);
+  float tn, tf_;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned int nsamp = 0, nshade = 0;
+  if (ray_box(eye, dir, g.G, tn, tf_)) {
+    const float D = fabsf(tf_ - tn);
+    const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
+    for (int j = 0;; ++j) {
+      const float s = float(j) * step;
+      if (!(s < D)) break;
+      const float h = fminf(step, D - s);
+      const f3 p = tp + dir * (s + 0.5f * h);
+      const float dens = sample_volume<T>(vox, g, p);
+      float4 src = sample_tf(tfp, ntf, dens);
+      ++nsamp;
+      if (src.w > 0.0f) {
+        ++nshade;
+        const float a = 1.0f - __expf(-src.w * h);
+        const float wgt = (1.0f - dst.w) * a;
+        dst.x += wgt * src.x;
+        dst.y += wgt * src.y;
+        dst.z += wgt * src.z;
+        dst.w += wgt;
+        if (dst.w > et) break;
+      }
+    }
+  }
+  out[size_t(py) * rf.W + px] = dst;
+  if (counters) {
+    atomicAdd(&counters[0], (unsigned long long)nsamp);
+    atomicAdd(&counters[1], (unsigned long long)nshade);
+  }
+}
+
+int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st) {
+  VolGeom g = geom_of(c);
+  float step = c->rc1p.step_size > 0 ? c->rc1p.step_size : default_step(c);
+  float et = c->rc1p.et_threshold;
+  int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
+  bool lds = c->tf_size <= 4096;
+  size_t shm = lds ? size_t(c->tf_size) * sizeof(float4) : 0;
+  unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
+  if (c->dtype == CVR_U8) {
+    if (lds)
+      hipLaunchKernelGGL((k_rc1p<uint8_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
+                         step, et, reinterpret_cast<float4*>(out), cnt);
+    else
+      hipLaunchKernelGGL((k_rc1p<uint8_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
+                         step, et, reinterpret_cast<float4*>(out), cnt);
+  } else {
+    if (lds)
+      hipLaunchKernelGGL((k_rc1p<uint16_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
+                         step, et, reinterpret_cast<float4*>(out), cnt);
+    else
+      hipLaunchKernelGGL((k_rc1p<uint16_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
+                         step, et, reinterpret_cast<float4*>(out), cnt);
+  }
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace cvr

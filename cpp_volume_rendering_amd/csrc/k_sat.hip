@@ -1,0 +1,158 @@
+// 3D summed-area table of extinction on the GPU, gfx950.
+//
+// Replaces the CPU build of the reference (ebsrenderer.cpp:624-723 +
+// vis_utils/summedareatable.h:218-278): a (nx+2)(ny+2)(nz+2) grid, zero border,
+// interior GetExtN(normalised voxel), inclusive prefix sums along x, y and z.
+// The prefix sums accumulate in fp32 as two-float (hi, lo) pairs (Knuth TwoSum),
+// so the table keeps ~2^-44 relative accuracy while every add is an fp32 add; the
+// cell is stored as float2 (hi, lo).
+//
+// Pass 1 (x): one wavefront per row, a 64-lane shuffle scan per 64-cell chunk
+//   with the chunk carry broadcast from lane 63; reads the raw voxels through the
+//   per-value extinction LUT, writes float2.
+// Pass 2 (y) / pass 3 (z): one lane per column, lanes along x (coalesced), each
+//   lane walks its column with 8 loads in flight.
+#include "cvr_ctx.hpp"
+
+namespace cvr {
+
+struct df {
+  float hi, lo;
+};
+__device__ __forceinline__ df two_sum(float a, float b) {
+  float s = a + b;
+  float bb = s - a;
+  float e = (a - (s - bb)) + (b - bb);
+  return df{s, e};
+}
+__device__ __forceinline__ df df_add(df a, df b) {
+  df s = two_sum(a.hi, b.hi);
+  float e = s.lo + (a.lo + b.lo);
+  float hi = s.hi + e;
+  float lo = e - (hi - s.hi);
+  return df{hi, lo};
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sat_x(const T* __restrict__ vox, const float* __restrict__ lut, int nx,
+                                                int ny, int nz, float2* __restrict__ sat) {
+  const int w = nx + 2, h = ny + 2, d = nz + 2;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= h * d) return;
+  const int y = row % h, z = row / h;
+  const bool border_row = (y == 0 || z == 0 || y == h - 1 || z == d - 1);
+  const T* src = border_row ? vox : vox + (size_t(z - 1) * ny + (y - 1)) * nx;
+  float2* dstrow = sat + size_t(row) * w;
+  df carry{0.f, 0.f};
+  for (int base = 0; base < w; base += 64) {
+    const int x = base + lane;
+    float v = 0.f;
+    if (!border_row && x >= 1 && x <= nx) v = lut[src[x - 1]];
+    df acc{v, 0.f};
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      float oh = __shfl_up(acc.hi, off, 64);
+      float ol = __shfl_up(acc.lo, off, 64);
+      if (lane >= off) acc = df_add(acc, df{oh, ol});
+    }
+    acc = df_add(acc, carry);
+    if (x < w) dstrow[x] = make_float2(acc.hi, acc.lo);
+    carry.hi = __shfl(acc.hi, 63, 64);
+    carry.lo = __shfl(acc.lo, 63, 64);
+  }
+}
+
+// Prefix along a strided axis: `n_line` cells per line, `stride` between them,
+// lines enumerated by (i0 in [0, n0), i1 in [0, n1)) with base offset i0 + i1*stride1.
+__global__ __launch_bounds__(256) void k_sat_strided(float2* __restrict__ sat, int n0, int n1, size_t stride1,
+                                                      int n_line, size_t stride) {
+  const int i0 = blockIdx.x * 256 + threadIdx.x;
+  const int i1 = blockIdx.y;
+  if (i0 >= n0 || i1 >= n1) return;
+  float2* p = sat + size_t(i0) + size_t(i1) * stride1;
+  df acc{0.f, 0.f};
+  int k = 0;
+  for (; k + 8 <= n_line; k += 8) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[size_t(k + u) * stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      acc = df_add(acc, df{v[u].x, v[u].y});
+      p[size_t(k + u) * stride] = make_float2(acc.hi, acc.lo);
+    }
+  }
+  for (; k < n_line; ++k) {
+    float2 v = p[size_t(k) * stride];
+    acc = df_add(acc, df{v.x, v.y});
+    p[size_t(k) * stride] = make_float2(acc.hi, acc.lo);
+  }
+}
+
+// Per raw value v: GetExtN(v / vmax) exactly as TransferFunction1D computes it on the
+// CPU (transferfunction1d.cpp:132-157, 189-197; transferfunction.h:79-82): lerp of the
+// FP64 table at v/vmax * max_density, rounded to float, then log(1/(1-alpha)) in
+// double, rounded to float.
+static void build_ext_lut(const cvr_ctx* c, std::vector<float>& lut) {
+  const int nvals = c->dtype == CVR_U8 ? 256 : 65536;
+  const double vmax = c->dtype == CVR_U8 ? 255.0 : 65535.0;
+  lut.resize(nvals);
+  const int md = c->tf_max_density;
+  for (int v = 0; v < nvals; ++v) {
+    double value = (double(v) / vmax) * (double(md) / 1.0);
+    float a;
+    if (value < 0.0f || value > float(md)) {
+      a = 0.f;
+    } else if (std::fabs(value - float(md)) < 0.000001) {
+      a = float(c->tf_tab[size_t(md) * 4 + 3]);
+    } else {
+      int i = int(value);
+      double t = value - i;
+      a = float((1.0 - t) * c->tf_tab[size_t(i) * 4 + 3] + t * c->tf_tab[size_t(i + 1) * 4 + 3]);
+    }
+    lut[v] = c->tf_ext_input ? a : float(std::log(1.0 / (1.0 - double(a))));
+  }
+}
+
+int build_sat(cvr_ctx* c, hipStream_t st) {
+  const uint64_t key = (c->vol_version << 32) ^ c->tf_version;
+  if (c->d_sat && c->sat_built_for == key) return 0;
+  const int w = c->nx + 2, h = c->ny + 2, d = c->nz + 2;
+  const size_t cells = size_t(w) * h * d;
+  if (!c->d_sat) {
+    if (hipMalloc(&c->d_sat, cells * sizeof(float2)) != hipSuccess) {
+      c->d_sat = nullptr;
+      set_error("cvr_prepare: out of device memory for the summed-area table");
+      return -5;
+    }
+  }
+  std::vector<float> lut;
+  build_ext_lut(c, lut);
+  if (c->d_ext_lut) CVR_HIP_CHECK(hipFree(c->d_ext_lut));
+  c->d_ext_lut = nullptr;
+  CVR_HIP_CHECK(hipMalloc(&c->d_ext_lut, lut.size() * sizeof(float)));
+  CVR_HIP_CHECK(hipMemcpyAsync(c->d_ext_lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  const int rows = h * d;
+  if (c->dtype == CVR_U8)
+    hipLaunchKernelGGL(k_sat_x<uint8_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
+                       static_cast<const uint8_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+  else
+    hipLaunchKernelGGL(k_sat_x<uint16_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
+                       static_cast<const uint16_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+  CVR_HIP_CHECK(hipGetLastError());
+  // y: lines over (x, z), stride w
+  hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, d), dim3(256), 0, st, c->d_sat, w, d, size_t(w) * h, h,
+                     size_t(w));
+  CVR_HIP_CHECK(hipGetLastError());
+  // z: lines over (x, y), stride w*h
+  hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, h), dim3(256), 0, st, c->d_sat, w, h, size_t(w), d,
+                     size_t(w) * h);
+  CVR_HIP_CHECK(hipGetLastError());
+  // the LUT is read by pass 1 only; keep it until the stream has consumed it
+  CVR_HIP_CHECK(hipStreamSynchronize(st));
+  c->sat_built_for = key;
+  return 0;
+}
+
+}  // namespace cvr

@@ -1,0 +1,214 @@
+"""ctypes binding of the C ABI in ``include/cvr.h`` (``libcvr.so``, built in-tree).
+
+The library is loaded from this package directory only; when it is missing the
+import fails loudly (there is no CPU fallback for the HIP path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libcvr.so"
+
+OK = 0
+ERR_INVALID = -1
+ERR_HIP = -2
+ERR_STATE = -3
+ERR_UNSUPPORTED = -4
+ERR_NOMEM = -5
+
+RC1P, EBS, CONE_GT, DOS, CONE_TRACE, CONE_NN = 0, 1, 2, 3, 4, 5
+U8, U16 = 0, 1
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("center", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy_deg", C.c_float), ("aspect", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("forward", C.c_float * 3), ("up", C.c_float * 3),
+                ("right", C.c_float * 3)]
+
+
+class Rc1pParams(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float)]
+
+
+class EbsParams(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
+                ("apply_shadow", C.c_int), ("ao_shells", C.c_int), ("ao_radius", C.c_float),
+                ("sdw_angle_deg", C.c_float), ("sdw_interval", C.c_float), ("sdw_initial", C.c_float),
+                ("sdw_weight", C.c_float), ("sdw_max_dist", C.c_float), ("shadow_type", C.c_int),
+                ("ka", C.c_float), ("kd", C.c_float)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("shaded_samples", C.c_uint64), ("last_ms", C.c_double)]
+
+
+PARAMS = {RC1P: Rc1pParams, EBS: EbsParams}
+
+# every symbol include/cvr.h and include/cvr_synth.h declare
+EXPORTS = [
+    "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
+    "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
+    "cvr_render", "cvr_get_sat", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_synth_blobs", "cvr_synth_noise",
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run `python -m cpp_volume_rendering_amd.build` "
+                          "(there is no CPU fallback for the HIP path)")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i, f, d, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
+    sig = {
+        "cvr_last_error": (C.c_char_p, []),
+        "cvr_technique_name": (C.c_char_p, [i]),
+        "cvr_create": (i, [i, i, C.POINTER(vp)]),
+        "cvr_destroy": (None, [vp]),
+        "cvr_set_volume": (i, [vp, i, i, i, i, f, f, f, vp, i]),
+        "cvr_set_transfer_function": (i, [vp, i, i, i, vp, i, vp]),
+        "cvr_default_params": (i, [i, vp, sz]),
+        "cvr_set_params": (i, [vp, vp, sz]),
+        "cvr_get_params": (i, [vp, vp, sz]),
+        "cvr_prepare": (i, [vp, vp]),
+        "cvr_render": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, vp, i, vp]),
+        "cvr_get_sat": (i, [vp, vp, sz]),
+        "cvr_get_stats": (i, [vp, C.POINTER(Stats)]),
+        "cvr_set_collect_stats": (i, [vp, i]),
+        "cvr_synth_blobs": (i, [i, i, i, i, i, d, i, C.c_uint64, vp]),
+        "cvr_synth_noise": (i, [i, i, i, i, i, C.c_uint64, i, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+class CvrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cvr error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise CvrError(rc, (lib().cvr_last_error() or b"").decode())
+
+
+def _ptr(a) -> int:
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor
+
+
+def synth_blobs(shape, dtype=U8, n_blobs=8, seed=1, noise_weight=0.0, noise_octaves=4) -> np.ndarray:
+    nx, ny, nz = shape
+    out = np.empty((nz, ny, nx), dtype=np.uint8 if dtype == U8 else np.uint16)
+    check(lib().cvr_synth_blobs(nx, ny, nz, dtype, n_blobs, noise_weight, noise_octaves, seed, out.ctypes.data))
+    return out
+
+
+def synth_noise(shape, dtype=U16, octaves=4, seed=2, bands=None) -> np.ndarray:
+    nx, ny, nz = shape
+    out = np.empty((nz, ny, nx), dtype=np.uint8 if dtype == U8 else np.uint16)
+    bv = None if bands is None else np.asarray(bands, dtype=np.float64)
+    check(lib().cvr_synth_noise(nx, ny, nz, dtype, octaves, seed, 0 if bv is None else len(bv),
+                                None if bv is None else bv.ctypes.data, out.ctypes.data))
+    return out
+
+
+class Renderer:
+    """One technique instance (a cvr_handle)."""
+
+    def __init__(self, technique: int, device: int = 0):
+        self.technique = technique
+        h = C.c_void_p()
+        check(lib().cvr_create(technique, device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().cvr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_volume(self, vol, scale=(1.0, 1.0, 1.0)):
+        """vol: numpy (nz, ny, nx) uint8/uint16 on the host, or a torch tensor on the device."""
+        if isinstance(vol, np.ndarray):
+            nz, ny, nx = vol.shape
+            dt = U8 if vol.dtype == np.uint8 else U16
+            vol = np.ascontiguousarray(vol)
+            check(lib().cvr_set_volume(self.h, nx, ny, nz, dt, *scale, vol.ctypes.data, 0))
+        else:
+            nz, ny, nx = vol.shape
+            dt = U8 if str(vol.dtype) == "torch.uint8" else U16
+            check(lib().cvr_set_volume(self.h, nx, ny, nz, dt, *scale, vol.data_ptr(), 1))
+
+    def set_transfer_function(self, rgb_iso, alpha_iso, max_density=255, extinction_input=False):
+        rgb = np.ascontiguousarray(rgb_iso, dtype=np.float64).reshape(-1, 4)
+        a = np.ascontiguousarray(alpha_iso, dtype=np.float64).reshape(-1, 2)
+        check(lib().cvr_set_transfer_function(self.h, max_density, int(extinction_input), len(rgb),
+                                              rgb.ctypes.data, len(a), a.ctypes.data))
+
+    def default_params(self):
+        p = PARAMS[self.technique]()
+        check(lib().cvr_default_params(self.technique, C.byref(p), C.sizeof(p)))
+        return p
+
+    def params(self):
+        p = PARAMS[self.technique]()
+        check(lib().cvr_get_params(self.h, C.byref(p), C.sizeof(p)))
+        return p
+
+    def set_params(self, p=None, **kw):
+        p = p if p is not None else self.params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        check(lib().cvr_set_params(self.h, C.byref(p), C.sizeof(p)))
+        return p
+
+    def prepare(self, stream=None):
+        check(lib().cvr_prepare(self.h, stream))
+
+    def render(self, cam: Camera, width: int, height: int, light: Light | None = None, out=None, stream=None):
+        """Render to a new host array (H, W, 4) float32, or into `out` (numpy or device torch tensor)."""
+        if out is None:
+            out = np.zeros((height, width, 4), dtype=np.float32)
+        is_dev = not isinstance(out, np.ndarray)
+        check(lib().cvr_render(self.h, C.byref(cam), C.byref(light) if light is not None else None, width, height,
+                               _ptr(out), int(is_dev), stream))
+        return out
+
+    def get_sat(self, shape) -> np.ndarray:
+        nx, ny, nz = shape
+        out = np.empty((nz + 2, ny + 2, nx + 2), dtype=np.float64)
+        check(lib().cvr_get_sat(self.h, out.ctypes.data, out.size))
+        return out
+
+    def collect_stats(self, on=True):
+        check(lib().cvr_set_collect_stats(self.h, int(on)))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().cvr_get_stats(self.h, C.byref(s)))
+        return s
