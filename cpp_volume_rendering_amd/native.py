@@ -71,6 +71,15 @@ def lib() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run `python -m cpp_volume_rendering_amd.build` "
                           "(there is no CPU fallback for the HIP path)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7) and loads it under a different file name, so a process that
+    # loads libcvr first and torch later would hold two runtimes, and streams or
+    # device pointers would not cross between them.  Loading torch first lets libcvr
+    # bind to the runtime already present by soname.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     vp, i, f, d, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
     sig = {
