@@ -67,8 +67,34 @@ static size_t params_size(int technique) {
   switch (technique) {
     case CVR_RC1P: return sizeof(cvr_rc1p_params);
     case CVR_EBS: return sizeof(cvr_ebs_params);
+    case CVR_CONE_TRACE: return sizeof(cvr_cone_params);
+    case CVR_CONE_NN: return sizeof(cvr_nn_params);
     default: return 0;
   }
+}
+
+// Defaults of the cone-traced configuration: 8 occlusion cones of 30 degrees (1 axial,
+// 7 on a ring at 60), one 10-degree shadow cone, 64 steps each, a 6-level pyramid at
+// the volume's resolution with sigma0 = 1 (extcoefvolumegenerator.cpp:10-15 uses 1.0).
+static void cone_defaults(cvr_cone_params* c) {
+  c->step_size = 0.f;
+  c->et_threshold = 0.99f;
+  c->apply_occlusion = 1;
+  c->apply_shadow = 1;
+  c->n_levels = 6;
+  c->base_res[0] = c->base_res[1] = c->base_res[2] = 0;
+  c->sigma0 = 1.0f;
+  c->n_steps = 64;
+  c->n_occ_cones = 8;
+  c->occ_half_deg = 30.f;
+  c->ring_deg = 60.f;
+  c->sdw_half_deg = 10.f;
+  c->t0 = 1.0f;
+  c->occ_max_dist = 0.f;
+  c->sdw_max_dist = 0.f;
+  c->r_sigma = 2.0f;
+  c->ka = 0.5f;
+  c->kd = 0.5f;
 }
 
 static void defaults(int technique, void* p) {
@@ -92,6 +118,21 @@ static void defaults(int technique, void* p) {
     e->shadow_type = 0;
     e->ka = 0.5f;
     e->kd = 0.5f;
+  } else if (technique == CVR_CONE_TRACE) {
+    cone_defaults(static_cast<cvr_cone_params*>(p));
+  } else if (technique == CVR_CONE_NN) {
+    cvr_nn_params* n = static_cast<cvr_nn_params*>(p);
+    cone_defaults(&n->cone);
+    n->cone.n_levels = 7;
+    n->n_train = 1 << 22;
+    n->n_val = 1 << 16;
+    n->epochs = 4;
+    n->batch = 16384;
+    n->lr = 2e-3f;
+    n->rmse_gate = 0.05f;
+    n->ood_margin = 0.02f;
+    n->force_exact = 0;
+    n->seed = 4;
   }
 }
 
@@ -103,6 +144,11 @@ static void free_all(cvr_ctx* c) {
   if (c->d_ext_lut) (void)hipFree(c->d_ext_lut);
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_pyr) (void)hipFree(c->d_pyr);
+  if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
+  for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
+                  static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
+    if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
 }
@@ -130,6 +176,8 @@ const char* cvr_technique_name(int technique) {
   switch (technique) {
     case CVR_RC1P: return "1-Pass - Ray Casting";
     case CVR_EBS: return "1-Pass - Ray Casting - Extinction-based";
+    case CVR_CONE_TRACE: return "1-Pass - Ray Casting - Cone-traced Directional Occlusion";
+    case CVR_CONE_NN: return "1-Pass - Ray Casting - Learned Cone Occlusion";
     default: return nullptr;
   }
 }
@@ -150,6 +198,8 @@ int cvr_create(int technique, int device, cvr_handle* out) {
   c->device = device;
   defaults(CVR_RC1P, &c->rc1p);
   defaults(CVR_EBS, &c->ebs);
+  defaults(CVR_CONE_TRACE, &c->cone);
+  defaults(CVR_CONE_NN, &c->nn);
   if (hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     free_all(c);
@@ -197,6 +247,7 @@ int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, fl
   h->nx = nx; h->ny = ny; h->nz = nz; h->dtype = dtype;
   h->sx = sx; h->sy = sy; h->sz = sz;
   h->vol_version++;
+  h->nn_trained = false;
   return CVR_OK;
 }
 
@@ -270,6 +321,7 @@ int cvr_set_transfer_function(cvr_handle h, int max_density, int extinction_inpu
   h->tf_ext_input = extinction_input != 0;
   h->tf_tab.swap(tab);
   h->tf_version++;
+  h->nn_trained = false;
   return CVR_OK;
 }
 
@@ -285,6 +337,8 @@ static void* live_params(cvr_ctx* c) {
   switch (c->technique) {
     case CVR_RC1P: return &c->rc1p;
     case CVR_EBS: return &c->ebs;
+    case CVR_CONE_TRACE: return &c->cone;
+    case CVR_CONE_NN: return &c->nn;
     default: return nullptr;
   }
 }
@@ -298,6 +352,21 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
       set_error("cvr_set_params: et_threshold must be > 0");
       return CVR_ERR_INVALID;
     }
+  }
+  if (h->technique == CVR_CONE_TRACE || h->technique == CVR_CONE_NN) {
+    const cvr_cone_params& cp = h->technique == CVR_CONE_TRACE ? *static_cast<const cvr_cone_params*>(params)
+                                                                : static_cast<const cvr_nn_params*>(params)->cone;
+    int rc = check_cone_params(cp);
+    if (rc) return rc;
+  }
+  if (h->technique == CVR_CONE_NN) {
+    // training inputs changed: the network is retrained on the next prepare/render
+    // (the gate, the range margin and force_exact apply at render time only)
+    const cvr_nn_params* n = static_cast<const cvr_nn_params*>(params);
+    const cvr_nn_params& o = h->nn;
+    if (std::memcmp(&n->cone, &o.cone, sizeof(cvr_cone_params)) != 0 || n->n_train != o.n_train ||
+        n->n_val != o.n_val || n->epochs != o.epochs || n->batch != o.batch || n->lr != o.lr || n->seed != o.seed)
+      h->nn_trained = false;
   }
   std::memcpy(live_params(h), params, bytes);
   return CVR_OK;
@@ -323,6 +392,12 @@ int cvr_prepare(cvr_handle h, void* hip_stream) {
   DeviceGuard dg(h->device);
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
   if (h->technique == CVR_EBS) return build_sat(h, st);
+  if (h->technique == CVR_CONE_TRACE) return prepare_cone(h, h->cone, st);
+  if (h->technique == CVR_CONE_NN) {
+    rc = prepare_cone(h, h->nn.cone, st);
+    if (!rc && !h->nn_trained) rc = nn_train(h, st);
+    return rc;
+  }
   return CVR_OK;
 }
 
@@ -356,6 +431,8 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
   switch (h->technique) {
     case CVR_RC1P: rc = launch_rc1p(h, rf, dst, st); break;
     case CVR_EBS: rc = launch_ebs(h, rf, light, dst, st); break;
+    case CVR_CONE_TRACE: rc = launch_cone(h, rf, light, dst, st); break;
+    case CVR_CONE_NN: rc = launch_nn(h, rf, light, dst, st); break;
     default: set_error("cvr_render: technique not available"); rc = CVR_ERR_UNSUPPORTED;
   }
   if (rc) return rc;
@@ -370,6 +447,7 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
     CVR_HIP_CHECK(hipStreamSynchronize(st));
     h->stats.samples = cnt[0];
     h->stats.shaded_samples = cnt[1];
+    h->stats.fallback_samples = cnt[2];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_ms = ms;
   }
@@ -389,6 +467,171 @@ int cvr_get_sat(cvr_handle h, double* out_host, size_t count) {
   std::vector<float2> tmp(cells);
   CVR_HIP_CHECK(hipMemcpy(tmp.data(), h->d_sat, cells * sizeof(float2), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < cells; ++i) out_host[i] = double(tmp[i].x) + double(tmp[i].y);
+  return CVR_OK;
+}
+
+static const cvr_cone_params* cone_of(cvr_ctx* h) {
+  if (h->technique == CVR_CONE_TRACE) return &h->cone;
+  if (h->technique == CVR_CONE_NN) return &h->nn.cone;
+  return nullptr;
+}
+
+int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels) {
+  if (!h || !levels) { set_error("cvr_pyramid_info: null argument"); return CVR_ERR_INVALID; }
+  const cvr_cone_params* cp = cone_of(h);
+  if (!cp) { set_error("cvr_pyramid_info: technique has no pyramid"); return CVR_ERR_UNSUPPORTED; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  rc = prepare_cone(h, *cp, nullptr);
+  if (rc) return rc;
+  *levels = h->pyr.levels;
+  if (dims)
+    for (int l = 0; l < std::min(max_levels, h->pyr.levels); ++l)
+      for (int k = 0; k < 3; ++k) dims[3 * l + k] = h->pyr.dims[l][k];
+  return CVR_OK;
+}
+
+int cvr_get_pyramid_level(cvr_handle h, int level, float* out_host, size_t count) {
+  if (!h || !out_host) { set_error("cvr_get_pyramid_level: null argument"); return CVR_ERR_INVALID; }
+  const cvr_cone_params* cp = cone_of(h);
+  if (!cp) { set_error("cvr_get_pyramid_level: technique has no pyramid"); return CVR_ERR_UNSUPPORTED; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  rc = prepare_cone(h, *cp, nullptr);
+  if (rc) return rc;
+  if (level < 0 || level >= h->pyr.levels) { set_error("cvr_get_pyramid_level: level out of range"); return CVR_ERR_INVALID; }
+  const size_t n = size_t(h->pyr.dims[level][0]) * h->pyr.dims[level][1] * h->pyr.dims[level][2];
+  if (count != n) { set_error("cvr_get_pyramid_level: count must be the level's voxel count"); return CVR_ERR_INVALID; }
+  CVR_HIP_CHECK(hipMemcpy(out_host, h->d_pyr + h->pyr.off[level], n * sizeof(float), hipMemcpyDeviceToHost));
+  return CVR_OK;
+}
+
+// Temporary device copies for host-array entry points.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int cvr_cone_eval(cvr_handle h, int n, const float* pos, const float* dirs, const float* lights, float* occ,
+                  float* sdw, float* feat) {
+  if (!h || !pos || !dirs || !lights || n < 0) { set_error("cvr_cone_eval: null argument"); return CVR_ERR_INVALID; }
+  const cvr_cone_params* cp = cone_of(h);
+  if (!cp) { set_error("cvr_cone_eval: technique has no cone tracer"); return CVR_ERR_UNSUPPORTED; }
+  int rc = check_inputs(h);
+  if (rc || n == 0) return rc;
+  DeviceGuard dg(h->device);
+  DevBuf in, o;
+  const size_t n3 = size_t(n) * 3;
+  CVR_HIP_CHECK(hipMalloc(&in.p, 3 * n3 * sizeof(float)));
+  CVR_HIP_CHECK(hipMalloc(&o.p, size_t(n) * 18 * sizeof(float)));
+  float* dp = static_cast<float*>(in.p);
+  float* dout = static_cast<float*>(o.p);
+  CVR_HIP_CHECK(hipMemcpy(dp, pos, n3 * 4, hipMemcpyHostToDevice));
+  CVR_HIP_CHECK(hipMemcpy(dp + n3, dirs, n3 * 4, hipMemcpyHostToDevice));
+  CVR_HIP_CHECK(hipMemcpy(dp + 2 * n3, lights, n3 * 4, hipMemcpyHostToDevice));
+  rc = cone_points(h, *cp, n, dp, dp + n3, dp + 2 * n3, dout, dout + n, dout + 2 * size_t(n), nullptr);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  if (occ) CVR_HIP_CHECK(hipMemcpy(occ, dout, size_t(n) * 4, hipMemcpyDeviceToHost));
+  if (sdw) CVR_HIP_CHECK(hipMemcpy(sdw, dout + n, size_t(n) * 4, hipMemcpyDeviceToHost));
+  if (feat) CVR_HIP_CHECK(hipMemcpy(feat, dout + 2 * size_t(n), size_t(n) * 16 * 4, hipMemcpyDeviceToHost));
+  return CVR_OK;
+}
+
+int cvr_nn_param_count(void) { return nn_param_count(); }
+
+static int nn_check(cvr_ctx* h) {
+  if (!h) { set_error("surrogate: null handle"); return CVR_ERR_INVALID; }
+  if (h->technique != CVR_CONE_NN) { set_error("surrogate: technique is not CVR_CONE_NN"); return CVR_ERR_UNSUPPORTED; }
+  return CVR_OK;
+}
+
+int cvr_nn_train(cvr_handle h, void* hip_stream) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  rc = prepare_cone(h, h->nn.cone, st);
+  if (!rc) rc = nn_train(h, st);
+  if (!rc) CVR_HIP_CHECK(hipStreamSynchronize(st));
+  return rc;
+}
+
+int cvr_nn_get_weights(cvr_handle h, float* out, size_t count) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!out || count != size_t(nn_param_count())) { set_error("cvr_nn_get_weights: count must be cvr_nn_param_count()"); return CVR_ERR_INVALID; }
+  if (!h->d_nn_w) { set_error("cvr_nn_get_weights: no network yet"); return CVR_ERR_STATE; }
+  DeviceGuard dg(h->device);
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  CVR_HIP_CHECK(hipMemcpy(out, h->d_nn_w, count * 4, hipMemcpyDeviceToHost));
+  return CVR_OK;
+}
+
+int cvr_nn_set_weights(cvr_handle h, const float* in, size_t count) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!in || count != size_t(nn_param_count())) { set_error("cvr_nn_set_weights: count must be cvr_nn_param_count()"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  rc = nn_alloc(h);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipMemcpy(h->d_nn_w, in, count * 4, hipMemcpyHostToDevice));
+  rc = nn_pack(h, nullptr);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  // no range information: every input is in range, and the gate is open
+  for (int i = 0; i < 16; ++i) { h->nn_lo[i] = -1e30f; h->nn_hi[i] = 1e30f; }
+  h->nn_report = cvr_nn_report{};
+  h->nn_trained = true;
+  return CVR_OK;
+}
+
+int cvr_nn_eval(cvr_handle h, int n, const float* X, float* Y, int path) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!X || !Y || n < 0) { set_error("cvr_nn_eval: null argument"); return CVR_ERR_INVALID; }
+  if (!h->d_nn_w) { set_error("cvr_nn_eval: no network yet"); return CVR_ERR_STATE; }
+  if (n == 0) return CVR_OK;
+  DeviceGuard dg(h->device);
+  DevBuf dx, dy;
+  CVR_HIP_CHECK(hipMalloc(&dx.p, size_t(n) * 16 * 4));
+  CVR_HIP_CHECK(hipMalloc(&dy.p, size_t(n) * 2 * 4));
+  CVR_HIP_CHECK(hipMemcpy(dx.p, X, size_t(n) * 16 * 4, hipMemcpyHostToDevice));
+  rc = nn_eval(h, static_cast<float*>(dx.p), n, static_cast<float*>(dy.p), path, nullptr);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  CVR_HIP_CHECK(hipMemcpy(Y, dy.p, size_t(n) * 2 * 4, hipMemcpyDeviceToHost));
+  return CVR_OK;
+}
+
+int cvr_nn_gradient(cvr_handle h, int n, const float* X, const float* T, float* grad, float* loss) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!X || !T || !grad || n < 1) { set_error("cvr_nn_gradient: null argument"); return CVR_ERR_INVALID; }
+  if (!h->d_nn_w) { set_error("cvr_nn_gradient: no network yet"); return CVR_ERR_STATE; }
+  DeviceGuard dg(h->device);
+  DevBuf dx, dt;
+  CVR_HIP_CHECK(hipMalloc(&dx.p, size_t(n) * 16 * 4));
+  CVR_HIP_CHECK(hipMalloc(&dt.p, size_t(n) * 2 * 4));
+  CVR_HIP_CHECK(hipMemcpy(dx.p, X, size_t(n) * 16 * 4, hipMemcpyHostToDevice));
+  CVR_HIP_CHECK(hipMemcpy(dt.p, T, size_t(n) * 2 * 4, hipMemcpyHostToDevice));
+  rc = nn_gradient(h, static_cast<float*>(dx.p), static_cast<float*>(dt.p), n, nullptr);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  CVR_HIP_CHECK(hipMemcpy(grad, h->d_nn_g, size_t(nn_param_count()) * 4, hipMemcpyDeviceToHost));
+  if (loss) CVR_HIP_CHECK(hipMemcpy(loss, h->d_nn_scalar, 4, hipMemcpyDeviceToHost));
+  return CVR_OK;
+}
+
+int cvr_nn_get_report(cvr_handle h, cvr_nn_report* out) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!out) { set_error("cvr_nn_get_report: null argument"); return CVR_ERR_INVALID; }
+  *out = h->nn_report;
   return CVR_OK;
 }
 

@@ -1,0 +1,119 @@
+// Device helpers of the cone-traced lighting: pyramid lookups, cone transmittance,
+// directional occlusion / shadow, and the surrogate's input features.
+#pragma once
+#include "cvr_ctx.hpp"
+
+namespace cvr {
+
+// One step of a cone march: midpoint distance, segment length, pyramid level.
+struct ConeStepF {
+  float tm, dt, lod, pad;
+};
+// One stencil point of the surrogate's features: distance along the axis, segment
+// length, pyramid level.
+struct StencilF {
+  float dist, seg, lod, pad;
+};
+
+constexpr int kNFeat = 16, kNStencil = 7, kMaxSteps = 256;
+
+// Per-frame cone-tracer constants.
+struct ConeK {
+  int n_steps, n_occ, apply_occ, apply_sdw;
+  float ring_cos, ring_sin, ka, kd;
+  float L[3];               // shadow cone axis (towards the light)
+  const ConeStepF* so;      // occlusion schedule [n_steps]
+  const ConeStepF* ss;      // shadow schedule [n_steps]
+  const StencilF* st;       // surrogate stencil: occlusion [7] then shadow [7]
+};
+
+// GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w).
+__device__ __forceinline__ float pyr_tri(const float* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
+                                         float w) {
+  const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
+  const float* __restrict__ a = pyr + pg.off[l];
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  lin_axis(u * float(nx) - 0.5f, nx, x0, x1, fx);
+  lin_axis(v * float(ny) - 0.5f, ny, y0, y1, fy);
+  lin_axis(w * float(nz) - 0.5f, nz, z0, z1, fz);
+  const int sy = nx, sz = nx * ny;
+  const int r00 = y0 * sy + z0 * sz, r10 = y1 * sy + z0 * sz, r01 = y0 * sy + z1 * sz, r11 = y1 * sy + z1 * sz;
+  const float a000 = a[r00 + x0], a100 = a[r00 + x1], a010 = a[r10 + x0], a110 = a[r10 + x1];
+  const float a001 = a[r01 + x0], a101 = a[r01 + x1], a011 = a[r11 + x0], a111 = a[r11 + x1];
+  const float c00 = a000 + fx * (a100 - a000), c10 = a010 + fx * (a110 - a010);
+  const float c01 = a001 + fx * (a101 - a001), c11 = a011 + fx * (a111 - a011);
+  const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+  return c0 + fz * (c1 - c0);
+}
+
+// textureLod with LINEAR_MIPMAP_LINEAR, then the border falloff of
+// GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:92-112).
+__device__ __forceinline__ float gauss_ext(const float* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
+  const float q = float(pg.levels - 1);
+  lod = fminf(fmaxf(lod, 0.f), q);
+  const int d1 = int(floorf(lod));
+  const float fr = lod - float(d1);
+  const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
+  float rg = pyr_tri(pyr, pg, d1, u, v, w);
+  if (fr > 0.f && d1 + 1 < pg.levels) rg = rg * (1.f - fr) + pyr_tri(pyr, pg, d1 + 1, u, v, w) * fr;
+  if (tp.x < 0.f || tp.x > pg.G[0] || tp.y < 0.f || tp.y > pg.G[1] || tp.z < 0.f || tp.z > pg.G[2]) {
+    const float sg = exp2f(lod);
+    const float cx = fminf(fmaxf(tp.x, 0.f), pg.G[0]) - tp.x;
+    const float cy = fminf(fmaxf(tp.y, 0.f), pg.G[1]) - tp.y;
+    const float cz = fminf(fmaxf(tp.z, 0.f), pg.G[2]) - tp.z;
+    rg = rg * __expf(-(cx * cx + cy * cy + cz * cz) / (2.0f * sg * sg));
+  }
+  return rg;
+}
+
+__device__ __forceinline__ float cone_T(const float* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
+                                        int n, f3 p, f3 axis) {
+  float od = 0.f;
+  for (int j = 0; j < n; ++j) {
+    const ConeStepF c = s[j];
+    od += gauss_ext(pyr, pg, p + axis * c.tm, c.lod) * c.dt;
+  }
+  return __expf(-od);
+}
+
+// Occlusion frame of a camera ray (gt_ray_marching.comp:396-399).
+__device__ __forceinline__ void occ_frame(f3 dir, f3& k, f3& up, f3& right) {
+  right = norm3(cross3(dir, mk3(0.f, 1.f, 0.f)));
+  up = norm3(cross3(mk3(-dir.x, -dir.y, -dir.z), right));
+  k = mk3(-dir.x, -dir.y, -dir.z);
+}
+
+// 1 axial cone + (n_occ-1) cones whose axes sit on a ring at ring angle from the axis,
+// cosine-weighted like gt_ray_marching.comp:161-168.
+__device__ __forceinline__ float ct_occlusion(const float* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
+                                              f3 k, f3 up, f3 right) {
+  float num = cone_T(pyr, pg, K.so, K.n_steps, p, k), den = 1.f;
+  const int nr = K.n_occ - 1;
+  for (int j = 0; j < nr; ++j) {
+    float sphi, cphi;
+    __sincosf(6.283185307179586f * float(j) / float(nr), &sphi, &cphi);
+    const f3 ax = k * K.ring_cos + (up * cphi + right * sphi) * K.ring_sin;
+    num += K.ring_cos * cone_T(pyr, pg, K.so, K.n_steps, p, ax);
+    den += K.ring_cos;
+  }
+  return num / den;
+}
+
+// The surrogate's 16 inputs (see the CPU oracle's nn_features): 7 segment opacities
+// along the occlusion axis, 7 along the light, the light/view cosine, the local opacity.
+__device__ __forceinline__ void nn_feats(const float* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
+                                         f3 p, f3 k, f3 L, float* f) {
+#pragma unroll
+  for (int i = 0; i < kNStencil; ++i) {
+    const StencilF so = st[i], ss = st[kNStencil + i];
+    const float to = gauss_ext(pyr, pg, p + k * so.dist, so.lod);
+    f[i] = 1.f - __expf(-to * so.seg);
+    const float ts = gauss_ext(pyr, pg, p + L * ss.dist, ss.lod);
+    f[kNStencil + i] = 1.f - __expf(-ts * ss.seg);
+  }
+  f[14] = dot3(k, L) * 0.5f + 0.5f;
+  f[15] = 1.f - __expf(-gauss_ext(pyr, pg, p, 0.f));
+}
+
+}  // namespace cvr

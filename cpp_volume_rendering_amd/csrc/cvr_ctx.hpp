@@ -4,6 +4,17 @@
 #include "../../include/cvr.h"
 #include <vector>
 
+namespace cvr {
+constexpr int kMaxLevels = 12;
+// Mip chain of the extinction pyramid: level dims and offsets into one fp32 buffer.
+struct PyrGeom {
+  int levels;
+  int dims[kMaxLevels][3];
+  size_t off[kMaxLevels];
+  float G[3];
+};
+}  // namespace cvr
+
 struct cvr_ctx {
   int technique = 0;
   int device = 0;
@@ -25,11 +36,27 @@ struct cvr_ctx {
   // parameters (one of these is live, chosen by `technique`)
   cvr_rc1p_params rc1p{};
   cvr_ebs_params ebs{};
+  cvr_cone_params cone{};
+  cvr_nn_params nn{};
 
   // EBS summed-area table, two-float (hi, lo) per cell
   float2* d_sat = nullptr;
   float* d_ext_lut = nullptr;  // per raw value extinction (GetExtN), 256 or 65536 entries
   uint64_t sat_built_for = ~0ull;
+
+  // Gaussian extinction pyramid (cone tracers, surrogate)
+  float* d_pyr = nullptr;
+  cvr::PyrGeom pyr{};
+  uint64_t pyr_built_for = ~0ull;
+  void* d_cone_tab = nullptr;  // cone step schedules + surrogate stencil
+  int cone_n_steps = 0;
+
+  // surrogate network: fp32 master weights, gradient, Adam moments, packed bf16 copy
+  float *d_nn_w = nullptr, *d_nn_g = nullptr, *d_nn_m = nullptr, *d_nn_v = nullptr, *d_nn_scalar = nullptr;
+  void* d_nn_pack = nullptr;
+  float nn_lo[16] = {0}, nn_hi[16] = {0};
+  bool nn_trained = false;
+  cvr_nn_report nn_report{};
 
   // stats
   bool collect_stats = false;
@@ -52,4 +79,17 @@ float volume_diagonal(const cvr_ctx* c);
 int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st);
 int build_sat(cvr_ctx* c, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, hipStream_t st);
+int check_cone_params(const cvr_cone_params& P);
+int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);
+int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos, const float* d_dirs,
+                const float* d_lights, float* d_occ, float* d_sdw, float* d_feat, hipStream_t st);
+int nn_alloc(cvr_ctx* c);
+int nn_pack(cvr_ctx* c, hipStream_t st);
+int nn_train(cvr_ctx* c, hipStream_t st);
+int nn_gradient(cvr_ctx* c, const float* d_X, const float* d_T, int n, hipStream_t st);
+int nn_eval(cvr_ctx* c, const float* d_X, int n, float* d_Y, int packed, hipStream_t st);
+int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+int nn_param_count();
 }  // namespace cvr

@@ -1,0 +1,790 @@
+// Learned occlusion/shadow surrogate: bf16 MFMA training and inference, gfx950.
+//
+// Network: 16 -> 64 -> 64 -> 64 -> 16 (outputs 0, 1 = ambient occlusion, shadow
+// transmittance), ReLU on the hidden layers, bf16 operands, fp32 accumulation.
+// Parameters (fp32 master copy) are flattened as W1[64][16], b1[64], W2[64][64], b2,
+// W3[64][64], b3, W4[16][64], b4[16], row = output feature.
+//
+// MFMA: v_mfma_f32_16x16x16_bf16.  Lane l (r = l%16, q = l/16) holds A[r][4q+e],
+// B[4q+e][r] and D[4q+e][r], e = 0..3.  Activations are kept feature-major
+// (features x samples), so a layer's D tile is, after bias + ReLU + bf16 rounding,
+// exactly the B operand of the next layer: inference keeps all activations in
+// registers; only the input features pass through LDS once.
+//
+// Training (k_mlp_train): one wavefront per workgroup walks `chunks` groups of 64
+// samples, keeps the activations of one group in LDS, runs the backward GEMMs
+// (W^T dY and dY H^T, operands gathered from LDS with transposed strides) and
+// accumulates the weight gradients in LDS; one atomic flush per workgroup.
+#include "cvr_cone.hpp"
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace cvr {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kH = 64, kOut = 16;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kNFeat;
+constexpr int kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kH;
+constexpr int kOffW4 = kOffB3 + kH, kOffB4 = kOffW4 + kOut * kH;
+constexpr int kNParams = kOffB4 + kOut;  // 10448
+// Packed inference weights: 40 A-operand tiles of 64 lanes x 4 bf16.
+constexpr int kTiles1 = 4, kTiles2 = 16, kTiles3 = 16, kTiles4 = 4;
+constexpr int kNTiles = kTiles1 + kTiles2 + kTiles3 + kTiles4;
+constexpr int kPackShorts = kNTiles * 64 * 4;
+constexpr int kNBias = kH + kH + kH + kOut;
+
+// fp32 -> bf16, round to nearest even (inputs are finite)
+__device__ __forceinline__ short f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(uint32_t(uint16_t(b)) << 16); }
+
+__device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Tile (m0.., n0..) += sum_k A(m,k) B(k,n) with A(m,k) = a[m*am + k*ak], B(k,n) = b[k*bk + n*bn].
+__device__ __forceinline__ f32x4_t tile_gemm(const float* a, int am, int ak, const float* b, int bk, int bn, int m0,
+                                             int n0, int K, f32x4_t acc) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x4_t av, bv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * q + e;
+      av[e] = f2bf(a[(m0 + r) * am + k * ak]);
+      bv[e] = f2bf(b[k * bk + (n0 + r) * bn]);
+    }
+    acc = mfma16(av, bv, acc);
+  }
+  return acc;
+}
+
+// Forward of one layer for 64 samples held feature-major in LDS.
+__device__ void layer_fwd(const float* __restrict__ W, const float* __restrict__ bias, int in, int out,
+                          const float* aIn, float* aOut, bool relu) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  for (int mt = 0; mt < out / 16; ++mt)
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t acc;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = bias[mt * 16 + 4 * q + e];
+      acc = tile_gemm(W, in, 1, aIn, 64, 1, mt * 16, nt * 16, in, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float v = acc[e];
+        aOut[(mt * 16 + 4 * q + e) * 64 + nt * 16 + r] = relu ? fmaxf(v, 0.f) : v;
+      }
+    }
+}
+
+// Gradients of one layer: dW += G_out H_in^T, db += row sums of G_out, and (if gIn)
+// gIn = (W^T G_out) masked by H_in > 0.
+__device__ void layer_bwd(const float* __restrict__ W, int in, int out, const float* gOut, const float* hIn,
+                          float* gIn, float* sGradW, float* sGradB) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  if (lane < out) {
+    float s = 0.f;
+    for (int j = 0; j < 64; ++j) s += gOut[lane * 64 + j];
+    sGradB[lane] += s;
+  }
+  for (int mt = 0; mt < out / 16; ++mt)
+    for (int nt = 0; nt < in / 16; ++nt) {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = tile_gemm(gOut, 64, 1, hIn, 1, 64, mt * 16, nt * 16, 64, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sGradW[(mt * 16 + 4 * q + e) * in + nt * 16 + r] += acc[e];
+    }
+  if (!gIn) return;
+  for (int mt = 0; mt < in / 16; ++mt)
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = tile_gemm(W, 1, in, gOut, 64, 1, mt * 16, nt * 16, out, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = mt * 16 + 4 * q + e, s = nt * 16 + r;
+        gIn[i * 64 + s] = hIn[i * 64 + s] > 0.f ? acc[e] : 0.f;
+      }
+    }
+}
+
+// X: [n][16] features, T: [n][2] targets.  grad (global, zeroed by the caller) +=
+// d/dparams of sum (y - t)^2 / (2 * norm_n); loss (global) += that sum.
+__global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ prm, const float* __restrict__ X,
+                                                  const float* __restrict__ Tg, int n, int chunks, float norm_n,
+                                                  float* __restrict__ grad, float* __restrict__ loss) {
+  extern __shared__ float lds[];
+  float* sX = lds;               // 16 x 64
+  float* sH1 = sX + 16 * 64;     // 64 x 64
+  float* sH2 = sH1 + 64 * 64;
+  float* sH3 = sH2 + 64 * 64;
+  float* sY = sH3 + 64 * 64;     // 16 x 64
+  float* sG1 = sY + 16 * 64;     // 64 x 64
+  float* sG2 = sG1 + 64 * 64;    // 64 x 64
+  float* sGrad = sG2 + 64 * 64;  // kNParams
+  const int lane = threadIdx.x;
+  for (int i = lane; i < kNParams; i += 64) sGrad[i] = 0.f;
+  float lsum = 0.f;
+  const float inv_n = 1.0f / norm_n;
+  for (int c = 0; c < chunks; ++c) {
+    const int s0 = (blockIdx.x * chunks + c) * 64;
+    if (s0 >= n) break;
+    __syncthreads();
+    const int s = s0 + lane;
+    const bool valid = s < n;
+    for (int f = 0; f < kNFeat; ++f) sX[f * 64 + lane] = valid ? X[size_t(s) * kNFeat + f] : 0.f;
+    __syncthreads();
+    layer_fwd(prm + kOffW1, prm + kOffB1, kNFeat, kH, sX, sH1, true);
+    __syncthreads();
+    layer_fwd(prm + kOffW2, prm + kOffB2, kH, kH, sH1, sH2, true);
+    __syncthreads();
+    layer_fwd(prm + kOffW3, prm + kOffB3, kH, kH, sH2, sH3, true);
+    __syncthreads();
+    layer_fwd(prm + kOffW4, prm + kOffB4, kH, kOut, sH3, sY, false);
+    __syncthreads();
+    // dL/dy for the 2 used outputs of valid samples
+    for (int o = 0; o < kOut; ++o) {
+      float g = 0.f;
+      if (o < 2 && valid) {
+        const float e = sY[o * 64 + lane] - Tg[size_t(s) * 2 + o];
+        lsum += e * e;
+        g = e * inv_n;
+      }
+      sG1[o * 64 + lane] = g;
+    }
+    __syncthreads();
+    layer_bwd(prm + kOffW4, kH, kOut, sG1, sH3, sG2, sGrad + kOffW4, sGrad + kOffB4);
+    __syncthreads();
+    layer_bwd(prm + kOffW3, kH, kH, sG2, sH2, sG1, sGrad + kOffW3, sGrad + kOffB3);
+    __syncthreads();
+    layer_bwd(prm + kOffW2, kH, kH, sG1, sH1, sG2, sGrad + kOffW2, sGrad + kOffB2);
+    __syncthreads();
+    layer_bwd(prm + kOffW1, kNFeat, kH, sG2, sX, nullptr, sGrad + kOffW1, sGrad + kOffB1);
+  }
+  __syncthreads();
+  for (int i = lane; i < kNParams; i += 64) atomicAdd(&grad[i], sGrad[i]);
+  // wave sum of the squared errors
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, 64);
+  if (lane == 0) atomicAdd(loss, lsum * 0.5f * inv_n);
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master parameters; clears the gradient.
+__global__ void k_adam(float* __restrict__ prm, float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  prm[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  grad[i] = 0.f;
+}
+
+__global__ __launch_bounds__(64) void k_mlp_eval(const float* __restrict__ prm, const float* __restrict__ X, int n,
+                                                 float* __restrict__ Y) {
+  __shared__ float sX[16 * 64], sH1[64 * 64], sH2[64 * 64], sY[16 * 64];
+  const int lane = threadIdx.x;
+  const int s = blockIdx.x * 64 + lane;
+  const bool valid = s < n;
+  for (int f = 0; f < kNFeat; ++f) sX[f * 64 + lane] = valid ? X[size_t(s) * kNFeat + f] : 0.f;
+  __syncthreads();
+  layer_fwd(prm + kOffW1, prm + kOffB1, kNFeat, kH, sX, sH1, true);
+  __syncthreads();
+  layer_fwd(prm + kOffW2, prm + kOffB2, kH, kH, sH1, sH2, true);
+  __syncthreads();
+  layer_fwd(prm + kOffW3, prm + kOffB3, kH, kH, sH2, sH1, true);
+  __syncthreads();
+  layer_fwd(prm + kOffW4, prm + kOffB4, kH, kOut, sH1, sY, false);
+  __syncthreads();
+  if (valid) {
+    Y[size_t(s) * 2] = sY[lane];
+    Y[size_t(s) * 2 + 1] = sY[64 + lane];
+  }
+}
+
+// Packs the fp32 master weights into bf16 A-operand tiles (tile-major, lane-major,
+// 4 consecutive k) and the biases into fp32.
+__global__ void k_mlp_pack(const float* __restrict__ prm, short* __restrict__ wpack, float* __restrict__ bpack) {
+  const int t = blockIdx.x;  // tile
+  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  int off, in, mt, kt;
+  if (t < kTiles1) { off = kOffW1; in = kNFeat; mt = t; kt = 0; }
+  else if (t < kTiles1 + kTiles2) { off = kOffW2; in = kH; mt = (t - kTiles1) / 4; kt = (t - kTiles1) % 4; }
+  else if (t < kTiles1 + kTiles2 + kTiles3) { off = kOffW3; in = kH; mt = (t - kTiles1 - kTiles2) / 4; kt = (t - kTiles1 - kTiles2) % 4; }
+  else { off = kOffW4; in = kH; mt = 0; kt = t - kTiles1 - kTiles2 - kTiles3; }
+  for (int e = 0; e < 4; ++e) wpack[(t * 64 + lane) * 4 + e] = f2bf(prm[off + (mt * 16 + r) * in + kt * 16 + 4 * q + e]);
+  if (t == 0) {
+    for (int i = lane; i < kNBias; i += 64) {
+      const int o = i < 3 * kH ? (i < kH ? kOffB1 + i : (i < 2 * kH ? kOffB2 + i - kH : kOffB3 + i - 2 * kH)) : kOffB4 + i - 3 * kH;
+      bpack[i] = prm[o];
+    }
+  }
+}
+
+// Wavefront-cooperative inference: every lane brings one sample's 16 features; the
+// 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
+// and biases in LDS; sX: this wave's 16x64 LDS scratch.  Returns outputs 0 and 1.
+__device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const float* __restrict__ sB, float* sX,
+                                         const float* f, float& o0, float& o1) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < kNFeat; ++i) sX[i * 64 + lane] = f[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const bf16x4_t* W = reinterpret_cast<const bf16x4_t*>(sW);
+  bf16x4_t hb[4][4];  // [k tile][n tile]
+  // layer 1: K = 16
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    bf16x4_t xb;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xb[e] = f2bf(sX[(4 * q + e) * 64 + nt * 16 + r]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4_t acc;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = sB[mt * 16 + 4 * q + e];
+      acc = mfma16(W[mt * 64 + lane], xb, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hb[mt][nt][e] = f2bf(fmaxf(acc[e], 0.f));
+    }
+  }
+  // layers 2 and 3: K = 64
+#pragma unroll
+  for (int l = 0; l < 2; ++l) {
+    const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
+    const float* bl = sB + kH * (l + 1);
+    bf16x4_t nb[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        f32x4_t acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] = bl[mt * 16 + 4 * q + e];
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt][nt], acc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) nb[mt][nt][e] = f2bf(fmaxf(acc[e], 0.f));
+      }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) hb[a][b] = nb[a][b];
+  }
+  // layer 4: 16 outputs (rows 0, 1 used)
+  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
+  float y0[4], y1[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    f32x4_t acc;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = sB[3 * kH + 4 * q + e];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt][nt], acc);
+    y0[nt] = acc[0];
+    y1[nt] = acc[1];
+  }
+  // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r; hand them back
+  __builtin_amdgcn_wave_barrier();
+  if (q == 0) {
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      sX[(nt * 16 + r) * 2] = y0[nt];
+      sX[(nt * 16 + r) * 2 + 1] = y1[nt];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  o0 = sX[lane * 2];
+  o1 = sX[lane * 2 + 1];
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Batch inference through the register path (the one the renderer uses).
+__global__ __launch_bounds__(256) void k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ bpack,
+                                                   const float* __restrict__ X, int n, float* __restrict__ Y) {
+  __shared__ short sW[kPackShorts];
+  __shared__ float sB[kNBias];
+  __shared__ float sX[4][16 * 64];
+  for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
+    reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
+  for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
+  __syncthreads();
+  const int w = threadIdx.x >> 6;
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  float f[kNFeat];
+  for (int i = 0; i < kNFeat; ++i) f[i] = s < n ? X[size_t(s) * kNFeat + i] : 0.f;
+  float o0, o1;
+  mlp_wave(sW, sB, sX[w], f, o0, o1);
+  if (s < n) {
+    Y[size_t(s) * 2] = o0;
+    Y[size_t(s) * 2 + 1] = o1;
+  }
+}
+
+// ---------------------------------------------------------------- training samples
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float u01(uint64_t& s) {
+  s += 0x9E3779B97F4A7C15ull;
+  return float(mix64(s) >> 40) * (1.0f / 16777216.0f);
+}
+__device__ __forceinline__ f3 unit_sphere(uint64_t& s) {
+  const float z = 2.f * u01(s) - 1.f, ph = 6.283185307179586f * u01(s);
+  const float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+  return mk3(r * cosf(ph), r * sinf(ph), z);
+}
+
+// Sample i: a position drawn uniformly in the volume box until the transfer function
+// gives it nonzero extinction (the samples the renderer shades), a view direction and
+// a light direction uniform on the sphere; outputs the 16 features and the exact
+// (occlusion, shadow) of the cone tracer.
+template <typename T>
+__global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const float* __restrict__ pyr, ConeK K,
+                                                    const float4* __restrict__ tf, int ntf, uint64_t seed, int n,
+                                                    float* __restrict__ X, float* __restrict__ Tg,
+                                                    float* __restrict__ pos_out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s = mix64(seed ^ (uint64_t(i) * 0xD1B54A32D192ED03ull));
+  const T* vox = static_cast<const T*>(g.vox);
+  f3 p = mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
+  for (int t = 0; t < 64; ++t) {
+    const f3 c = mk3(u01(s) * g.G[0], u01(s) * g.G[1], u01(s) * g.G[2]);
+    p = c;
+    if (sample_tf(tf, ntf, sample_volume<T>(vox, g, c)).w > 0.f) break;
+  }
+  const f3 dir = unit_sphere(s);
+  const f3 L = unit_sphere(s);
+  f3 k, up, right;
+  occ_frame(dir, k, up, right);
+  float f[kNFeat];
+  nn_feats(pyr, pg, K.st, p, k, L, f);
+  for (int c = 0; c < kNFeat; ++c) X[size_t(i) * kNFeat + c] = f[c];
+  Tg[size_t(i) * 2] = ct_occlusion(pyr, pg, K, p, k, up, right);
+  Tg[size_t(i) * 2 + 1] = cone_T(pyr, pg, K.ss, K.n_steps, p, L);
+  if (pos_out) {
+    pos_out[size_t(i) * 6 + 0] = p.x; pos_out[size_t(i) * 6 + 1] = p.y; pos_out[size_t(i) * 6 + 2] = p.z;
+    pos_out[size_t(i) * 6 + 3] = dir.x; pos_out[size_t(i) * 6 + 4] = dir.y; pos_out[size_t(i) * 6 + 5] = dir.z;
+  }
+}
+
+// Per-feature range of the training set (features are >= 0, so the float bit pattern
+// orders like the value).
+__global__ void k_feat_range(const float* __restrict__ X, int n, unsigned int* __restrict__ lo,
+                             unsigned int* __restrict__ hi) {
+  __shared__ unsigned int slo[kNFeat], shi[kNFeat];
+  if (threadIdx.x < kNFeat) { slo[threadIdx.x] = 0x7F800000u; shi[threadIdx.x] = 0u; }
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int c = 0; c < kNFeat; ++c) {
+      const unsigned int b = __float_as_uint(fmaxf(X[size_t(i) * kNFeat + c], 0.f));
+      atomicMin(&slo[c], b);
+      atomicMax(&shi[c], b);
+    }
+  __syncthreads();
+  if (threadIdx.x < kNFeat) {
+    atomicMin(&lo[threadIdx.x], slo[threadIdx.x]);
+    atomicMax(&hi[threadIdx.x], shi[threadIdx.x]);
+  }
+}
+
+// ---------------------------------------------------------------- surrogate renderer
+struct NnK {
+  float lo[kNFeat], hi[kNFeat];  // training range (+- margin): outside -> exact tracer
+  int exact_all;                 // validation gate failed: every sample uses the exact tracer
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const float* __restrict__ pyr,
+                                                   ConeK K, NnK nk, const short* __restrict__ wpack,
+                                                   const float* __restrict__ bpack, const float4* __restrict__ tf,
+                                                   int ntf, float step, float et, float4* __restrict__ out,
+                                                   unsigned long long* __restrict__ counters) {
+  __shared__ short sW[kPackShorts];
+  __shared__ float sB[kNBias];
+  __shared__ float sX[4][16 * 64];
+  extern __shared__ float4 s_tf[];
+  for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
+    reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
+  for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
+  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+  __syncthreads();
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_x = (rf.W + 15) >> 4;
+  const int tx = bid % tiles_x, ty = bid / tiles_x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
+  const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+  const bool in_img = px < rf.W && py < rf.H;
+  const T* vox = static_cast<const T*>(g.vox);
+  const f3 dir = camera_ray(rf, in_img ? px : 0, in_img ? py : 0);
+  const f3 eye = ld3(rf.eye);
+  float tn = 0.f, tf_ = 0.f;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned int nsamp = 0, nshade = 0, nfall = 0;
+  const float ka = K.apply_occ ? K.ka : 0.f, kd = K.apply_sdw ? K.kd : 0.f;
+  const float inv_k = 1.0f / (ka + kd);
+  bool active = in_img && ray_box(eye, dir, g.G, tn, tf_);
+  f3 k, up, right;
+  occ_frame(dir, k, up, right);
+  const f3 L = ld3(K.L);
+  const float D = fabsf(tf_ - tn);
+  const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
+  int j = 0;
+  // The loop runs while any lane of the wave marches; each iteration every marching
+  // lane takes its next sample, and when any lane must be shaded the whole wave
+  // evaluates the network together.
+  while (__any(active)) {
+    bool need = false;
+    float h = 0.f;
+    f3 p = tp;
+    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) {
+      const float s = float(j) * step;
+      if (!(s < D)) {
+        active = false;
+      } else {
+        h = fminf(step, D - s);
+        p = tp + dir * (s + 0.5f * h);
+        src = sample_tf(s_tf, ntf, sample_volume<T>(vox, g, p));
+        ++nsamp;
+        need = src.w > 0.f;
+        if (!need) ++j;
+      }
+    }
+    if (__any(need)) {
+      float f[kNFeat];
+      bool ood = false;
+      if (need) {
+        nn_feats(pyr, pg, K.st, p, k, L, f);
+#pragma unroll
+        for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < kNFeat; ++c) f[c] = 0.f;
+      }
+      float Io, Is;
+      mlp_wave(sW, sB, sX[w], f, Io, Is);
+      if (need) {
+        ++nshade;
+        if (ood || nk.exact_all) {
+          ++nfall;
+          Io = K.apply_occ ? ct_occlusion(pyr, pg, K, p, k, up, right) : 0.f;
+          Is = K.apply_sdw ? cone_T(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
+        }
+        Io = fminf(fmaxf(Io, 0.f), 1.f);
+        Is = fminf(fmaxf(Is, 0.f), 1.f);
+        const float m = inv_k * (Io * ka + Is * kd);
+        const float a = 1.0f - __expf(-src.w * h);
+        const float wgt = (1.0f - dst.w) * a;
+        dst.x += wgt * src.x * m;
+        dst.y += wgt * src.y * m;
+        dst.z += wgt * src.z * m;
+        dst.w += wgt;
+        ++j;
+        if (dst.w > et) active = false;
+      }
+    }
+  }
+  if (in_img) out[size_t(py) * rf.W + px] = dst;
+  if (counters && in_img) {
+    atomicAdd(&counters[0], (unsigned long long)nsamp);
+    atomicAdd(&counters[1], (unsigned long long)nshade);
+    atomicAdd(&counters[2], (unsigned long long)nfall);
+  }
+}
+
+// ---------------------------------------------------------------- host side
+int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos, const float* d_dirs,
+                const float* d_lights, float* d_occ, float* d_sdw, float* d_feat, hipStream_t st);
+int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);
+ConeK cone_constants(const cvr_ctx* c, const cvr_cone_params& P, const float L[3]);
+void light_dir(const RayFrame& rf, const cvr_light* light, float L[3]);
+int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+
+static int ensure(void** p, size_t bytes) {
+  if (*p) return 0;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    set_error("surrogate: out of device memory");
+    return -5;
+  }
+  return 0;
+}
+
+int nn_alloc(cvr_ctx* c) {
+  int rc = 0;
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_w), kNParams * sizeof(float));
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_g), kNParams * sizeof(float));
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_m), kNParams * sizeof(float));
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_v), kNParams * sizeof(float));
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_pack), kPackShorts * sizeof(short) + kNBias * sizeof(float));
+  rc |= ensure(reinterpret_cast<void**>(&c->d_nn_scalar), 64 * sizeof(float));
+  return rc ? -5 : 0;
+}
+
+int nn_pack(cvr_ctx* c, hipStream_t st) {
+  short* wp = static_cast<short*>(c->d_nn_pack);
+  float* bp = reinterpret_cast<float*>(wp + kPackShorts);
+  hipLaunchKernelGGL(k_mlp_pack, dim3(kNTiles), dim3(64), 0, st, c->d_nn_w, wp, bp);
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// He-uniform initial weights (seeded), zero biases.
+static void init_weights(std::vector<float>& w, uint64_t seed) {
+  w.assign(kNParams, 0.f);
+  uint64_t s = seed ^ 0xA0761D6478BD642Full;
+  auto u = [&]() {
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const int offs[4] = {kOffW1, kOffW2, kOffW3, kOffW4}, ins[4] = {kNFeat, kH, kH, kH}, outs[4] = {kH, kH, kH, kOut};
+  for (int l = 0; l < 4; ++l) {
+    const double a = std::sqrt(6.0 / ins[l]) * (l == 3 ? 0.5 : 1.0);
+    for (int i = 0; i < outs[l] * ins[l]; ++i) w[offs[l] + i] = float((2.0 * u() - 1.0) * a);
+  }
+}
+
+constexpr size_t kTrainLds = (16 * 64 * 2 + 64 * 64 * 5 + kNParams) * sizeof(float);
+
+// The training kernel keeps one 64-sample group's activations and the whole gradient
+// in LDS (~129 KiB of the 160 KiB gfx950 gives a workgroup).
+static int train_kernel_attr() {
+  static int done = 0;
+  if (!done) {
+    CVR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_train),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kTrainLds)));
+    done = 1;
+  }
+  return 0;
+}
+
+// Gradient of L = sum (y - t)^2 / (2 n) over n samples at the current weights.
+int nn_gradient(cvr_ctx* c, const float* d_X, const float* d_T, int n, hipStream_t st) {
+  CVR_HIP_CHECK(hipMemsetAsync(c->d_nn_g, 0, kNParams * sizeof(float), st));
+  CVR_HIP_CHECK(hipMemsetAsync(c->d_nn_scalar, 0, sizeof(float), st));
+  const int chunks = 4;
+  const int blocks = (n + 64 * chunks - 1) / (64 * chunks);
+  const size_t shm = kTrainLds;
+  int rc = train_kernel_attr();
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64), shm, st, c->d_nn_w, d_X, d_T, n, chunks, float(n),
+                     c->d_nn_g, c->d_nn_scalar);
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int nn_eval(cvr_ctx* c, const float* d_X, int n, float* d_Y, int packed, hipStream_t st) {
+  if (packed) {
+    short* wp = static_cast<short*>(c->d_nn_pack);
+    float* bp = reinterpret_cast<float*>(wp + kPackShorts);
+    hipLaunchKernelGGL(k_mlp_infer, dim3((n + 255) / 256), dim3(256), 0, st, wp, bp, d_X, n, d_Y);
+  } else {
+    hipLaunchKernelGGL(k_mlp_eval, dim3((n + 63) / 64), dim3(64), 0, st, c->d_nn_w, d_X, n, d_Y);
+  }
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int nn_generate(cvr_ctx* c, const cvr_cone_params& P, uint64_t seed, int n, float* d_X, float* d_T, float* d_pos,
+                hipStream_t st) {
+  int rc = prepare_cone(c, P, st);
+  if (rc) return rc;
+  float L[3] = {0, 0, 1};
+  ConeK K = cone_constants(c, P, L);
+  VolGeom g = geom_of(c);
+  if (c->dtype == CVR_U8)
+    hipLaunchKernelGGL(k_nn_samples<uint8_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr, K,
+                       c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
+  else
+    hipLaunchKernelGGL(k_nn_samples<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr, K,
+                       c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Generates the training and validation sets with the exact tracer, trains with Adam,
+// measures the validation RMSE of the network and of the network-free baseline, and
+// records the feature range used to send out-of-range samples to the exact tracer.
+int nn_train(cvr_ctx* c, hipStream_t st) {
+  const cvr_nn_params& N = c->nn;
+  int rc = nn_alloc(c);
+  if (rc) return rc;
+  const int n = N.n_train, nv = N.n_val;
+  if (n < 64 || nv < 64 || N.batch < 64 || N.epochs < 0) {
+    set_error("surrogate: n_train, n_val, batch >= 64 and epochs >= 0 required");
+    return -1;
+  }
+  float *d_X = nullptr, *d_T = nullptr, *d_Xv = nullptr, *d_Tv = nullptr, *d_Yv = nullptr;
+  unsigned int* d_rng = nullptr;
+  auto cleanup = [&]() {
+    if (d_X) (void)hipFree(d_X);
+    if (d_T) (void)hipFree(d_T);
+    if (d_Xv) (void)hipFree(d_Xv);
+    if (d_Tv) (void)hipFree(d_Tv);
+    if (d_Yv) (void)hipFree(d_Yv);
+    if (d_rng) (void)hipFree(d_rng);
+  };
+  if (hipMalloc(&d_X, size_t(n) * kNFeat * 4) != hipSuccess || hipMalloc(&d_T, size_t(n) * 2 * 4) != hipSuccess ||
+      hipMalloc(&d_Xv, size_t(nv) * kNFeat * 4) != hipSuccess || hipMalloc(&d_Tv, size_t(nv) * 2 * 4) != hipSuccess ||
+      hipMalloc(&d_Yv, size_t(nv) * 2 * 4) != hipSuccess || hipMalloc(&d_rng, 2 * kNFeat * 4) != hipSuccess) {
+    cleanup();
+    set_error("surrogate: out of device memory for the training set");
+    return -5;
+  }
+  rc = nn_generate(c, N.cone, N.seed, n, d_X, d_T, nullptr, st);
+  if (!rc) rc = nn_generate(c, N.cone, N.seed ^ 0x5851F42D4C957F2Dull, nv, d_Xv, d_Tv, nullptr, st);
+  if (rc) { cleanup(); return rc; }
+  // feature range
+  std::vector<unsigned int> init(2 * kNFeat);
+  for (int i = 0; i < kNFeat; ++i) { init[i] = 0x7F800000u; init[kNFeat + i] = 0u; }
+  (void)hipMemcpy(d_rng, init.data(), init.size() * 4, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_feat_range, dim3(256), dim3(256), 0, st, d_X, n, d_rng, d_rng + kNFeat);
+  std::vector<unsigned int> rng(2 * kNFeat);
+  if (hipMemcpyAsync(rng.data(), d_rng, rng.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    cleanup();
+    set_error("surrogate: feature range copy failed");
+    return -2;
+  }
+  for (int i = 0; i < kNFeat; ++i) {
+    float lo, hi;
+    std::memcpy(&lo, &rng[i], 4);
+    std::memcpy(&hi, &rng[kNFeat + i], 4);
+    c->nn_lo[i] = lo;
+    c->nn_hi[i] = hi;
+  }
+  // weights and optimiser state
+  std::vector<float> w0;
+  init_weights(w0, N.seed);
+  (void)hipMemcpy(c->d_nn_w, w0.data(), kNParams * 4, hipMemcpyHostToDevice);
+  (void)hipMemset(c->d_nn_m, 0, kNParams * 4);
+  (void)hipMemset(c->d_nn_v, 0, kNParams * 4);
+  (void)hipMemset(c->d_nn_g, 0, kNParams * 4);
+  const int B = std::min(N.batch, n);
+  const int steps_per_epoch = n / B;
+  const int total = steps_per_epoch * N.epochs;
+  const int chunks = 4;
+  const size_t shm = kTrainLds;
+  rc = train_kernel_attr();
+  if (rc) { cleanup(); return rc; }
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double bc1 = 1.0, bc2 = 1.0;
+  for (int t = 0; t < total; ++t) {
+    const int off = (t % steps_per_epoch) * B;
+    (void)hipMemsetAsync(c->d_nn_scalar, 0, sizeof(float), st);
+    hipLaunchKernelGGL(k_mlp_train, dim3((B + 64 * chunks - 1) / (64 * chunks)), dim3(64), shm, st, c->d_nn_w,
+                       d_X + size_t(off) * kNFeat, d_T + size_t(off) * 2, B, chunks, float(B), c->d_nn_g,
+                       c->d_nn_scalar);
+    bc1 *= b1;
+    bc2 *= b2;
+    // cosine decay of the learning rate over the run
+    const float lr = float(N.lr * 0.5 * (1.0 + std::cos(3.14159265358979 * double(t) / double(std::max(total, 1)))));
+    hipLaunchKernelGGL(k_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, c->d_nn_w, c->d_nn_g, c->d_nn_m,
+                       c->d_nn_v, kNParams, lr, b1, b2, eps, float(1.0 - bc1), float(1.0 - bc2));
+  }
+  if (hipGetLastError() != hipSuccess) { cleanup(); set_error("surrogate: training launch failed"); return -2; }
+  float last_loss = 0.f;
+  (void)hipMemcpyAsync(&last_loss, c->d_nn_scalar, 4, hipMemcpyDeviceToHost, st);
+  rc = nn_pack(c, st);
+  if (!rc) rc = nn_eval(c, d_Xv, nv, d_Yv, 1, st);
+  std::vector<float> Xv(size_t(nv) * kNFeat), Tv(size_t(nv) * 2), Yv(size_t(nv) * 2);
+  if (!rc && (hipMemcpyAsync(Xv.data(), d_Xv, Xv.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipMemcpyAsync(Tv.data(), d_Tv, Tv.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipMemcpyAsync(Yv.data(), d_Yv, Yv.size() * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+              hipStreamSynchronize(st) != hipSuccess)) {
+    set_error("surrogate: validation copy failed");
+    rc = -2;
+  }
+  cleanup();
+  if (rc) return rc;
+  double se[2] = {0, 0}, sb[2] = {0, 0};
+  for (int i = 0; i < nv; ++i) {
+    double base_o = 1.0, base_s = 1.0;
+    for (int s = 0; s < kNStencil; ++s) {
+      base_o *= 1.0 - Xv[size_t(i) * kNFeat + s];
+      base_s *= 1.0 - Xv[size_t(i) * kNFeat + kNStencil + s];
+    }
+    const double yo = std::min(std::max(double(Yv[i * 2]), 0.0), 1.0);
+    const double ys = std::min(std::max(double(Yv[i * 2 + 1]), 0.0), 1.0);
+    se[0] += (yo - Tv[i * 2]) * (yo - Tv[i * 2]);
+    se[1] += (ys - Tv[i * 2 + 1]) * (ys - Tv[i * 2 + 1]);
+    sb[0] += (base_o - Tv[i * 2]) * (base_o - Tv[i * 2]);
+    sb[1] += (base_s - Tv[i * 2 + 1]) * (base_s - Tv[i * 2 + 1]);
+  }
+  c->nn_report.train_loss = last_loss;
+  c->nn_report.val_rmse_occ = std::sqrt(se[0] / nv);
+  c->nn_report.val_rmse_sdw = std::sqrt(se[1] / nv);
+  c->nn_report.base_rmse_occ = std::sqrt(sb[0] / nv);
+  c->nn_report.base_rmse_sdw = std::sqrt(sb[1] / nv);
+  c->nn_report.steps = total;
+  c->nn_report.exact_fallback = (std::max(c->nn_report.val_rmse_occ, c->nn_report.val_rmse_sdw) > N.rmse_gate) ? 1 : 0;
+  c->nn_trained = true;
+  return 0;
+}
+
+int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
+  const cvr_nn_params& N = c->nn;
+  int rc = prepare_cone(c, N.cone, st);
+  if (rc) return rc;
+  if (!c->nn_trained) {
+    rc = nn_train(c, st);
+    if (rc) return rc;
+  }
+  float L[3];
+  light_dir(rf, light, L);
+  ConeK K = cone_constants(c, N.cone, L);
+  NnK nk{};
+  for (int i = 0; i < kNFeat; ++i) { nk.lo[i] = c->nn_lo[i] - N.ood_margin; nk.hi[i] = c->nn_hi[i] + N.ood_margin; }
+  c->nn_report.exact_fallback =
+      (std::max(c->nn_report.val_rmse_occ, c->nn_report.val_rmse_sdw) > N.rmse_gate) ? 1 : 0;
+  nk.exact_all = (N.force_exact || c->nn_report.exact_fallback) ? 1 : 0;
+  if (nk.exact_all) {
+    // the validation gate failed: the exact tracer renders the frame
+    cvr_cone_params saved = c->cone;
+    c->cone = N.cone;
+    rc = launch_cone(c, rf, light, out, st);
+    c->cone = saved;
+    return rc;
+  }
+  VolGeom g = geom_of(c);
+  const float step = N.cone.step_size > 0 ? N.cone.step_size : default_step(c);
+  const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
+  const size_t shm = size_t(c->tf_size) * sizeof(float4);
+  if (shm > 32 * 1024) { set_error("surrogate renderer: transfer function larger than 2048 entries"); return -4; }
+  short* wp = static_cast<short*>(c->d_nn_pack);
+  float* bp = reinterpret_cast<float*>(wp + kPackShorts);
+  unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
+  if (c->dtype == CVR_U8)
+    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K, nk, wp, bp,
+                       c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  else
+    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K, nk, wp, bp,
+                       c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int nn_param_count() { return kNParams; }
+
+}  // namespace cvr

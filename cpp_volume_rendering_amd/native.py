@@ -47,17 +47,41 @@ class EbsParams(C.Structure):
                 ("ka", C.c_float), ("kd", C.c_float)]
 
 
+class ConeParams(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
+                ("apply_shadow", C.c_int), ("n_levels", C.c_int), ("base_res", C.c_int * 3), ("sigma0", C.c_float),
+                ("n_steps", C.c_int), ("n_occ_cones", C.c_int), ("occ_half_deg", C.c_float),
+                ("ring_deg", C.c_float), ("sdw_half_deg", C.c_float), ("t0", C.c_float),
+                ("occ_max_dist", C.c_float), ("sdw_max_dist", C.c_float), ("r_sigma", C.c_float),
+                ("ka", C.c_float), ("kd", C.c_float)]
+
+
+class NnParams(C.Structure):
+    _fields_ = [("cone", ConeParams), ("n_train", C.c_int), ("n_val", C.c_int), ("epochs", C.c_int),
+                ("batch", C.c_int), ("lr", C.c_float), ("rmse_gate", C.c_float), ("ood_margin", C.c_float),
+                ("force_exact", C.c_int), ("seed", C.c_uint64)]
+
+
+class NnReport(C.Structure):
+    _fields_ = [("train_loss", C.c_double), ("val_rmse_occ", C.c_double), ("val_rmse_sdw", C.c_double),
+                ("base_rmse_occ", C.c_double), ("base_rmse_sdw", C.c_double), ("steps", C.c_int),
+                ("exact_fallback", C.c_int)]
+
+
 class Stats(C.Structure):
-    _fields_ = [("samples", C.c_uint64), ("shaded_samples", C.c_uint64), ("last_ms", C.c_double)]
+    _fields_ = [("samples", C.c_uint64), ("shaded_samples", C.c_uint64), ("fallback_samples", C.c_uint64),
+                ("last_ms", C.c_double)]
 
 
-PARAMS = {RC1P: Rc1pParams, EBS: EbsParams}
+PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_TRACE: ConeParams, CONE_NN: NnParams}
 
 # every symbol include/cvr.h and include/cvr_synth.h declare
 EXPORTS = [
     "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
     "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
     "cvr_render", "cvr_get_sat", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
+    "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
     "cvr_synth_blobs", "cvr_synth_noise",
 ]
 
@@ -97,6 +121,16 @@ def lib() -> C.CDLL:
         "cvr_get_sat": (i, [vp, vp, sz]),
         "cvr_get_stats": (i, [vp, C.POINTER(Stats)]),
         "cvr_set_collect_stats": (i, [vp, i]),
+        "cvr_pyramid_info": (i, [vp, vp, vp, i]),
+        "cvr_get_pyramid_level": (i, [vp, i, vp, sz]),
+        "cvr_cone_eval": (i, [vp, i, vp, vp, vp, vp, vp, vp]),
+        "cvr_nn_param_count": (i, []),
+        "cvr_nn_train": (i, [vp, vp]),
+        "cvr_nn_get_weights": (i, [vp, vp, sz]),
+        "cvr_nn_set_weights": (i, [vp, vp, sz]),
+        "cvr_nn_eval": (i, [vp, i, vp, vp, i]),
+        "cvr_nn_gradient": (i, [vp, i, vp, vp, vp, vp]),
+        "cvr_nn_get_report": (i, [vp, C.POINTER(NnReport)]),
         "cvr_synth_blobs": (i, [i, i, i, i, i, d, i, C.c_uint64, vp]),
         "cvr_synth_noise": (i, [i, i, i, i, i, C.c_uint64, i, vp, vp]),
     }
@@ -213,6 +247,61 @@ class Renderer:
         out = np.empty((nz + 2, ny + 2, nx + 2), dtype=np.float64)
         check(lib().cvr_get_sat(self.h, out.ctypes.data, out.size))
         return out
+
+    # ------------------------------------------------------------ cone tracer / surrogate
+    def pyramid(self):
+        """List of the pyramid's extinction levels as (nz, ny, nx) float32 arrays."""
+        n = C.c_int()
+        dims = np.zeros(3 * 16, np.int32)
+        check(lib().cvr_pyramid_info(self.h, C.byref(n), dims.ctypes.data, 16))
+        out = []
+        for l in range(n.value):
+            nx, ny, nz = dims[3 * l:3 * l + 3]
+            a = np.empty((nz, ny, nx), np.float32)
+            check(lib().cvr_get_pyramid_level(self.h, l, a.ctypes.data, a.size))
+            out.append(a)
+        return out
+
+    def cone_eval(self, pos, dirs, lights):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 3)
+        n = len(pos)
+        occ, sdw, feat = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros((n, 16), np.float32)
+        check(lib().cvr_cone_eval(self.h, n, pos.ctypes.data, dirs.ctypes.data, lights.ctypes.data,
+                                  occ.ctypes.data, sdw.ctypes.data, feat.ctypes.data))
+        return occ, sdw, feat
+
+    def nn_train(self, stream=None):
+        check(lib().cvr_nn_train(self.h, stream))
+
+    def nn_weights(self):
+        w = np.zeros(lib().cvr_nn_param_count(), np.float32)
+        check(lib().cvr_nn_get_weights(self.h, w.ctypes.data, w.size))
+        return w
+
+    def nn_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        check(lib().cvr_nn_set_weights(self.h, w.ctypes.data, w.size))
+
+    def nn_eval(self, X, path=1):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        Y = np.zeros((len(X), 2), np.float32)
+        check(lib().cvr_nn_eval(self.h, len(X), X.ctypes.data, Y.ctypes.data, path))
+        return Y
+
+    def nn_gradient(self, X, T):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 16)
+        T = np.ascontiguousarray(T, np.float32).reshape(-1, 2)
+        g = np.zeros(lib().cvr_nn_param_count(), np.float32)
+        loss = C.c_float()
+        check(lib().cvr_nn_gradient(self.h, len(X), X.ctypes.data, T.ctypes.data, g.ctypes.data, C.byref(loss)))
+        return loss.value, g
+
+    def nn_report(self) -> NnReport:
+        r = NnReport()
+        check(lib().cvr_nn_get_report(self.h, C.byref(r)))
+        return r
 
     def collect_stats(self, on=True):
         check(lib().cvr_set_collect_stats(self.h, int(on)))

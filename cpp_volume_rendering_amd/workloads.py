@@ -101,3 +101,16 @@ def banded_noise(n=256, width=1024, height=1024, seed=2) -> Workload:
     alpha = np.array([0.0, 0.0, 0.05, 0.05, 0.3, 0.3, 0.8, 0.8])
     return Workload(f"ebs_noise{n}", vol, (1.0, 1.0, 1.0), np.column_stack([rgb, iso]),
                     np.column_stack([alpha, iso]), width, height)
+
+
+def cone_blobs(n=256, width=1024, height=1024, seed=3, n_blobs=64, noise_weight=0.0) -> Workload:
+    """Cone-traced lighting: n^3 uint8 sum of 64 Gaussian blobs (+ optional value noise),
+    4-point TF, one directional light."""
+    vol = N.synth_blobs((n, n, n), N.U8, n_blobs=n_blobs, seed=seed, noise_weight=noise_weight, noise_octaves=4)
+    rgb, a = seeded_tf(seed + 100, 4)
+    return Workload(f"cone_blobs{n}", vol, (1.0, 1.0, 1.0), rgb, a, width, height)
+
+
+def surrogate_blobs_noise(n=512, width=1024, height=1024, seed=4) -> Workload:
+    """Learned surrogate: n^3 uint8 of 64 Gaussian blobs plus 4-octave value noise."""
+    return cone_blobs(n, width, height, seed=seed, n_blobs=64, noise_weight=0.25)

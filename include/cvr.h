@@ -34,8 +34,15 @@ extern "C" {
 /* Techniques.  Each replaces one reference renderer class:
  *   CVR_RC1P        RayCasting1Pass            cppvolrend/structured/rc1pass/rc1prenderer.h:32-78
  *   CVR_EBS         RC1PExtinctionBasedShading cppvolrend/structured/rc1pextbsd/ebsrenderer.h:45-119
+ *   CVR_CONE_TRACE  cone-traced directional occlusion + shadows over the Gaussian
+ *                   extinction pyramid of RC1PConeTracingDirOcclusionShading
+ *                   (cppvolrend/structured/rc1pdosct/dosrcrenderer.h:38-121 and
+ *                   extcoefvolumegenerator.cpp:230-408), with 1 axial + ring
+ *                   occlusion cones and one shadow cone marched in fixed steps
+ *   CVR_CONE_NN     the same renderer with the per-sample cone integration replaced
+ *                   by a learned bf16 surrogate; no reference counterpart
  */
-enum cvr_technique { CVR_RC1P = 0, CVR_EBS = 1 };
+enum cvr_technique { CVR_RC1P = 0, CVR_EBS = 1, CVR_CONE_TRACE = 4, CVR_CONE_NN = 5 };
 
 enum cvr_dtype { CVR_U8 = 0, CVR_U16 = 1 };
 
@@ -83,6 +90,55 @@ typedef struct cvr_ebs_params {
   float ka, kd;          /* Kambient, Kdiffuse: 0.5, 0.5 */
 } cvr_ebs_params;
 
+/* Cone-traced lighting over the Gaussian extinction pyramid.  The pyramid follows
+ * ExtinctionCoefficientVolume (extcoefvolumegenerator.cpp:10-15, 230-408: 7^3
+ * Gaussian taps, sigma0 * 2^i per level, extinction = -log(1 - opacity)).  Cone k
+ * is marched over n_steps geometric distances t0 * q^j (q = (max/t0)^(1/n_steps));
+ * each step reads the level whose sigma is r/r_sigma for the cone radius r there. */
+typedef struct cvr_cone_params {
+  float step_size;      /* <= 0: reference default */
+  float et_threshold;   /* 0.99 */
+  int apply_occlusion;  /* 1 */
+  int apply_shadow;     /* 1 */
+  int n_levels;         /* pyramid levels kept; <= 0: the full chain down to 1^3 */
+  int base_res[3];      /* pyramid level-0 resolution; 0: the volume's */
+  float sigma0;         /* level-0 Gaussian sigma, world units (1.0) */
+  int n_steps;          /* steps per cone (64, at most 256) */
+  int n_occ_cones;      /* occlusion cones: 1 axial + n-1 on a ring (8) */
+  float occ_half_deg;   /* occlusion cone half angle (30) */
+  float ring_deg;       /* angle of the ring cones' axes to the occlusion axis (60) */
+  float sdw_half_deg;   /* shadow cone half angle (10) */
+  float t0;             /* first distance, world units (1.0) */
+  float occ_max_dist;   /* <= 0: 0.5 * diagonal */
+  float sdw_max_dist;   /* <= 0: 0.75 * diagonal */
+  float r_sigma;        /* cone radius / Gaussian sigma at the level read (2.0) */
+  float ka, kd;         /* 0.5, 0.5 */
+} cvr_cone_params;
+
+/* Surrogate: the cone parameters it replaces (and falls back to) plus training. */
+typedef struct cvr_nn_params {
+  cvr_cone_params cone;
+  int n_train;          /* exact-tracer samples generated for training (2^22) */
+  int n_val;            /* held-out samples judging the network (65536) */
+  int epochs;           /* 4 */
+  int batch;            /* 16384 */
+  float lr;             /* peak Adam learning rate, cosine decay (2e-3) */
+  float rmse_gate;      /* validation RMSE above which the exact tracer renders (0.05) */
+  float ood_margin;     /* inputs this far outside the training range use the exact tracer (0.02) */
+  int force_exact;      /* nonzero: every sample uses the exact tracer */
+  uint64_t seed;        /* training-set and initialisation seed */
+} cvr_nn_params;
+
+typedef struct cvr_nn_report {
+  double train_loss;     /* loss of the last minibatch */
+  double val_rmse_occ;   /* validation RMSE of the network vs the exact tracer */
+  double val_rmse_sdw;
+  double base_rmse_occ;  /* the same for the network-free stencil product */
+  double base_rmse_sdw;
+  int steps;             /* optimiser steps taken */
+  int exact_fallback;    /* 1 when the gate sends every sample to the exact tracer */
+} cvr_nn_report;
+
 /* Library / handle lifetime. */
 const char* cvr_last_error(void);
 const char* cvr_technique_name(int technique);
@@ -126,12 +182,41 @@ int cvr_get_sat(cvr_handle h, double* out_host, size_t count);
 
 /* Counters of the last render (samples marched, shaded samples). */
 typedef struct cvr_stats {
-  uint64_t samples;        /* volume samples taken along camera rays */
-  uint64_t shaded_samples; /* samples with nonzero extinction (lighting evaluated) */
-  double last_ms;          /* device time of the last render's main kernel */
+  uint64_t samples;          /* volume samples taken along camera rays */
+  uint64_t shaded_samples;   /* samples with nonzero extinction (lighting evaluated) */
+  uint64_t fallback_samples; /* surrogate: samples the exact tracer shaded */
+  double last_ms;            /* device time of the last render's main kernel */
 } cvr_stats;
 int cvr_get_stats(cvr_handle h, cvr_stats* out);
 int cvr_set_collect_stats(cvr_handle h, int enable);
+
+/* The extinction pyramid of CVR_CONE_TRACE / CVR_CONE_NN (built if needed): level
+ * count and dims (3 ints per level, x first), and one level's extinction (fp32,
+ * x fastest).  Replaces reading back ExtinctionCoefficientVolume's texture. */
+int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels);
+int cvr_get_pyramid_level(cvr_handle h, int level, float* out_host, size_t count);
+
+/* Exact cone-traced occlusion and shadow at n points (host arrays): positions in
+ * texture space [0, G], camera-ray directions, light directions (3 floats each);
+ * outputs occ[n], sdw[n] and the surrogate's 16 inputs feat[16 n] (each may be NULL). */
+int cvr_cone_eval(cvr_handle h, int n, const float* pos, const float* dirs, const float* lights, float* occ,
+                  float* sdw, float* feat);
+
+/* Surrogate network (CVR_CONE_NN).  cvr_nn_train generates the training set with the
+ * exact tracer and trains on the GPU (cvr_prepare does it when needed).  Weights are
+ * fp32, cvr_nn_param_count() of them, flattened W1[64][16] b1 W2[64][64] b2
+ * W3[64][64] b3 W4[16][64] b4[16]; setting them marks the network trained.
+ * cvr_nn_eval runs X[n][16] -> Y[n][2] on the GPU through the training kernels'
+ * path (path 0) or the renderer's register-resident path (path 1).
+ * cvr_nn_gradient returns dL/dparams of L = sum (y - t)^2 / (2 n) at the current
+ * weights, computed by the training kernel.  No reference counterpart. */
+int cvr_nn_param_count(void);
+int cvr_nn_train(cvr_handle h, void* hip_stream);
+int cvr_nn_get_weights(cvr_handle h, float* out, size_t count);
+int cvr_nn_set_weights(cvr_handle h, const float* in, size_t count);
+int cvr_nn_eval(cvr_handle h, int n, const float* X, float* Y, int path);
+int cvr_nn_gradient(cvr_handle h, int n, const float* X, const float* T, float* grad, float* loss);
+int cvr_nn_get_report(cvr_handle h, cvr_nn_report* out);
 
 #ifdef __cplusplus
 }

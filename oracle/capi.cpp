@@ -145,4 +145,150 @@ void orc_ebs_point(int nx, int ny, int nz, double sx, double sy, double sz, cons
   *sdw = e.shadows(p, nullptr);
 }
 
+// ------------------------------------------------------------------ pyramid / cones / MLP
+struct ConeC {
+  int n_steps, n_occ, apply_occ, apply_sdw;
+  double occ_half_deg, ring_deg, sdw_half_deg, t0, occ_max, sdw_max, r_sigma, sigma0, ka, kd;
+  double light_fwd[3];
+};
+static ConeParams cone_from(const ConeC* c) {
+  ConeParams p;
+  p.n_steps = c->n_steps; p.n_occ = c->n_occ; p.apply_occ = c->apply_occ; p.apply_sdw = c->apply_sdw;
+  p.occ_half_deg = c->occ_half_deg; p.ring_deg = c->ring_deg; p.sdw_half_deg = c->sdw_half_deg;
+  p.t0 = c->t0; p.occ_max = c->occ_max; p.sdw_max = c->sdw_max; p.r_sigma = c->r_sigma; p.sigma0 = c->sigma0;
+  p.ka = c->ka; p.kd = c->kd;
+  for (int i = 0; i < 3; ++i) p.light_fwd[i] = c->light_fwd[i];
+  return p;
+}
+
+void* orc_pyr_new(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                  int bx, int by, int bz, double S0, int nlev, int nthreads) {
+  threads(nthreads);
+  Pyramid* P = new Pyramid();
+  pyramid_build(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), bx, by, bz, S0, nlev, *P);
+  return P;
+}
+// A pyramid made of given extinction levels (concatenated, level 0 first).
+void* orc_pyr_wrap(double gx, double gy, double gz, int nlev, const int* dims, const double* tau) {
+  Pyramid* P = new Pyramid();
+  P->G[0] = gx; P->G[1] = gy; P->G[2] = gz;
+  P->levels = nlev;
+  size_t o = 0;
+  for (int i = 0; i < nlev; ++i) {
+    P->nx.push_back(dims[3 * i]); P->ny.push_back(dims[3 * i + 1]); P->nz.push_back(dims[3 * i + 2]);
+    size_t n = size_t(dims[3 * i]) * dims[3 * i + 1] * dims[3 * i + 2];
+    P->tau.emplace_back(tau + o, tau + o + n);
+    o += n;
+  }
+  return P;
+}
+void orc_pyr_free(void* h) { delete static_cast<Pyramid*>(h); }
+int orc_pyr_levels(void* h) { return static_cast<Pyramid*>(h)->levels; }
+void orc_pyr_dims(void* h, int l, int* d) {
+  Pyramid* P = static_cast<Pyramid*>(h);
+  d[0] = P->nx[l]; d[1] = P->ny[l]; d[2] = P->nz[l];
+}
+void orc_pyr_copy(void* h, int l, double* tau, double* op) {
+  Pyramid* P = static_cast<Pyramid*>(h);
+  if (tau) std::memcpy(tau, P->tau[l].data(), P->tau[l].size() * sizeof(double));
+  if (op && !P->op.empty()) std::memcpy(op, P->op[l].data(), P->op[l].size() * sizeof(double));
+}
+// One voxel of level 0 (opacity), for sparse checks of large pyramids.
+double orc_pyr_level0_at(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                         void* tf, int bx, int by, int bz, double S0, int x, int y, int z) {
+  Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  const TF& t = *static_cast<TF*>(tf);
+  double G[3] = {nx * sx, ny * sy, nz * sz};
+  double vs[3] = {G[0] / bx, G[1] / by, G[2] / bz};
+  double gp[3] = {(x + 0.5) * vs[0], (y + 0.5) * vs[1], (z + 0.5) * vs[2]};
+  double swc = 0, sw = 0;
+  for (int i = -3; i <= 3; ++i)
+    for (int j = -3; j <= 3; ++j)
+      for (int k = -3; k <= 3; ++k) {
+        double f[3] = {i * S0, j * S0, k * S0};
+        double wk = (S0 * S0 * S0) * std::exp(-(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) / (2.0 * S0 * S0));
+        double tt[3] = {(gp[0] + f[0]) / G[0], (gp[1] + f[1]) / G[1], (gp[2] + f[2]) / G[2]};
+        double ck = 0;
+        if (!(tt[0] < 0 || tt[1] < 0 || tt[2] < 0 || tt[0] > 1 || tt[1] > 1 || tt[2] > 1)) {
+          double d = tex3d_volume<double>(v, V3<double>(tt[0], tt[1], tt[2]));
+          double c[4];
+          tex1d_tf<double>(t.rgba.data(), t.size(), d, c);
+          ck = c[3];
+        }
+        swc += wk * ck;
+        sw += wk;
+      }
+  return swc / sw;
+}
+// One voxel of level i (opacity) from the given opacity level i-1.
+double orc_pyr_next_at(const double* prev, int px, int py, int pz, int nx, int ny, int nz, double Si, double gx,
+                       double gy, double gz, int x, int y, int z) {
+  double G[3] = {gx, gy, gz};
+  double lvs[3] = {G[0] / nx, G[1] / ny, G[2] / nz};
+  double gp[3] = {(x + 0.5) * lvs[0], (y + 0.5) * lvs[1], (z + 0.5) * lvs[2]};
+  double swc = 0, sw = 0;
+  for (int i = -3; i <= 3; ++i)
+    for (int j = -3; j <= 3; ++j)
+      for (int k = -3; k <= 3; ++k) {
+        double f[3] = {i * Si, j * Si, k * Si};
+        double wk = (Si * Si * Si) * std::exp(-(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]) / (2.0 * Si * Si));
+        double tt[3] = {(gp[0] + f[0]) / G[0], (gp[1] + f[1]) / G[1], (gp[2] + f[2]) / G[2]};
+        double ck = 0;
+        if (!(tt[0] < 0 || tt[1] < 0 || tt[2] < 0 || tt[0] > 1 || tt[1] > 1 || tt[2] > 1))
+          ck = tex3d_grid<double, double>(prev, px, py, pz, V3<double>(tt[0], tt[1], tt[2]));
+        swc += wk * ck;
+        sw += wk;
+      }
+  return swc / sw;
+}
+double orc_gauss_ext(void* h, double x, double y, double z, double lod) {
+  return gauss_ext<double>(*static_cast<Pyramid*>(h), V3<double>(x, y, z), lod);
+}
+void orc_cone_schedule(int n, double t0, double tmax, double half_deg, double r_sigma, double sigma0, int levels,
+                       double* out) {
+  std::vector<ConeStep> s = cone_schedule(n, t0, tmax, half_deg, r_sigma, sigma0, levels);
+  for (int j = 0; j < n; ++j) { out[3 * j] = s[j].tm; out[3 * j + 1] = s[j].dt; out[3 * j + 2] = s[j].lod; }
+}
+void orc_render_cone_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                         void* tf, void* pyr, const ConeC* c, const float* cam, int W, int H, double step, double et,
+                         double* out, double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  render_cone<double>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr),
+                      cone_from(c), mkcam(cam), W, H, step, et, out, margin, stride);
+}
+void orc_render_cone_f32(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                         void* tf, void* pyr, const ConeC* c, const float* cam, int W, int H, float step, float et,
+                         float* out, int nthreads, int stride) {
+  threads(nthreads);
+  render_cone<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr),
+                     cone_from(c), mkcam(cam), W, H, step, et, out, nullptr, stride);
+}
+// Exact occlusion / shadow and the surrogate's features at n samples: pos[3n] (texture
+// space), dir[3n] (camera ray directions), light[3]; outputs occ[n], sdw[n], feat[16n].
+void orc_cone_points(void* pyr, const ConeC* c, int n, const double* pos, const double* dir, double* occ,
+                     double* sdw, double* feat, int nthreads) {
+  threads(nthreads);
+  const Pyramid& P = *static_cast<Pyramid*>(pyr);
+  ConeParams cp = cone_from(c);
+  ConeTracer<double> ct(P, cp);
+  V3<double> L = normalize(V3<double>(cp.light_fwd[0], cp.light_fwd[1], cp.light_fwd[2]));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    V3<double> p(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    V3<double> d = normalize(V3<double>(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
+    V3<double> k, up, right;
+    occ_frame(d, k, up, right);
+    if (occ) occ[i] = ct.occlusion(p, k, up, right);
+    if (sdw) sdw[i] = ct.shadow(p);
+    if (feat) nn_features<double>(P, cp, p, k, L, feat + size_t(i) * kNFeat);
+  }
+}
+int orc_mlp_nparams() { return kNParams; }
+void orc_mlp_forward(const double* prm, int n, const double* X, double* Y) {
+  for (int i = 0; i < n; ++i) mlp_forward(prm, X + size_t(i) * kNFeat, Y + size_t(i) * kOut);
+}
+double orc_mlp_loss_grad(const double* prm, const double* X, const double* T, int n, double* grad) {
+  return mlp_loss_grad(prm, X, T, n, grad);
+}
+
 }  // extern "C"

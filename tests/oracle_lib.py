@@ -53,6 +53,22 @@ def lib():
         "orc_render_ebs_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
         "orc_render_ebs_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, f, f, vp, i]),
         "orc_ebs_point": (None, [i, i, i, d, d, d, vp, vp, d, d, d, vp, vp]),
+        "orc_pyr_new": (vp, [i, i, i, i, vp, d, d, d, vp, i, i, i, d, i, i]),
+        "orc_pyr_wrap": (vp, [d, d, d, i, vp, vp]),
+        "orc_pyr_free": (None, [vp]),
+        "orc_pyr_levels": (i, [vp]),
+        "orc_pyr_dims": (None, [vp, i, vp]),
+        "orc_pyr_copy": (None, [vp, i, vp, vp]),
+        "orc_pyr_level0_at": (d, [i, i, i, i, vp, d, d, d, vp, i, i, i, d, i, i, i]),
+        "orc_pyr_next_at": (d, [vp, i, i, i, i, i, i, d, d, d, d, i, i, i]),
+        "orc_gauss_ext": (d, [vp, d, d, d, d]),
+        "orc_cone_schedule": (None, [i, d, d, d, d, d, i, vp]),
+        "orc_render_cone_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_cone_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, f, f, vp, i, i]),
+        "orc_cone_points": (None, [vp, vp, i, vp, vp, vp, vp, vp, i]),
+        "orc_mlp_nparams": (i, []),
+        "orc_mlp_forward": (None, [vp, i, vp, vp]),
+        "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -216,3 +232,115 @@ def compare(img_gpu, img_ref, margin=None, tol=2e-3, stride=1):
         "n_straddle": int(strad.sum()),
         "n_pixels": int(diff.shape[0] * diff.shape[1]),
     }
+
+
+# ---------------------------------------------------------------------------- cones
+class ConeC(C.Structure):
+    _fields_ = [("n_steps", C.c_int), ("n_occ", C.c_int), ("apply_occ", C.c_int), ("apply_sdw", C.c_int),
+                ("occ_half_deg", C.c_double), ("ring_deg", C.c_double), ("sdw_half_deg", C.c_double),
+                ("t0", C.c_double), ("occ_max", C.c_double), ("sdw_max", C.c_double), ("r_sigma", C.c_double),
+                ("sigma0", C.c_double), ("ka", C.c_double), ("kd", C.c_double), ("light_fwd", C.c_double * 3)]
+
+
+def cone_params(grid=None, light=None, **kw) -> ConeC:
+    """Defaults of the cone-traced configuration: 8 occlusion cones of 30 degrees (ring at
+    60), one 10-degree shadow cone, 64 steps from t0 = 1 to 0.5 / 0.75 of the diagonal."""
+    c = ConeC()
+    c.n_steps, c.n_occ, c.apply_occ, c.apply_sdw = 64, 8, 1, 1
+    c.occ_half_deg, c.ring_deg, c.sdw_half_deg, c.t0 = 30.0, 60.0, 10.0, 1.0
+    c.r_sigma, c.sigma0, c.ka, c.kd = 2.0, 1.0, 0.5, 0.5
+    if grid is not None:
+        diag = float(np.float32(np.sqrt(np.sum(np.square(np.asarray(grid, np.float32))))))
+        c.occ_max, c.sdw_max = 0.5 * diag, 0.75 * diag
+    if light is not None:
+        c.light_fwd[:] = list(light.forward)
+    for k, v in kw.items():
+        if k == "light_fwd":
+            c.light_fwd[:] = list(v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class OPyr:
+    """Oracle extinction pyramid: built from a volume, or wrapping given levels."""
+
+    def __init__(self, h):
+        self.h = h
+        self.levels = lib().orc_pyr_levels(h)
+
+    @classmethod
+    def build(cls, vol, tf: OTF, base=None, sigma0=1.0, nlev=0, scale=(1, 1, 1), threads=0):
+        vol, va = _vol_args(vol, scale)
+        nz, ny, nx = vol.shape
+        bx, by, bz = base if base is not None else (nx, ny, nz)
+        return cls(lib().orc_pyr_new(*va, tf.h, bx, by, bz, sigma0, nlev, threads))
+
+    @classmethod
+    def wrap(cls, grid, levels):
+        """levels: list of (nz, ny, nx) float64 extinction arrays."""
+        dims = np.array([[a.shape[2], a.shape[1], a.shape[0]] for a in levels], np.int32).ravel()
+        cat = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64).ravel() for a in levels]))
+        return cls(lib().orc_pyr_wrap(*map(float, grid), len(levels), dims.ctypes.data, cat.ctypes.data))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_pyr_free(self.h)
+
+    def dims(self, l):
+        d = np.zeros(3, np.int32)
+        lib().orc_pyr_dims(self.h, l, d.ctypes.data)
+        return tuple(int(x) for x in d)
+
+    def level(self, l, opacity=False):
+        nx, ny, nz = self.dims(l)
+        tau = np.zeros((nz, ny, nx), np.float64)
+        op = np.zeros((nz, ny, nx), np.float64)
+        lib().orc_pyr_copy(self.h, l, tau.ctypes.data, op.ctypes.data)
+        return op if opacity else tau
+
+    def ext(self, pos, lod):
+        return lib().orc_gauss_ext(self.h, *map(float, pos), float(lod))
+
+
+def render_cone(vol, tf: OTF, pyr: OPyr, prm: ConeC, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1), fp32=False,
+                threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    if fp32:
+        out = np.zeros((H, W, 4), np.float32)
+        lib().orc_render_cone_f32(*va, tf.h, pyr.h, C.byref(prm), c.ctypes.data, W, H, step, et, out.ctypes.data,
+                                  threads, stride)
+        return out
+    out = np.zeros((H, W, 4), np.float64)
+    margin = np.zeros((H, W, 2), np.float64)
+    lib().orc_render_cone_f64(*va, tf.h, pyr.h, C.byref(prm), c.ctypes.data, W, H, step, et, out.ctypes.data,
+                              margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+def cone_points(pyr: OPyr, prm: ConeC, pos, dirs, threads=0):
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = len(pos)
+    occ, sdw, feat = np.zeros(n), np.zeros(n), np.zeros((n, 16))
+    lib().orc_cone_points(pyr.h, C.byref(prm), n, pos.ctypes.data, dirs.ctypes.data, occ.ctypes.data,
+                          sdw.ctypes.data, feat.ctypes.data, threads)
+    return occ, sdw, feat
+
+
+def mlp_forward(prm, X):
+    prm = np.ascontiguousarray(prm, np.float64)
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 16)
+    Y = np.zeros((len(X), 16))
+    lib().orc_mlp_forward(prm.ctypes.data, len(X), X.ctypes.data, Y.ctypes.data)
+    return Y
+
+
+def mlp_loss_grad(prm, X, T):
+    prm = np.ascontiguousarray(prm, np.float64)
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 16)
+    T = np.ascontiguousarray(T, np.float64).reshape(-1, 2)
+    g = np.zeros_like(prm)
+    loss = lib().orc_mlp_loss_grad(prm.ctypes.data, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
+    return loss, g

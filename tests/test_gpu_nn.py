@@ -1,0 +1,150 @@
+"""The learned occlusion/shadow surrogate (CVR_CONE_NN): bf16-MFMA inference and
+training kernels against the FP64 MLP of the oracle, and the trained renderer
+against the exact cone tracer it replaces."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cpp_volume_rendering_amd import native as N
+from cpp_volume_rendering_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    a = np.asarray(a, np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def random_net(seed, scale=0.25):
+    rng = np.random.default_rng(seed)
+    P = N.lib().cvr_nn_param_count()
+    w = rng.normal(0, scale, P).astype(np.float32)
+    return w
+
+
+def ref_forward_bf16(w, X):
+    """FP64 forward with the bf16 rounding points of the GPU: weights and layer inputs
+    rounded to bf16, products accumulated exactly, biases in fp32."""
+    lay = [(64, 16), (64, 64), (64, 64), (16, 64)]
+    o = 0
+    a = bf16(X)
+    for l, (out, inp) in enumerate(lay):
+        Wm = bf16(w[o:o + out * inp]).reshape(out, inp)
+        o += out * inp
+        b = w[o:o + out].astype(np.float64)
+        o += out
+        z = a @ Wm.T + b
+        a = bf16(np.maximum(z, 0)) if l < 3 else z
+    return a[:, :2]
+
+
+@pytest.fixture(scope="module")
+def nn_handle():
+    wl = W.cone_blobs(n=32, width=8, height=8)
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(wl.volume)
+    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    return r
+
+
+@pytest.mark.parametrize("path", [0, 1])
+def test_inference_matches_fp64(nn_handle, path):
+    w = random_net(1)
+    nn_handle.nn_set_weights(w)
+    X = np.random.default_rng(2).uniform(0, 1, (1000, 16)).astype(np.float32)
+    Y = nn_handle.nn_eval(X, path)
+    ref = ref_forward_bf16(w, X)
+    full = O.mlp_forward(w.astype(np.float64), X.astype(np.float64))[:, :2]
+    print(np.abs(Y - ref).max(), np.abs(Y - full).max())
+    assert np.abs(Y - ref).max() <= 2e-3 * (1 + np.abs(ref).max())
+    assert np.abs(Y - full).max() <= 3e-2 * (1 + np.abs(full).max())
+
+
+def test_register_path_equals_training_path(nn_handle):
+    nn_handle.nn_set_weights(random_net(3))
+    X = np.random.default_rng(4).uniform(0, 1, (777, 16)).astype(np.float32)
+    a, b = nn_handle.nn_eval(X, 0), nn_handle.nn_eval(X, 1)
+    assert np.abs(a - b).max() <= 1e-5
+
+
+def test_gradient_matches_fp64(nn_handle):
+    w = random_net(5, 0.2)
+    nn_handle.nn_set_weights(w)
+    rng = np.random.default_rng(6)
+    X = rng.uniform(0, 1, (1024, 16)).astype(np.float32)
+    T = rng.uniform(0, 1, (1024, 2)).astype(np.float32)
+    loss, g = nn_handle.nn_gradient(X, T)
+    rl, rg = O.mlp_loss_grad(w.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert loss == pytest.approx(rl, rel=3e-2)
+    offs = [0, 1024, 1088, 5184, 5248, 9344, 9408, 10432, 10448]
+    for a, b in zip(offs[:-1], offs[1:]):
+        ga, ra = g[a:b].astype(np.float64), rg[a:b]
+        rel = np.linalg.norm(ga - ra) / max(np.linalg.norm(ra), 1e-12)
+        cos = ga @ ra / max(np.linalg.norm(ga) * np.linalg.norm(ra), 1e-30)
+        print(a, b, rel, cos)
+        assert rel <= 5e-2 and cos >= 0.998, (a, b, rel, cos)
+
+
+def train_small(n=64, n_train=1 << 17, epochs=6, seed=21):
+    wl = W.cone_blobs(n=n, width=96, height=96, seed=seed)
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(wl.volume)
+    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    p = r.params()
+    p.n_train, p.n_val, p.epochs, p.batch = n_train, 1 << 14, epochs, 8192
+    p.cone.n_levels = 6
+    r.set_params(p)
+    r.nn_train()
+    return wl, r, p
+
+
+def test_training_beats_the_network_free_baseline():
+    wl, r, p = train_small()
+    rep = r.nn_report()
+    print(rep.train_loss, rep.val_rmse_occ, rep.val_rmse_sdw, rep.base_rmse_occ, rep.base_rmse_sdw)
+    assert rep.steps == (p.n_train // p.batch) * p.epochs
+    assert rep.val_rmse_occ < 0.05 and rep.val_rmse_sdw < 0.05
+    assert rep.val_rmse_occ < 0.7 * rep.base_rmse_occ
+    assert rep.val_rmse_sdw < 0.7 * rep.base_rmse_sdw
+    assert rep.exact_fallback == 0
+
+
+def test_surrogate_image_close_to_exact_tracer():
+    wl, r, p = train_small()
+    light = wl.light()
+    cam = wl.camera()
+    r.collect_stats(True)
+    img = r.render(cam, 96, 96, light=light)
+    st = r.stats()
+    ct = N.Renderer(N.CONE_TRACE)
+    ct.set_volume(wl.volume)
+    ct.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    ct.set_params(p.cone)
+    ref = ct.render(cam, 96, 96, light=light)
+    rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
+    print("rmse", rmse, "fallback", st.fallback_samples, "shaded", st.shaded_samples)
+    assert rmse <= 0.01
+    assert st.shaded_samples > 0 and st.fallback_samples <= 0.05 * st.shaded_samples
+    # forcing the exact tracer reproduces the cone-traced renderer (no retraining)
+    p.force_exact = 1
+    r.set_params(p)
+    exact = r.render(cam, 96, 96, light=light)
+    assert np.array_equal(exact, ref)
+
+
+def test_weights_roundtrip_is_a_checkpoint():
+    wl, r, p = train_small(n=32, n_train=1 << 15, epochs=2)
+    w = r.nn_weights()
+    cam, light = wl.camera(), wl.light()
+    a = r.render(cam, 48, 48, light=light)
+    r2 = N.Renderer(N.CONE_NN)
+    r2.set_volume(wl.volume)
+    r2.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    r2.set_params(p)
+    r2.nn_set_weights(w)
+    b = r2.render(cam, 48, 48, light=light)
+    assert np.array_equal(a, b) or np.abs(a - b).max() < 1e-6

@@ -1,0 +1,158 @@
+"""Pins the oracle's extinction pyramid, cone tracer, surrogate features and FP64 MLP
+against separable numpy convolutions, closed forms and finite differences (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+W7 = np.exp(-0.5 * np.arange(-3, 4) ** 2)
+W7 = W7 / W7.sum()
+
+
+def tf_opacity_numpy(vol, rgba, T=256):
+    """texture(TF_RGBA, v/255).a at voxel centres: GL_LINEAR at texel v/255*T - 1/2."""
+    d = vol.astype(np.float64) / 255.0
+    x = d * T - 0.5
+    i0 = np.floor(x)
+    f = x - i0
+    i0 = i0.astype(int)
+    a = rgba[np.clip(i0, 0, T - 1), 3].astype(np.float64)
+    b = rgba[np.clip(i0 + 1, 0, T - 1), 3].astype(np.float64)
+    return a * (1 - f) + b * f
+
+
+def conv_axis(a, axis, w, stride=1, pre_avg=False):
+    """Zero-padded 7-tap convolution along `axis`; with pre_avg each tap first averages the
+    pair (2m+2k, 2m+2k+1) of the finer level (clamp-to-edge) and outputs every 2nd cell."""
+    a = np.moveaxis(a, axis, -1)
+    n = a.shape[-1]
+    if not pre_avg:
+        out = np.zeros_like(a)
+        for k in range(-3, 4):
+            src = np.arange(n) + k
+            ok = (src >= 0) & (src < n)
+            out[..., ok] += w[k + 3] * a[..., src[ok]]
+        return np.moveaxis(out, -1, axis)
+    m = max(1, n // 2)
+    out = np.zeros(a.shape[:-1] + (m,))
+    for k in range(-3, 4):
+        pos = 2 * np.arange(m) + 1 + 2 * k  # centre of the tap in finer-level cells
+        inside = (pos >= 0) & (pos <= n)
+        i0 = np.clip(pos - 1, 0, n - 1)
+        i1 = np.clip(pos, 0, n - 1)
+        val = 0.5 * (a[..., i0] + a[..., i1])
+        out[..., inside] += w[k + 3] * val[..., inside]
+    return np.moveaxis(out, -1, axis)
+
+
+def test_pyramid_levels_equal_separable_convolutions():
+    rng = np.random.default_rng(21)
+    vol = rng.integers(0, 256, (12, 10, 16), dtype=np.uint8)  # (nz, ny, nx)
+    tf = O.OTF([[1, 1, 1, 0], [1, 1, 1, 255]], [[0.0, 0], [0.2, 90], [0.6, 200], [0.9, 255]])
+    rgba = tf.tables()[2]
+    pyr = O.OPyr.build(vol, tf, nlev=3)
+    assert pyr.levels == 3
+    op0 = tf_opacity_numpy(vol, rgba)
+    l0 = op0
+    for ax in range(3):
+        l0 = conv_axis(l0, ax, W7)
+    assert np.allclose(pyr.level(0, opacity=True), l0, atol=1e-12)
+    l1 = l0
+    for ax in range(3):
+        l1 = conv_axis(l1, ax, W7, pre_avg=True)
+    assert pyr.dims(1) == (8, 5, 6)
+    assert np.allclose(pyr.level(1, opacity=True), l1, atol=1e-12)
+    assert np.allclose(pyr.level(1), -np.log(1 - l1), atol=1e-12)
+
+
+def test_pyramid_constant_interior():
+    vol = np.full((40, 40, 40), 200, np.uint8)
+    tf = O.OTF([[1, 1, 1, 0], [1, 1, 1, 255]], [[0.4, 0], [0.4, 255]])
+    pyr = O.OPyr.build(vol, tf, nlev=2)
+    a = float(np.float32(0.4))
+    assert np.allclose(pyr.level(0, True)[3:-3, 3:-3, 3:-3], a, atol=1e-12)
+    assert pyr.level(0, True)[0, 0, 0] < a  # zero outside the volume pulls the border down
+    # level-1 taps reach +-6 world units: cells whose taps stay off the level-0 border band
+    assert np.allclose(pyr.level(1, True)[5:-5, 5:-5, 5:-5], a, atol=1e-12)
+
+
+def test_gauss_ext_lod_and_border_falloff():
+    L0 = np.full((8, 8, 8), 2.0)
+    L1 = np.full((4, 4, 4), 1.0)
+    pyr = O.OPyr.wrap((8, 8, 8), [L0, L1])
+    assert pyr.ext((4, 4, 4), 0) == pytest.approx(2.0)
+    assert pyr.ext((4, 4, 4), 0.25) == pytest.approx(1.75)
+    assert pyr.ext((4, 4, 4), 7.0) == pytest.approx(1.0)  # clamped to the last level
+    # outside: edge value times exp(-d^2 / (2 (2^lod)^2))
+    assert pyr.ext((11, 4, 4), 0) == pytest.approx(2.0 * math.exp(-9 / 2))
+    assert pyr.ext((4, -2, 12), 1) == pytest.approx(1.0 * math.exp(-(4 + 16) / 8))
+
+
+def test_cone_schedule_geometry():
+    out = np.zeros((64, 3))
+    O.lib().orc_cone_schedule(64, 1.0, 200.0, 30.0, 2.0, 1.0, 6, out.ctypes.data)
+    tm, dt, lod = out.T
+    assert dt.sum() == pytest.approx(199.0)
+    q = 200.0 ** (1 / 64)
+    assert np.allclose(np.diff(tm) / tm[:-1], q - 1, atol=1e-12)
+    assert lod.min() == 0 and lod.max() == 5
+    j = 40
+    assert lod[j] == pytest.approx(math.log2(tm[j] * math.tan(math.radians(30)) / 2.0))
+
+
+def test_cone_uniform_medium_closed_form():
+    n = 64
+    tau = 0.013
+    levels = [np.full((n >> i, n >> i, n >> i), tau) for i in range(6)]
+    pyr = O.OPyr.wrap((n, n, n), levels)
+    prm = O.cone_params(occ_max=25.0, sdw_max=30.0, light_fwd=(0.3, 0.8, -0.5))
+    pos = np.array([[32.0, 31.0, 33.0], [30.5, 32.2, 29.9]])
+    dirs = np.array([[0.2, -0.1, -1.0], [1.0, 0.3, 0.1]])
+    occ, sdw, feat = O.cone_points(pyr, prm, pos, dirs)
+    assert np.allclose(occ, math.exp(-tau * 24.0), rtol=1e-12)
+    assert np.allclose(sdw, math.exp(-tau * 29.0), rtol=1e-12)
+    # features: 7 segment opacities covering [t0, max] each way, cosine, local opacity
+    seg_o = 1 - np.exp(-tau * np.diff(25.0 ** (np.arange(8) / 7)))
+    seg_s = 1 - np.exp(-tau * np.diff(30.0 ** (np.arange(8) / 7)))
+    assert np.allclose(feat[:, :7], seg_o, rtol=1e-12)
+    assert np.allclose(feat[:, 7:14], seg_s, rtol=1e-12)
+    assert np.allclose(feat[:, 15], 1 - math.exp(-tau))
+    L = np.array([0.3, 0.8, -0.5]) / np.linalg.norm([0.3, 0.8, -0.5])
+    k = -dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+    assert np.allclose(feat[:, 14], (k @ L) * 0.5 + 0.5)
+
+
+def test_cone_renderer_without_medium_equals_plain():
+    """Zero extinction in the pyramid: occlusion = shadow = 1, so the shaded colour is the
+    TF colour and the image equals the plain ray caster's."""
+    import cpp_volume_rendering_amd.native as N
+    from cpp_volume_rendering_amd.workloads import plain_blobs
+    wl = plain_blobs(n=24, width=24, height=24, seed=4)
+    tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
+    pyr = O.OPyr.wrap(wl.grid, [np.zeros((24 >> i,) * 3) for i in range(4)])
+    prm = O.cone_params(wl.grid, wl.light())
+    cam = wl.camera()
+    a, _ = O.render_cone(wl.volume, tf, pyr, prm, cam, 24, 24)
+    b, _ = O.render_rc1p(wl.volume, tf, cam, 24, 24)
+    assert np.allclose(a, b, atol=1e-12)
+    assert b[..., 3].max() > 0.1
+
+
+def test_mlp_gradient_finite_differences():
+    rng = np.random.default_rng(3)
+    P = O.lib().orc_mlp_nparams()
+    assert P == 64 * 16 + 64 + 2 * (64 * 64 + 64) + 16 * 64 + 16
+    prm = rng.normal(0, 0.2, P)
+    X = rng.uniform(0, 1, (9, 16))
+    T = rng.uniform(0, 1, (9, 2))
+    loss, g = O.mlp_loss_grad(prm, X, T)
+    Y = O.mlp_forward(prm, X)
+    assert loss == pytest.approx(((Y[:, :2] - T) ** 2).sum() / (2 * 9), rel=1e-12)
+    assert np.all(g[-16 + 2:] == 0)  # padded outputs carry no gradient
+    for idx in rng.choice(P, 40, replace=False):
+        e = np.zeros(P)
+        e[idx] = 1e-6
+        fd = (O.mlp_loss_grad(prm + e, X, T)[0] - O.mlp_loss_grad(prm - e, X, T)[0]) / 2e-6
+        assert g[idx] == pytest.approx(fd, rel=1e-5, abs=1e-9)
