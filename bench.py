@@ -2,15 +2,19 @@
 
 python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
-One step = one rendered frame of the workload with the volume, transfer function
-and any preprocessing (SAT, pyramid, trained network) already resident in HBM.
-Prints one JSON line.  Only one GPU is used; --gpus above 1 is refused.
+Default workload (the headline metric): 512^3 uint8 (64 seeded Gaussian blobs plus
+4-octave value noise), 1024^2 image, directional occlusion + shadow.  One step = one
+frame of the learned-surrogate renderer (CVR_CONE_NN) with volume, pyramid and
+trained network resident in HBM; the exact cone-traced renderer (CVR_CONE_TRACE) is
+timed the same way beside it and judges the surrogate's image RMSE.  Other
+workloads: rc1p (128^3 plain caster at 512^2), ebs (256^3 extinction-based shading
+at 1024^2), cone (256^3 exact cone tracer at 1024^2).  Prints one JSON line.
+Only one GPU is used; --gpus above 1 is refused.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -21,64 +25,92 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "frames/s 512³@1024² DAO+shadow: neural vs cone-traced at RMSE≤0.01; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_SAMPLE = 2 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 16)  # 20480
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(kind: str, wl, cam, light, budget_s: float = 15.0):
-    """The oracle's FP32 instantiation on the host cores (all of them, OpenMP over rows)
-    over a bounded sample of the same frame: every k-th row, k chosen so one pass
-    costs a few seconds; frames/s is scaled from the fraction of rows rendered."""
+def time_frames(r, wl, cam, light, out, stream, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
+    e1.record(stream)
+    stream.synchronize()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def stats_frame(r, wl, cam, light, out, stream):
+    r.collect_stats(True)
+    r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
+    stream.synchronize()
+    st = r.stats()
+    r.collect_stats(False)
+    return st
+
+
+def cpu_baseline(kind, wl, cam, light, params=None, budget_s=20.0):
+    """The oracle's FP32 instantiation on all host cores (OpenMP over rows) on a bounded
+    sample of the same frame: every `stride`-th pixel in x and y, frames/s scaled by the
+    fraction of pixels rendered (stride^2)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import numpy as np
     import oracle_lib as O
 
-    L = O.lib()
-    threads = L.orc_max_threads()
+    threads = O.lib().orc_max_threads()
     tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
     W_, H_ = wl.width, wl.height
+    if kind == "rc1p":
+        t0 = time.perf_counter()
+        frames = 0
+        while True:
+            O.render_rc1p(wl.volume, tf, cam, W_, H_, fp32=True, threads=threads)
+            frames += 1
+            t = time.perf_counter() - t0
+            if t > budget_s or frames >= 50:
+                break
+        return {"value": frames / t, "unit": "frames/s", "cores": threads, "kind": "port",
+                "sample": f"{frames} full {W_}x{H_} frames, oracle FP32, OpenMP over rows"}
     if kind == "ebs":
         sat = O.sat(wl.volume, tf).astype(np.float32)
         prm = O.ebs_params(None, light, shadow_type=1)
         prm.sdw_max_dist = np.float32(0.75) * np.float32(np.linalg.norm(wl.grid))
-
-    def run(rows):
-        # render `rows` image rows spread over the frame (a camera with a shorter frame
-        # would change the rays; instead render the full-width rows with a row stride)
-        cam2 = cam
+        stride = 8
         t0 = time.perf_counter()
-        if kind == "rc1p":
-            O.render_rc1p(wl.volume, tf, cam2, W_, H_, fp32=True, threads=threads) if rows == H_ else None
-        return time.perf_counter() - t0
-
-    # sample: full frame for the plain caster when it fits the budget, else a row subset
-    t0 = time.perf_counter()
-    if kind == "rc1p":
-        O.render_rc1p(wl.volume, tf, cam, W_, H_, fp32=True, threads=threads)
-        frames = 1
+        O.render_ebs(wl.volume, tf, sat, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
         t = time.perf_counter() - t0
-        while t < budget_s and frames < 50:
-            O.render_rc1p(wl.volume, tf, cam, W_, H_, fp32=True, threads=threads)
-            frames += 1
-            t = time.perf_counter() - t0
-        return {"value": frames / t, "unit": "frames/s", "cores": threads, "kind": "port",
-                "sample": f"{frames} full {W_}x{H_} frames of the same workload, oracle FP32, OpenMP rows"}
-    # EBS: a 1/16 subset of pixels (every 4th pixel in x and y), scaled by 16
-    out = O.render_ebs(wl.volume, tf, sat, prm, cam, W_, H_, fp32=True, threads=threads)
+        return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
+                "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), oracle FP32 "
+                          f"with fp32 SAT, OpenMP over rows; scaled"}
+    # cone tracer: the exact tracer's FP32 port on the GPU-built pyramid (pyramid build not timed)
+    pyr = params["pyramid"]
+    prm = O.cone_params(wl.grid, light)
+    stride = params.get("stride", 32)
+    t0 = time.perf_counter()
+    O.render_cone(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
     t = time.perf_counter() - t0
-    return {"value": 1.0 / t, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"1 full {W_}x{H_} frame, oracle FP32 with fp32 SAT, OpenMP rows"}
+    return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), exact cone "
+                      f"tracer, oracle FP32, OpenMP over rows; scaled"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="rc1p", choices=["rc1p", "ebs"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="nn", choices=["nn", "cone", "rc1p", "ebs"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, default=0, help="override the volume size (tests)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -92,70 +124,95 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
+    t_setup = time.perf_counter()
+    extra = {}
     if args.workload == "rc1p":
-        wl = W.plain_blobs(n=128, width=512, height=512, seed=1)
+        wl = W.plain_blobs(n=args.size or 128, width=512, height=512, seed=1)
         tech, dtype, bpv = N.RC1P, "f32", 1
-    else:
-        wl = W.banded_noise(n=256, width=1024, height=1024, seed=2)
+    elif args.workload == "ebs":
+        wl = W.banded_noise(n=args.size or 256, width=1024, height=1024, seed=2)
         tech, dtype, bpv = N.EBS, "f32", 2
+    elif args.workload == "cone":
+        wl = W.cone_blobs(n=args.size or 256, width=1024, height=1024, seed=3)
+        tech, dtype, bpv = N.CONE_TRACE, "f32", 1
+    else:
+        wl = W.surrogate_blobs_noise(n=args.size or 512, width=1024, height=1024, seed=4)
+        tech, dtype, bpv = N.CONE_NN, "bf16", 1
+    log(f"workload {wl.name}: volume {wl.volume.shape} ready in {time.perf_counter() - t_setup:.1f}s")
     cam, light = wl.camera(), wl.light()
     r = N.Renderer(tech)
     r.set_volume(wl.volume, wl.scale)
     r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
     if tech == N.EBS:
         r.set_params(shadow_type=1)
+    t_prep = time.perf_counter()
     r.prepare(stream.cuda_stream)
+    stream.synchronize()
+    prep_s = time.perf_counter() - t_prep
+    log(f"prepare (pyramid / SAT / training) {prep_s:.1f}s")
     out = torch.empty((wl.height, wl.width, 4), dtype=torch.float32, device=dev)
 
-    # counters from one extra frame (not timed)
-    r.collect_stats(True)
-    r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
-    stream.synchronize()
-    st = r.stats()
-    r.collect_stats(False)
-
-    for _ in range(args.warmup):
-        r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
-    e1.record(stream)
-    stream.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
+    st = stats_frame(r, wl, cam, light, out, stream)
+    ms = time_frames(r, wl, cam, light, out, stream, args.steps, args.warmup)
     fps = 1000.0 / ms
+    log(f"{wl.name}: {ms:.3f} ms/frame, {fps:.2f} frames/s, samples {st.samples}, shaded {st.shaded_samples}, "
+        f"fallback {st.fallback_samples}")
 
     nz, ny, nx = wl.volume.shape
-    # algorithmic bytes per frame: every trilinear sample gathers 8 voxels, plus the frame
-    gathered = st.samples * 8 * bpv + wl.width * wl.height * 16
-    achieved = gathered / (ms * 1e-3) / 1e9
+    config = {"workload": wl.name, "volume": f"{nx}x{ny}x{nz} {'u8' if bpv == 1 else 'u16'}",
+              "image": f"{wl.width}x{wl.height}", "step": 0.5,
+              "samples_per_frame": int(st.samples), "shaded_samples_per_frame": int(st.shaded_samples)}
+    if tech == N.CONE_NN:
+        rep = r.nn_report()
+        p = r.params()
+        ct = N.Renderer(N.CONE_TRACE)
+        ct.set_volume(wl.volume, wl.scale)
+        ct.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        ct.set_params(p.cone)
+        ct.prepare(stream.cuda_stream)
+        out_ct = torch.empty_like(out)
+        st_ct = stats_frame(ct, wl, cam, light, out_ct, stream)
+        ms_ct = time_frames(ct, wl, cam, light, out_ct, stream, max(2, args.steps // 4), 1)
+        r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
+        stream.synchronize()
+        rmse = float(torch.sqrt(torch.mean((out - out_ct) ** 2)).item())
+        log(f"exact cone tracer: {ms_ct:.3f} ms/frame; surrogate speedup {ms_ct / ms:.2f}x, image RMSE {rmse:.5f}")
+        config.update({"pyramid_levels": p.cone.n_levels, "cones": "8 occlusion @30 deg + 1 shadow @10 deg, 64 steps",
+                       "mlp": "16-64-64-64-16 ReLU, bf16 MFMA, fp32 accumulate",
+                       "train_samples": p.n_train, "train_steps": rep.steps, "prepare_s": round(prep_s, 2),
+                       "fallback_samples_per_frame": int(st.fallback_samples),
+                       "val_rmse_occ": round(rep.val_rmse_occ, 5), "val_rmse_sdw": round(rep.val_rmse_sdw, 5),
+                       "baseline_rmse_occ": round(rep.base_rmse_occ, 5), "baseline_rmse_sdw": round(rep.base_rmse_sdw, 5),
+                       "exact_cone_fps": round(1000.0 / ms_ct, 4), "exact_cone_ms": round(ms_ct, 4),
+                       "speedup_vs_exact": round(ms_ct / ms, 3), "image_rmse_vs_exact": round(rmse, 6),
+                       "exact_gate_fallback": int(rep.exact_fallback)})
+        # roofline of the dominant kernel (the surrogate render): MLP flops per launch
+        flops = st.shaded_samples * MLP_FLOP_PER_SAMPLE
+        achieved = flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / BF16_PEAK_TFLOPS, 6), "traffic": None}
+    else:
+        # every trilinear sample gathers 8 voxels, plus the frame written once
+        gathered = st.samples * 8 * bpv + wl.width * wl.height * 16
+        achieved = gathered / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
     line = {
-        "metric": METRIC,
-        "value": round(fps, 4),
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype,
-        "data": "synthetic (seeded)",
-        "config": {"workload": wl.name, "volume": f"{nx}x{ny}x{nz} {'u8' if bpv == 1 else 'u16'}",
-                   "image": f"{wl.width}x{wl.height}", "step": 0.5,
-                   "samples_per_frame": int(st.samples), "shaded_samples_per_frame": int(st.shaded_samples)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
-        "wall_s": round(wall, 3),
+        "metric": METRIC, "value": round(fps, 4), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config, "roofline": roof,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.workload, wl, cam, light)
+        kind = {N.RC1P: "rc1p", N.EBS: "ebs"}.get(tech, "cone")
+        params = None
+        if kind == "cone":
+            sys.path.insert(0, str(ROOT / "tests"))
+            import oracle_lib as O
+            levels = (r if tech == N.CONE_TRACE else ct).pyramid()
+            params = {"pyramid": O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in levels]), "stride": 64}
+        t0 = time.perf_counter()
+        line["cpu_baseline"] = cpu_baseline(kind, wl, cam, light, params)
+        log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {line['cpu_baseline']}")
     print(json.dumps(line), flush=True)
 
 

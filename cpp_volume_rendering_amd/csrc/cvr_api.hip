@@ -635,6 +635,65 @@ int cvr_nn_get_report(cvr_handle h, cvr_nn_report* out) {
   return CVR_OK;
 }
 
+// Checkpoint layout: "CVRNNCK1" | u32 version | u32 nparams | w | m | v (fp32 each) |
+// lo[16] | hi[16] | cvr_nn_report.
+static const char kCkMagic[8] = {'C', 'V', 'R', 'N', 'N', 'C', 'K', '1'};
+size_t cvr_nn_checkpoint_size(void) {
+  return 8 + 4 + 4 + 3 * size_t(nn_param_count()) * 4 + 32 * 4 + sizeof(cvr_nn_report);
+}
+
+int cvr_nn_save(cvr_handle h, void* out, size_t bytes) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!out || bytes != cvr_nn_checkpoint_size()) { set_error("cvr_nn_save: buffer must be cvr_nn_checkpoint_size() bytes"); return CVR_ERR_INVALID; }
+  if (!h->d_nn_w || !h->nn_trained) { set_error("cvr_nn_save: no trained network"); return CVR_ERR_STATE; }
+  DeviceGuard dg(h->device);
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  char* p = static_cast<char*>(out);
+  const uint32_t ver = 1, np = uint32_t(nn_param_count());
+  std::memcpy(p, kCkMagic, 8); p += 8;
+  std::memcpy(p, &ver, 4); p += 4;
+  std::memcpy(p, &np, 4); p += 4;
+  for (float* d : {h->d_nn_w, h->d_nn_m, h->d_nn_v}) {
+    CVR_HIP_CHECK(hipMemcpy(p, d, size_t(np) * 4, hipMemcpyDeviceToHost));
+    p += size_t(np) * 4;
+  }
+  std::memcpy(p, h->nn_lo, 64); p += 64;
+  std::memcpy(p, h->nn_hi, 64); p += 64;
+  std::memcpy(p, &h->nn_report, sizeof(cvr_nn_report));
+  return CVR_OK;
+}
+
+int cvr_nn_load(cvr_handle h, const void* in, size_t bytes) {
+  int rc = nn_check(h);
+  if (rc) return rc;
+  if (!in || bytes != cvr_nn_checkpoint_size()) { set_error("cvr_nn_load: size mismatch"); return CVR_ERR_INVALID; }
+  const char* p = static_cast<const char*>(in);
+  uint32_t ver = 0, np = 0;
+  std::memcpy(&ver, p + 8, 4);
+  std::memcpy(&np, p + 12, 4);
+  if (std::memcmp(p, kCkMagic, 8) != 0 || ver != 1 || np != uint32_t(nn_param_count())) {
+    set_error("cvr_nn_load: not a version-1 surrogate checkpoint");
+    return CVR_ERR_INVALID;
+  }
+  p += 16;
+  DeviceGuard dg(h->device);
+  rc = nn_alloc(h);
+  if (rc) return rc;
+  for (float* d : {h->d_nn_w, h->d_nn_m, h->d_nn_v}) {
+    CVR_HIP_CHECK(hipMemcpy(d, p, size_t(np) * 4, hipMemcpyHostToDevice));
+    p += size_t(np) * 4;
+  }
+  std::memcpy(h->nn_lo, p, 64); p += 64;
+  std::memcpy(h->nn_hi, p, 64); p += 64;
+  std::memcpy(&h->nn_report, p, sizeof(cvr_nn_report));
+  rc = nn_pack(h, nullptr);
+  if (rc) return rc;
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  h->nn_trained = true;
+  return CVR_OK;
+}
+
 int cvr_get_stats(cvr_handle h, cvr_stats* out) {
   if (!h || !out) { set_error("cvr_get_stats: null argument"); return CVR_ERR_INVALID; }
   *out = h->stats;

@@ -18,6 +18,10 @@ void set_error(const std::string& msg);
     }                                                                                         \
   } while (0)
 
+// Upper bound on samples per camera ray: every march loop stops there even if a
+// caller passes a vanishing step size.
+constexpr int kMaxSamples = 1 << 21;
+
 // --------------------------------------------------------------------------- camera
 // Per-frame ray set-up, computed on the host in double and rounded once.
 // dir(px,py) = normalize(a*s + b*u + f) is the expansion of

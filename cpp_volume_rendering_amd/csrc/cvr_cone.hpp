@@ -15,7 +15,9 @@ struct StencilF {
   float dist, seg, lod, pad;
 };
 
-constexpr int kNFeat = 16, kNStencil = 7, kMaxSteps = 256;
+// Surrogate inputs: kNOcc occlusion segments read twice (on the axial cone, on the
+// ring), kNSdw shadow segments, the local opacity.
+constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7, kNStencil = 2 * kNOcc + kNSdw, kMaxSteps = 256;
 
 // Per-frame cone-tracer constants.
 struct ConeK {
@@ -24,7 +26,7 @@ struct ConeK {
   float L[3];               // shadow cone axis (towards the light)
   const ConeStepF* so;      // occlusion schedule [n_steps]
   const ConeStepF* ss;      // shadow schedule [n_steps]
-  const StencilF* st;       // surrogate stencil: occlusion [7] then shadow [7]
+  const StencilF* st;       // surrogate stencil: axial/ring pairs [2*kNOcc], then shadow [kNSdw]
 };
 
 // GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w).
@@ -100,19 +102,21 @@ __device__ __forceinline__ float ct_occlusion(const float* __restrict__ pyr, con
   return num / den;
 }
 
-// The surrogate's 16 inputs (see the CPU oracle's nn_features): 7 segment opacities
-// along the occlusion axis, 7 along the light, the light/view cosine, the local opacity.
+// The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
+// the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
+// opacities along the light (8..14), the local opacity (15).
 __device__ __forceinline__ void nn_feats(const float* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
                                          f3 p, f3 k, f3 L, float* f) {
 #pragma unroll
-  for (int i = 0; i < kNStencil; ++i) {
-    const StencilF so = st[i], ss = st[kNStencil + i];
-    const float to = gauss_ext(pyr, pg, p + k * so.dist, so.lod);
-    f[i] = 1.f - __expf(-to * so.seg);
-    const float ts = gauss_ext(pyr, pg, p + L * ss.dist, ss.lod);
-    f[kNStencil + i] = 1.f - __expf(-ts * ss.seg);
+  for (int i = 0; i < 2 * kNOcc; ++i) {
+    const StencilF so = st[i];
+    f[i] = 1.f - __expf(-gauss_ext(pyr, pg, p + k * so.dist, so.lod) * so.seg);
   }
-  f[14] = dot3(k, L) * 0.5f + 0.5f;
+#pragma unroll
+  for (int i = 0; i < kNSdw; ++i) {
+    const StencilF ss = st[2 * kNOcc + i];
+    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext(pyr, pg, p + L * ss.dist, ss.lod) * ss.seg);
+  }
   f[15] = 1.f - __expf(-gauss_ext(pyr, pg, p, 0.f));
 }
 

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
     const f3 L = ld3(K.L);
     const float D = fabsf(tf_ - tn);
     const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-    for (int j = 0;; ++j) {
+    for (int j = 0; j < kMaxSamples; ++j) {
       const float s = float(j) * step;
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
@@ -115,22 +115,27 @@ static void schedule(int n, double t0, double tmax, double half_deg, double r_si
   }
 }
 
-// The surrogate's stencil: 7 geometric segments over [t0, max] per axis.
+// The surrogate's stencil: kNOcc geometric segments over [t0, occ_max], each read on
+// the axis at distance D = sqrt(ab) (level of the axial cone's radius) and at D cos(ring)
+// (level of the ring's radius D sin(ring)); kNSdw segments over [t0, sdw_max] along the
+// light (level of the shadow cone's radius).
 static void stencil(const cvr_cone_params& P, double occ_max, double sdw_max, int levels, std::vector<StencilF>& st) {
-  st.resize(2 * kNStencil);
-  const double pi = 3.14159265358979323846;
-  const double se = std::sin(double(P.ring_deg) * pi / 180.0), ts = std::tan(double(P.sdw_half_deg) * pi / 180.0);
-  const double t0 = P.t0, rs = P.r_sigma, s0 = P.sigma0;
-  for (int i = 0; i < kNStencil; ++i) {
-    double a = t0 * std::pow(occ_max / t0, double(i) / kNStencil), b = t0 * std::pow(occ_max / t0, double(i + 1) / kNStencil);
-    double D = std::sqrt(a * b);
-    double lod = std::min(std::max(std::log2(D * se / (rs * s0)), 0.0), double(levels - 1));
-    st[i] = StencilF{float(0.6 * D), float(b - a), float(lod), 0.f};
-    a = t0 * std::pow(sdw_max / t0, double(i) / kNStencil);
-    b = t0 * std::pow(sdw_max / t0, double(i + 1) / kNStencil);
-    D = std::sqrt(a * b);
-    lod = std::min(std::max(std::log2(D * ts / (rs * s0)), 0.0), double(levels - 1));
-    st[kNStencil + i] = StencilF{float(D), float(b - a), float(lod), 0.f};
+  st.resize(kNStencil);
+  const double d2r = 3.14159265358979323846 / 180.0;
+  const double tha = std::tan(double(P.occ_half_deg) * d2r), cr = std::cos(double(P.ring_deg) * d2r);
+  const double sr = std::sin(double(P.ring_deg) * d2r), ts = std::tan(double(P.sdw_half_deg) * d2r);
+  const double t0 = P.t0, rs = double(P.r_sigma) * double(P.sigma0), top = double(levels - 1);
+  auto lvl = [&](double r) { return std::min(std::max(std::log2(r / rs), 0.0), top); };
+  for (int i = 0; i < kNOcc; ++i) {
+    const double a = t0 * std::pow(occ_max / t0, double(i) / kNOcc), b = t0 * std::pow(occ_max / t0, double(i + 1) / kNOcc);
+    const double D = std::sqrt(a * b);
+    st[2 * i] = StencilF{float(D), float(b - a), float(lvl(D * tha)), 0.f};
+    st[2 * i + 1] = StencilF{float(D * cr), float(b - a), float(lvl(D * sr)), 0.f};
+  }
+  for (int i = 0; i < kNSdw; ++i) {
+    const double a = t0 * std::pow(sdw_max / t0, double(i) / kNSdw), b = t0 * std::pow(sdw_max / t0, double(i + 1) / kNSdw);
+    const double D = std::sqrt(a * b);
+    st[2 * kNOcc + i] = StencilF{float(D), float(b - a), float(lvl(D * ts)), 0.f};
   }
 }
 
@@ -170,7 +175,7 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   std::vector<StencilF> stn;
   stencil(P, occ_max, sdw_max, c->pyr.levels, stn);
   const size_t bytes = (so.size() + ss.size()) * sizeof(ConeStepF) + stn.size() * sizeof(StencilF);
-  if (!c->d_cone_tab) CVR_HIP_CHECK(hipMalloc(&c->d_cone_tab, 2 * kMaxSteps * sizeof(ConeStepF) + 2 * kNStencil * sizeof(StencilF)));
+  if (!c->d_cone_tab) CVR_HIP_CHECK(hipMalloc(&c->d_cone_tab, 2 * kMaxSteps * sizeof(ConeStepF) + kNStencil * sizeof(StencilF)));
   std::vector<char> buf(bytes);
   std::memcpy(buf.data(), so.data(), so.size() * sizeof(ConeStepF));
   std::memcpy(buf.data() + so.size() * sizeof(ConeStepF), ss.data(), ss.size() * sizeof(ConeStepF));

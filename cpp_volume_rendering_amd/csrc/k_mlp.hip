@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
     float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active) {
       const float s = float(j) * step;
-      if (!(s < D)) {
+      if (!(s < D) || j >= kMaxSamples) {
         active = false;
       } else {
         h = fminf(step, D - s);
@@ -684,7 +684,8 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   const int B = std::min(N.batch, n);
   const int steps_per_epoch = n / B;
   const int total = steps_per_epoch * N.epochs;
-  const int chunks = 4;
+  // 64-sample groups per workgroup: enough workgroups to fill the 256 CUs
+  const int chunks = std::max(1, std::min(4, B / (64 * 256)));
   const size_t shm = kTrainLds;
   rc = train_kernel_attr();
   if (rc) { cleanup(); return rc; }
@@ -718,13 +719,17 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   }
   cleanup();
   if (rc) return rc;
+  // network-free baseline from the same lookups: transmittance products along the
+  // axial cone and the ring, weighted like the exact tracer's cones, and along the light
+  const cvr_cone_params& P = N.cone;
+  const double w_ring = double(P.n_occ_cones - 1) * std::cos(double(P.ring_deg) * 3.14159265358979323846 / 180.0);
   double se[2] = {0, 0}, sb[2] = {0, 0};
   for (int i = 0; i < nv; ++i) {
-    double base_o = 1.0, base_s = 1.0;
-    for (int s = 0; s < kNStencil; ++s) {
-      base_o *= 1.0 - Xv[size_t(i) * kNFeat + s];
-      base_s *= 1.0 - Xv[size_t(i) * kNFeat + kNStencil + s];
-    }
+    const float* x = &Xv[size_t(i) * kNFeat];
+    double ta = 1.0, tr = 1.0, base_s = 1.0;
+    for (int s = 0; s < kNOcc; ++s) { ta *= 1.0 - x[2 * s]; tr *= 1.0 - x[2 * s + 1]; }
+    for (int s = 0; s < kNSdw; ++s) base_s *= 1.0 - x[2 * kNOcc + s];
+    const double base_o = (ta + w_ring * tr) / (1.0 + w_ring);
     const double yo = std::min(std::max(double(Yv[i * 2]), 0.0), 1.0);
     const double ys = std::min(std::max(double(Yv[i * 2 + 1]), 0.0), 1.0);
     se[0] += (yo - Tv[i * 2]) * (yo - Tv[i * 2]);

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
   if (ray_box(eye, dir, g.G, tn, tf_)) {
     const float D = fabsf(tf_ - tn);
     const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-    for (int j = 0;; ++j) {
+    for (int j = 0; j < kMaxSamples; ++j) {
       const float s = float(j) * step;
       if (!(s < D)) break;
       const float h = fminf(step, D - s);

@@ -82,6 +82,7 @@ EXPORTS = [
     "cvr_render", "cvr_get_sat", "cvr_get_stats", "cvr_set_collect_stats",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
+    "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load",
     "cvr_synth_blobs", "cvr_synth_noise",
 ]
 
@@ -131,6 +132,9 @@ def lib() -> C.CDLL:
         "cvr_nn_eval": (i, [vp, i, vp, vp, i]),
         "cvr_nn_gradient": (i, [vp, i, vp, vp, vp, vp]),
         "cvr_nn_get_report": (i, [vp, C.POINTER(NnReport)]),
+        "cvr_nn_checkpoint_size": (sz, []),
+        "cvr_nn_save": (i, [vp, vp, sz]),
+        "cvr_nn_load": (i, [vp, vp, sz]),
         "cvr_synth_blobs": (i, [i, i, i, i, i, d, i, C.c_uint64, vp]),
         "cvr_synth_noise": (i, [i, i, i, i, i, C.c_uint64, i, vp, vp]),
     }
@@ -297,6 +301,15 @@ class Renderer:
         loss = C.c_float()
         check(lib().cvr_nn_gradient(self.h, len(X), X.ctypes.data, T.ctypes.data, g.ctypes.data, C.byref(loss)))
         return loss.value, g
+
+    def nn_save(self) -> bytes:
+        n = lib().cvr_nn_checkpoint_size()
+        buf = C.create_string_buffer(n)
+        check(lib().cvr_nn_save(self.h, buf, n))
+        return buf.raw
+
+    def nn_load(self, blob: bytes):
+        check(lib().cvr_nn_load(self.h, blob, len(blob)))
 
     def nn_report(self) -> NnReport:
         r = NnReport()

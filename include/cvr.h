@@ -218,6 +218,13 @@ int cvr_nn_eval(cvr_handle h, int n, const float* X, float* Y, int path);
 int cvr_nn_gradient(cvr_handle h, int n, const float* X, const float* T, float* grad, float* loss);
 int cvr_nn_get_report(cvr_handle h, cvr_nn_report* out);
 
+/* Checkpoint of a trained surrogate: a versioned binary holding the fp32 weights, the
+ * Adam moments, the training-set input range and the report, so a network is restored
+ * exactly (and training could resume).  cvr_nn_checkpoint_size gives the byte count. */
+size_t cvr_nn_checkpoint_size(void);
+int cvr_nn_save(cvr_handle h, void* out, size_t bytes);
+int cvr_nn_load(cvr_handle h, const void* in, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,30 +250,37 @@ inline void render_cone(const Volume& vol, const TF& tf, const Pyramid& P, const
 
 // ---------------------------------------------------------------- surrogate features
 // 16 inputs of the learned occlusion/shadow surrogate at sample position p with
-// occlusion axis k and light direction L: 7 opacities of segments along k and 7 along L
-// (geometric distances over the same ranges as the cone marches, each read from the
-// pyramid level whose sigma matches the spread there), the light/view cosine mapped to
-// [0,1], and the opacity of the level-0 extinction at p over one unit.
-constexpr int kNFeat = 16, kNStencil = 7;
+// occlusion axis k and light direction L.  Occlusion: 4 geometric segments over
+// [t0, occ_max]; for segment i (length b-a, mid distance D = sqrt(ab)) the opacity
+// 1 - exp(-tau * (b-a)) of the pyramid read on the axis at p + k D with the level of
+// the axial cone's radius D tan(half) (inputs 2i), and read at p + k D cos(ring) with
+// the level of the ring's radius D sin(ring) (inputs 2i+1).  Shadow: 7 segments over
+// [t0, sdw_max] along L with the level of the shadow cone's radius (inputs 8..14).
+// Input 15: the opacity of level 0 at p over one unit.
+constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7;
 template <class R>
 inline void nn_features(const Pyramid& P, const ConeParams& cp, const V3<R>& p, const V3<R>& k, const V3<R>& L,
                         R* f) {
-  const double se = std::sin(cp.ring_deg * kPi / 180.0), ts = std::tan(cp.sdw_half_deg * kPi / 180.0);
-  for (int i = 0; i < kNStencil; ++i) {
-    double a = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i) / kNStencil);
-    double b = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i + 1) / kNStencil);
-    double D = std::sqrt(a * b), lod = std::log2(D * se / (cp.r_sigma * cp.sigma0));
-    lod = std::min(std::max(lod, 0.0), double(P.levels - 1));
-    R t = gauss_ext<R>(P, p + k * R(0.6 * D), R(lod));
-    f[i] = R(1) - std::exp(-t * R(b - a));
-    a = cp.t0 * std::pow(cp.sdw_max / cp.t0, double(i) / kNStencil);
-    b = cp.t0 * std::pow(cp.sdw_max / cp.t0, double(i + 1) / kNStencil);
-    D = std::sqrt(a * b);
-    lod = std::min(std::max(std::log2(D * ts / (cp.r_sigma * cp.sigma0)), 0.0), double(P.levels - 1));
-    t = gauss_ext<R>(P, p + L * R(D), R(lod));
-    f[kNStencil + i] = R(1) - std::exp(-t * R(b - a));
+  const double d2r = kPi / 180.0;
+  const double tha = std::tan(cp.occ_half_deg * d2r), cr = std::cos(cp.ring_deg * d2r), sr = std::sin(cp.ring_deg * d2r);
+  const double ts = std::tan(cp.sdw_half_deg * d2r), rs = cp.r_sigma * cp.sigma0, top = double(P.levels - 1);
+  auto lvl = [&](double r) { return std::min(std::max(std::log2(r / rs), 0.0), top); };
+  for (int i = 0; i < kNOcc; ++i) {
+    double a = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i) / kNOcc);
+    double b = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i + 1) / kNOcc);
+    double D = std::sqrt(a * b);
+    R t = gauss_ext<R>(P, p + k * R(D), R(lvl(D * tha)));
+    f[2 * i] = R(1) - std::exp(-t * R(b - a));
+    t = gauss_ext<R>(P, p + k * R(D * cr), R(lvl(D * sr)));
+    f[2 * i + 1] = R(1) - std::exp(-t * R(b - a));
   }
-  f[14] = dot(k, L) * R(0.5) + R(0.5);
+  for (int i = 0; i < kNSdw; ++i) {
+    double a = cp.t0 * std::pow(cp.sdw_max / cp.t0, double(i) / kNSdw);
+    double b = cp.t0 * std::pow(cp.sdw_max / cp.t0, double(i + 1) / kNSdw);
+    double D = std::sqrt(a * b);
+    R t = gauss_ext<R>(P, p + L * R(D), R(lvl(D * ts)));
+    f[8 + i] = R(1) - std::exp(-t * R(b - a));
+  }
   f[15] = R(1) - std::exp(-gauss_ext<R>(P, p, R(0)));
 }
 

@@ -113,15 +113,14 @@ def test_cone_uniform_medium_closed_form():
     occ, sdw, feat = O.cone_points(pyr, prm, pos, dirs)
     assert np.allclose(occ, math.exp(-tau * 24.0), rtol=1e-12)
     assert np.allclose(sdw, math.exp(-tau * 29.0), rtol=1e-12)
-    # features: 7 segment opacities covering [t0, max] each way, cosine, local opacity
-    seg_o = 1 - np.exp(-tau * np.diff(25.0 ** (np.arange(8) / 7)))
+    # features: 4 occlusion segments over [t0, occ_max] (each read on the axial cone and
+    # on the ring), 7 shadow segments over [t0, sdw_max], the local opacity
+    seg_o = 1 - np.exp(-tau * np.diff(25.0 ** (np.arange(5) / 4)))
     seg_s = 1 - np.exp(-tau * np.diff(30.0 ** (np.arange(8) / 7)))
-    assert np.allclose(feat[:, :7], seg_o, rtol=1e-12)
-    assert np.allclose(feat[:, 7:14], seg_s, rtol=1e-12)
+    assert np.allclose(feat[:, 0:8:2], seg_o, rtol=1e-12)
+    assert np.allclose(feat[:, 1:8:2], seg_o, rtol=1e-12)
+    assert np.allclose(feat[:, 8:15], seg_s, rtol=1e-12)
     assert np.allclose(feat[:, 15], 1 - math.exp(-tau))
-    L = np.array([0.3, 0.8, -0.5]) / np.linalg.norm([0.3, 0.8, -0.5])
-    k = -dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
-    assert np.allclose(feat[:, 14], (k @ L) * 0.5 + 0.5)
 
 
 def test_cone_renderer_without_medium_equals_plain():
