@@ -49,29 +49,43 @@ __device__ __forceinline__ float pyr_tri(const float* __restrict__ pyr, const Py
   return c0 + fz * (c1 - c0);
 }
 
-// textureLod with LINEAR_MIPMAP_LINEAR, then the border falloff of
-// GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:92-112).
+// Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
+// exp(-|clamp(p, 0, G) - p|^2 / (2 (2^lod)^2)); the distance is 0 inside the box, so the
+// factor is exactly 1 there and no branch is needed (the loads of many lookups can then
+// be issued back to back).
+__device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float inv2sg2) {
+  const float cx = fminf(fmaxf(tp.x, 0.f), pg.G[0]) - tp.x;
+  const float cy = fminf(fmaxf(tp.y, 0.f), pg.G[1]) - tp.y;
+  const float cz = fminf(fmaxf(tp.z, 0.f), pg.G[2]) - tp.z;
+  return __expf(-(cx * cx + cy * cy + cz * cz) * inv2sg2);
+}
+
+// textureLod with LINEAR_MIPMAP_LINEAR (lerp of levels floor(lod) and floor(lod)+1,
+// the second clamped to the chain), then the border falloff.
 __device__ __forceinline__ float gauss_ext(const float* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
   const float q = float(pg.levels - 1);
   lod = fminf(fmaxf(lod, 0.f), q);
   const int d1 = int(floorf(lod));
+  const int d2 = min(d1 + 1, pg.levels - 1);
   const float fr = lod - float(d1);
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  float rg = pyr_tri(pyr, pg, d1, u, v, w);
-  if (fr > 0.f && d1 + 1 < pg.levels) rg = rg * (1.f - fr) + pyr_tri(pyr, pg, d1 + 1, u, v, w) * fr;
-  if (tp.x < 0.f || tp.x > pg.G[0] || tp.y < 0.f || tp.y > pg.G[1] || tp.z < 0.f || tp.z > pg.G[2]) {
-    const float sg = exp2f(lod);
-    const float cx = fminf(fmaxf(tp.x, 0.f), pg.G[0]) - tp.x;
-    const float cy = fminf(fmaxf(tp.y, 0.f), pg.G[1]) - tp.y;
-    const float cz = fminf(fmaxf(tp.z, 0.f), pg.G[2]) - tp.z;
-    rg = rg * __expf(-(cx * cx + cy * cy + cz * cz) / (2.0f * sg * sg));
-  }
-  return rg;
+  const float a = pyr_tri(pyr, pg, d1, u, v, w);
+  const float b = pyr_tri(pyr, pg, d2, u, v, w);
+  const float sg = exp2f(lod);
+  return (a * (1.f - fr) + b * fr) * border_falloff(pg, tp, 0.5f / (sg * sg));
+}
+
+// The same lookup at a whole level `l` (one trilinear read).
+__device__ __forceinline__ float gauss_ext_level(const float* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
+  const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
+  const float sg = float(1 << l);
+  return pyr_tri(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
 __device__ __forceinline__ float cone_T(const float* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
                                         int n, f3 p, f3 axis) {
   float od = 0.f;
+#pragma unroll 4
   for (int j = 0; j < n; ++j) {
     const ConeStepF c = s[j];
     od += gauss_ext(pyr, pg, p + axis * c.tm, c.lod) * c.dt;
@@ -110,14 +124,27 @@ __device__ __forceinline__ void nn_feats(const float* __restrict__ pyr, const Py
 #pragma unroll
   for (int i = 0; i < 2 * kNOcc; ++i) {
     const StencilF so = st[i];
-    f[i] = 1.f - __expf(-gauss_ext(pyr, pg, p + k * so.dist, so.lod) * so.seg);
+    f[i] = 1.f - __expf(-gauss_ext_level(pyr, pg, p + k * so.dist, int(so.lod)) * so.seg);
   }
 #pragma unroll
   for (int i = 0; i < kNSdw; ++i) {
     const StencilF ss = st[2 * kNOcc + i];
-    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext(pyr, pg, p + L * ss.dist, ss.lod) * ss.seg);
+    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext_level(pyr, pg, p + L * ss.dist, int(ss.lod)) * ss.seg);
   }
-  f[15] = 1.f - __expf(-gauss_ext(pyr, pg, p, 0.f));
+  f[15] = 1.f - __expf(-gauss_ext_level(pyr, pg, p, 0));
+}
+
+// Network-free estimate from the same inputs (the network predicts the residual):
+// transmittance products along the axial cone and the ring, weighted like the exact
+// tracer's cones (ring weight w_ring = (n_occ-1) cos(ring)), and along the light.
+__device__ __forceinline__ void nn_base(const float* f, float w_ring, float& occ, float& sdw) {
+  float ta = 1.f, tr = 1.f, ts = 1.f;
+#pragma unroll
+  for (int i = 0; i < kNOcc; ++i) { ta *= 1.f - f[2 * i]; tr *= 1.f - f[2 * i + 1]; }
+#pragma unroll
+  for (int i = 0; i < kNSdw; ++i) ts *= 1.f - f[2 * kNOcc + i];
+  occ = (ta + w_ring * tr) / (1.f + w_ring);
+  sdw = ts;
 }
 
 }  // namespace cvr

@@ -118,14 +118,15 @@ static void schedule(int n, double t0, double tmax, double half_deg, double r_si
 // The surrogate's stencil: kNOcc geometric segments over [t0, occ_max], each read on
 // the axis at distance D = sqrt(ab) (level of the axial cone's radius) and at D cos(ring)
 // (level of the ring's radius D sin(ring)); kNSdw segments over [t0, sdw_max] along the
-// light (level of the shadow cone's radius).
+// light (level of the shadow cone's radius).  Levels are rounded to whole levels, so
+// each input costs one trilinear read.
 static void stencil(const cvr_cone_params& P, double occ_max, double sdw_max, int levels, std::vector<StencilF>& st) {
   st.resize(kNStencil);
   const double d2r = 3.14159265358979323846 / 180.0;
   const double tha = std::tan(double(P.occ_half_deg) * d2r), cr = std::cos(double(P.ring_deg) * d2r);
   const double sr = std::sin(double(P.ring_deg) * d2r), ts = std::tan(double(P.sdw_half_deg) * d2r);
   const double t0 = P.t0, rs = double(P.r_sigma) * double(P.sigma0), top = double(levels - 1);
-  auto lvl = [&](double r) { return std::min(std::max(std::log2(r / rs), 0.0), top); };
+  auto lvl = [&](double r) { return std::min(std::max(std::round(std::log2(r / rs)), 0.0), top); };
   for (int i = 0; i < kNOcc; ++i) {
     const double a = t0 * std::pow(occ_max / t0, double(i) / kNOcc), b = t0 * std::pow(occ_max / t0, double(i + 1) / kNOcc);
     const double D = std::sqrt(a * b);

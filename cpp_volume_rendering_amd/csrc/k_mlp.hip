@@ -36,6 +36,7 @@ constexpr int kTiles1 = 4, kTiles2 = 16, kTiles3 = 16, kTiles4 = 4;
 constexpr int kNTiles = kTiles1 + kTiles2 + kTiles3 + kTiles4;
 constexpr int kPackShorts = kNTiles * 64 * 4;
 constexpr int kNBias = kH + kH + kH + kOut;
+constexpr int kWaveScratch = kNFeat * 64 + 2 * 64;
 
 // fp32 -> bf16, round to nearest even (inputs are finite)
 __device__ __forceinline__ short f2bf(float x) {
@@ -231,7 +232,8 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, short* __restrict__ wp
 
 // Wavefront-cooperative inference: every lane brings one sample's 16 features; the
 // 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
-// and biases in LDS; sX: this wave's 16x64 LDS scratch.  Returns outputs 0 and 1.
+// and biases in LDS; sX: this wave's LDS scratch of kWaveScratch floats (16x64 inputs,
+// then 64x2 outputs).  Returns outputs 0 and 1.
 __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const float* __restrict__ sB, float* sX,
                                          const float* f, float& o0, float& o1) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
@@ -241,13 +243,15 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const bf16x4_t* W = reinterpret_cast<const bf16x4_t*>(sW);
-  bf16x4_t hb[4][4];  // [k tile][n tile]
-  // layer 1: K = 16
-#pragma unroll
+  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
+  // Sample tiles are independent through the network: each 16-sample column tile goes
+  // through all four layers before the next, so only its 4 activation tiles are live.
+#pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
-    bf16x4_t xb;
+    bf16x4_t xb, hb[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) xb[e] = f2bf(sX[(4 * q + e) * 64 + nt * 16 + r]);
+    // layer 1: K = 16
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       f32x4_t acc;
@@ -255,59 +259,45 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
       for (int e = 0; e < 4; ++e) acc[e] = sB[mt * 16 + 4 * q + e];
       acc = mfma16(W[mt * 64 + lane], xb, acc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) hb[mt][nt][e] = f2bf(fmaxf(acc[e], 0.f));
+      for (int e = 0; e < 4; ++e) hb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
     }
-  }
-  // layers 2 and 3: K = 64
+    // layers 2 and 3: K = 64
 #pragma unroll
-  for (int l = 0; l < 2; ++l) {
-    const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
-    const float* bl = sB + kH * (l + 1);
-    bf16x4_t nb[4][4];
+    for (int l = 0; l < 2; ++l) {
+      const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
+      const float* bl = sB + kH * (l + 1);
+      bf16x4_t nb[4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
+      for (int mt = 0; mt < 4; ++mt) {
         f32x4_t acc;
 #pragma unroll
         for (int e = 0; e < 4; ++e) acc[e] = bl[mt * 16 + 4 * q + e];
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt][nt], acc);
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt], acc);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) nb[mt][nt][e] = f2bf(fmaxf(acc[e], 0.f));
+        for (int e = 0; e < 4; ++e) nb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
       }
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) hb[a][b] = nb[a][b];
-  }
-  // layer 4: 16 outputs (rows 0, 1 used)
-  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
-  float y0[4], y1[4];
-#pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
+      for (int mt = 0; mt < 4; ++mt) hb[mt] = nb[mt];
+    }
+    // layer 4: 16 outputs (rows 0, 1 used)
     f32x4_t acc;
 #pragma unroll
     for (int e = 0; e < 4; ++e) acc[e] = sB[3 * kH + 4 * q + e];
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt][nt], acc);
-    y0[nt] = acc[0];
-    y1[nt] = acc[1];
-  }
-  // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r; hand them back
-  __builtin_amdgcn_wave_barrier();
-  if (q == 0) {
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      sX[(nt * 16 + r) * 2] = y0[nt];
-      sX[(nt * 16 + r) * 2 + 1] = y1[nt];
+    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt], acc);
+    // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r
+    if (q == 0) {
+      sX[kNFeat * 64 + (nt * 16 + r) * 2] = acc[0];
+      sX[kNFeat * 64 + (nt * 16 + r) * 2 + 1] = acc[1];
     }
   }
+  // lanes 0..15 wrote outputs 0 and 1 of every sample; hand them back
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  o0 = sX[lane * 2];
-  o1 = sX[lane * 2 + 1];
+  o0 = sX[kNFeat * 64 + lane * 2];
+  o1 = sX[kNFeat * 64 + lane * 2 + 1];
   __builtin_amdgcn_wave_barrier();
 }
 
@@ -316,7 +306,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const short* __restrict__ wpa
                                                    const float* __restrict__ X, int n, float* __restrict__ Y) {
   __shared__ short sW[kPackShorts];
   __shared__ float sB[kNBias];
-  __shared__ float sX[4][16 * 64];
+  __shared__ float sX[4][kWaveScratch];
   for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
     reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
   for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
@@ -352,7 +342,7 @@ __device__ __forceinline__ f3 unit_sphere(uint64_t& s) {
 // Sample i: a position drawn uniformly in the volume box until the transfer function
 // gives it nonzero extinction (the samples the renderer shades), a view direction and
 // a light direction uniform on the sphere; outputs the 16 features and the exact
-// (occlusion, shadow) of the cone tracer.
+// (occlusion, shadow) of the cone tracer minus the network-free estimate.
 template <typename T>
 __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const float* __restrict__ pyr, ConeK K,
                                                     const float4* __restrict__ tf, int ntf, uint64_t seed, int n,
@@ -375,8 +365,11 @@ __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const
   float f[kNFeat];
   nn_feats(pyr, pg, K.st, p, k, L, f);
   for (int c = 0; c < kNFeat; ++c) X[size_t(i) * kNFeat + c] = f[c];
-  Tg[size_t(i) * 2] = ct_occlusion(pyr, pg, K, p, k, up, right);
-  Tg[size_t(i) * 2 + 1] = cone_T(pyr, pg, K.ss, K.n_steps, p, L);
+  float bo, bs;
+  nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
+  // targets are the residuals of the exact tracer to the network-free estimate
+  Tg[size_t(i) * 2] = ct_occlusion(pyr, pg, K, p, k, up, right) - bo;
+  Tg[size_t(i) * 2 + 1] = cone_T(pyr, pg, K.ss, K.n_steps, p, L) - bs;
   if (pos_out) {
     pos_out[size_t(i) * 6 + 0] = p.x; pos_out[size_t(i) * 6 + 1] = p.y; pos_out[size_t(i) * 6 + 2] = p.z;
     pos_out[size_t(i) * 6 + 3] = dir.x; pos_out[size_t(i) * 6 + 4] = dir.y; pos_out[size_t(i) * 6 + 5] = dir.z;
@@ -417,7 +410,7 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
                                                    unsigned long long* __restrict__ counters) {
   __shared__ short sW[kPackShorts];
   __shared__ float sB[kNBias];
-  __shared__ float sX[4][16 * 64];
+  __shared__ float sX[4][kWaveScratch];
   extern __shared__ float4 s_tf[];
   for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
     reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
@@ -481,6 +474,10 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
       float Io, Is;
       mlp_wave(sW, sB, sX[w], f, Io, Is);
       if (need) {
+        float bo, bs;
+        nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
+        Io += bo;
+        Is += bs;
         ++nshade;
         if (ood || nk.exact_all) {
           ++nfall;
@@ -730,12 +727,13 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
     for (int s = 0; s < kNOcc; ++s) { ta *= 1.0 - x[2 * s]; tr *= 1.0 - x[2 * s + 1]; }
     for (int s = 0; s < kNSdw; ++s) base_s *= 1.0 - x[2 * kNOcc + s];
     const double base_o = (ta + w_ring * tr) / (1.0 + w_ring);
-    const double yo = std::min(std::max(double(Yv[i * 2]), 0.0), 1.0);
-    const double ys = std::min(std::max(double(Yv[i * 2 + 1]), 0.0), 1.0);
-    se[0] += (yo - Tv[i * 2]) * (yo - Tv[i * 2]);
-    se[1] += (ys - Tv[i * 2 + 1]) * (ys - Tv[i * 2 + 1]);
-    sb[0] += (base_o - Tv[i * 2]) * (base_o - Tv[i * 2]);
-    sb[1] += (base_s - Tv[i * 2 + 1]) * (base_s - Tv[i * 2 + 1]);
+    const double to = double(Tv[i * 2]) + base_o, ts = double(Tv[i * 2 + 1]) + base_s;  // exact values
+    const double yo = std::min(std::max(double(Yv[i * 2]) + base_o, 0.0), 1.0);
+    const double ys = std::min(std::max(double(Yv[i * 2 + 1]) + base_s, 0.0), 1.0);
+    se[0] += (yo - to) * (yo - to);
+    se[1] += (ys - ts) * (ys - ts);
+    sb[0] += (base_o - to) * (base_o - to);
+    sb[1] += (base_s - ts) * (base_s - ts);
   }
   c->nn_report.train_loss = last_loss;
   c->nn_report.val_rmse_occ = std::sqrt(se[0] / nv);

@@ -252,11 +252,12 @@ inline void render_cone(const Volume& vol, const TF& tf, const Pyramid& P, const
 // 16 inputs of the learned occlusion/shadow surrogate at sample position p with
 // occlusion axis k and light direction L.  Occlusion: 4 geometric segments over
 // [t0, occ_max]; for segment i (length b-a, mid distance D = sqrt(ab)) the opacity
-// 1 - exp(-tau * (b-a)) of the pyramid read on the axis at p + k D with the level of
-// the axial cone's radius D tan(half) (inputs 2i), and read at p + k D cos(ring) with
+// 1 - exp(-tau * (b-a)) of the pyramid read on the axis at p + k D at the level of
+// the axial cone's radius D tan(half) (inputs 2i), and read at p + k D cos(ring) at
 // the level of the ring's radius D sin(ring) (inputs 2i+1).  Shadow: 7 segments over
-// [t0, sdw_max] along L with the level of the shadow cone's radius (inputs 8..14).
-// Input 15: the opacity of level 0 at p over one unit.
+// [t0, sdw_max] along L at the level of the shadow cone's radius (inputs 8..14).
+// Input 15: the opacity of level 0 at p over one unit.  Levels are rounded to whole
+// levels (one trilinear read per input).
 constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7;
 template <class R>
 inline void nn_features(const Pyramid& P, const ConeParams& cp, const V3<R>& p, const V3<R>& k, const V3<R>& L,
@@ -264,7 +265,7 @@ inline void nn_features(const Pyramid& P, const ConeParams& cp, const V3<R>& p, 
   const double d2r = kPi / 180.0;
   const double tha = std::tan(cp.occ_half_deg * d2r), cr = std::cos(cp.ring_deg * d2r), sr = std::sin(cp.ring_deg * d2r);
   const double ts = std::tan(cp.sdw_half_deg * d2r), rs = cp.r_sigma * cp.sigma0, top = double(P.levels - 1);
-  auto lvl = [&](double r) { return std::min(std::max(std::log2(r / rs), 0.0), top); };
+  auto lvl = [&](double r) { return std::min(std::max(std::round(std::log2(r / rs)), 0.0), top); };
   for (int i = 0; i < kNOcc; ++i) {
     double a = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i) / kNOcc);
     double b = cp.t0 * std::pow(cp.occ_max / cp.t0, double(i + 1) / kNOcc);
@@ -282,6 +283,18 @@ inline void nn_features(const Pyramid& P, const ConeParams& cp, const V3<R>& p, 
     f[8 + i] = R(1) - std::exp(-t * R(b - a));
   }
   f[15] = R(1) - std::exp(-gauss_ext<R>(P, p, R(0)));
+}
+
+// Network-free estimate from the same inputs: transmittance products along the axial
+// cone and along the ring, weighted like the exact tracer's cones, and along the light.
+// The network predicts the residual to it.
+inline void nn_baseline(const ConeParams& cp, const double* f, double& occ, double& sdw) {
+  double ta = 1, tr = 1, ts = 1;
+  for (int i = 0; i < kNOcc; ++i) { ta *= 1 - f[2 * i]; tr *= 1 - f[2 * i + 1]; }
+  for (int i = 0; i < kNSdw; ++i) ts *= 1 - f[2 * kNOcc + i];
+  const double wr = (cp.n_occ - 1) * std::cos(cp.ring_deg * kPi / 180.0);
+  occ = (ta + wr * tr) / (1 + wr);
+  sdw = ts;
 }
 
 // ---------------------------------------------------------------- MLP (FP64)
