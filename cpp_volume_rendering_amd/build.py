@@ -2,7 +2,9 @@
 
 Every ``csrc/*.hip`` and ``csrc/host/*.cpp`` file is compiled by ``hipcc`` into an
 object under ``csrc/_build/`` and linked in-tree into
-``cpp_volume_rendering_amd/libcvr.so``.  Run ``python -m cpp_volume_rendering_amd.build``.
+``cpp_volume_rendering_amd/libcvr.so``; ``csrc/app/cvr_cli.cpp`` becomes the headless
+application ``cpp_volume_rendering_amd/cvr_cli``.  Run
+``python -m cpp_volume_rendering_amd.build``.
 """
 from __future__ import annotations
 
@@ -17,6 +19,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OBJ = CSRC / "_build"
 LIB = PKG / "libcvr.so"
+CLI = PKG / "cvr_cli"
 INCLUDE = PKG.parent / "include"
 
 ARCH = os.environ.get("CVR_OFFLOAD_ARCH", "gfx950")
@@ -70,6 +73,15 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    # the headless application, linked against the library next to it
+    app = CSRC / "app" / "cvr_cli.cpp"
+    cmd = [HIPCC, *COMMON, str(app), "-o", str(CLI) + ".tmp", f"-L{PKG}", "-lcvr", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cvr_cli build failed:\n" + r.stderr)
+    os.replace(str(CLI) + ".tmp", CLI)
     return LIB
 
 

@@ -1,0 +1,151 @@
+// Headless plugin surface mirroring the reference's BaseVolumeRenderer
+// (cppvolrend/volrenderbase.h:25-98) and RenderingManager
+// (cppvolrend/renderingmanager.h:29-167).  Each technique class is a thin shim over
+// one cvr_handle of the C ABI; the only interface change forced by dropping OpenGL is
+// GetScreenTextureID() -> GetFrame() (a host RGBA float buffer, row 0 at the bottom).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "vis.h"
+
+class BaseVolumeRenderer {
+ public:
+  BaseVolumeRenderer();
+  virtual ~BaseVolumeRenderer();
+
+  // volrenderbase.cpp:20-24: non-owning external resources
+  void SetExternalResources(vis::DataManager* data_mgr, vis::RenderingParameters* rdr_prm);
+
+  virtual const char* GetName() = 0;
+  virtual const char* GetAbbreviationName() = 0;
+
+  virtual void Clean();
+  virtual void ReloadShaders() {}
+  virtual bool Init(int shader_width, int shader_height) = 0;
+  virtual bool Update(vis::Camera* camera) = 0;
+  virtual void Redraw() = 0;
+  virtual void Reshape(int w, int h);
+  virtual vis::GRID_VOLUME_DATA_TYPE GetDataTypeSupport() { return vis::GRID_VOLUME_DATA_TYPE::STRUCTURED; }
+
+  // volrenderbase.cpp:77-84
+  void PrepareRender(vis::Camera* camera);
+  virtual void SetOutdated() { vr_outdated = true; }
+  bool IsOutdated() const { return vr_outdated; }
+  bool IsBuilt() const { return vr_built; }
+
+  // The rendered frame: width * height premultiplied RGBA floats, row 0 at the bottom.
+  const std::vector<float>& GetFrame() const { return m_frame; }
+  int GetFrameWidth() const { return m_width; }
+  int GetFrameHeight() const { return m_height; }
+  // Status of the last failing call of the C ABI (empty when none).
+  const std::string& GetLastError() const { return m_error; }
+
+ protected:
+  void SetBuilt(bool b) { vr_built = b; }
+
+  bool vr_built = false, vr_outdated = true;
+  vis::DataManager* m_ext_data_manager = nullptr;
+  vis::RenderingParameters* m_ext_rendering_parameters = nullptr;
+  int m_width = 0, m_height = 0;
+  std::vector<float> m_frame;
+  std::string m_error;
+};
+
+// One technique of the C ABI behind the plugin surface.
+class GpuVolumeRenderer : public BaseVolumeRenderer {
+ public:
+  explicit GpuVolumeRenderer(int technique, int device = 0);
+  ~GpuVolumeRenderer() override;
+  void Clean() override;
+  bool Init(int shader_width, int shader_height) override;
+  bool Update(vis::Camera* camera) override;
+  void Redraw() override;
+  // Technique parameters (the cvr_*_params POD of this technique).
+  bool SetParams(const void* params, size_t bytes);
+  bool GetParams(void* params, size_t bytes);
+  cvr_handle Handle() const { return m_handle; }
+
+ protected:
+  bool Check(int rc);
+  int m_technique, m_device;
+  cvr_handle m_handle = nullptr;
+  cvr_camera m_cam{};
+  cvr_light m_light{};
+};
+
+// RayCasting1Pass (rc1prenderer.h:32-78)
+class RayCasting1Pass : public GpuVolumeRenderer {
+ public:
+  RayCasting1Pass() : GpuVolumeRenderer(CVR_RC1P) {}
+  const char* GetName() override { return "1-Pass - Ray Casting"; }
+  const char* GetAbbreviationName() override { return "s_1rc"; }
+};
+
+// RC1PExtinctionBasedShading (ebsrenderer.h:45-119)
+class RC1PExtinctionBasedShading : public GpuVolumeRenderer {
+ public:
+  RC1PExtinctionBasedShading() : GpuVolumeRenderer(CVR_EBS) {}
+  const char* GetName() override { return "1-Pass - Ray Casting - Extinction-based"; }
+  const char* GetAbbreviationName() override { return "s_1rc_eb"; }
+};
+
+// Cone-traced directional occlusion + shadows over the Gaussian extinction pyramid
+// (the pyramid and shading of RC1PConeTracingDirOcclusionShading, dosrcrenderer.h:38-121)
+class RC1PConeTracedOcclusion : public GpuVolumeRenderer {
+ public:
+  RC1PConeTracedOcclusion() : GpuVolumeRenderer(CVR_CONE_TRACE) {}
+  const char* GetName() override { return "1-Pass - Ray Casting - Cone-traced Directional Occlusion"; }
+  const char* GetAbbreviationName() override { return "s_1rc_ct"; }
+};
+
+// The same renderer with the learned surrogate (no reference counterpart).
+class RC1PLearnedConeOcclusion : public GpuVolumeRenderer {
+ public:
+  RC1PLearnedConeOcclusion() : GpuVolumeRenderer(CVR_CONE_NN) {}
+  const char* GetName() override { return "1-Pass - Ray Casting - Learned Cone Occlusion"; }
+  const char* GetAbbreviationName() override { return "s_1rc_nn"; }
+};
+
+// Headless RenderingManager: owns the renderers (renderingmanager.cpp:119-123,
+// 372-379), the data manager, camera and rendering parameters; Display() drives one
+// frame (:174-319); EvaluateFrames() is the eval.csv harness (:261-317, 807-868).
+class RenderingManager {
+ public:
+  static RenderingManager* Instance();
+  static void DestroyInstance();
+
+  void AddVolumeRenderer(BaseVolumeRenderer* vr);  // takes ownership
+  int GetNumberOfVolumeRenderers() const { return int(m_renderers.size()); }
+  BaseVolumeRenderer* GetVolumeRenderer(int i) const { return m_renderers[i]; }
+  // renderingmanager.cpp:521-540; false if the technique cannot handle the data
+  bool SetCurrentVolumeRenderer(int id);
+  BaseVolumeRenderer* GetCurrentVolumeRenderer() const { return m_current; }
+
+  // renderingmanager.cpp:126-172 with the volume and transfer function given directly.
+  bool InitData(vis::StructuredGridVolume* vol, vis::TransferFunction1D* tf, int width, int height);
+  // Light axes from the camera basis (renderingmanager.cpp:168, 421-426), or a direction.
+  void UpdateLightSourceCameraVectors();
+  void SetLightDirection(const std::array<float, 3>& towards_light);
+
+  bool Display();
+  bool SaveFramePFM(const std::string& path) const;  // float RGBA -> PFM (RGB)
+  bool SaveFramePPM(const std::string& path) const;  // 8-bit RGB over a white background
+  // Renders `frames` frames of the current renderer; appends "name,ms,fps" to `csv`.
+  bool EvaluateFrames(int frames, const std::string& csv, double* ms_per_frame = nullptr);
+
+  vis::Camera& GetCamera() { return m_camera; }
+  vis::RenderingParameters& GetRenderingParameters() { return m_params; }
+  vis::DataManager& GetDataManager() { return m_data; }
+  void CloseFunc();
+
+ private:
+  RenderingManager() = default;
+  ~RenderingManager();
+  std::vector<BaseVolumeRenderer*> m_renderers;
+  BaseVolumeRenderer* m_current = nullptr;
+  vis::DataManager m_data;
+  vis::RenderingParameters m_params;
+  vis::Camera m_camera;
+  static RenderingManager* s_instance;
+};
