@@ -23,6 +23,7 @@ VolGeom geom_of(const cvr_ctx* c) {
     g.tex_mul[i] = float(double(n[i]) / double(g.G[i]));
   }
   g.vox = c->d_vol;
+  g.quad = c->d_vol_quad;
   return g;
 }
 
@@ -145,6 +146,8 @@ static void free_all(cvr_ctx* c) {
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_pyr) (void)hipFree(c->d_pyr);
+  if (c->d_pyr_quad) (void)hipFree(c->d_pyr_quad);
+  if (c->d_vol_quad) (void)hipFree(c->d_vol_quad);
   if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
                   static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
@@ -245,6 +248,7 @@ int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, fl
   }
   CVR_HIP_CHECK(hipMemcpy(h->d_vol, voxels, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   h->nx = nx; h->ny = ny; h->nz = nz; h->dtype = dtype;
+  if (const int rc = build_volume_quad(h, nullptr)) return rc;
   h->sx = sx; h->sy = sy; h->sz = sz;
   h->vol_version++;
   h->nn_trained = false;

@@ -29,24 +29,23 @@ struct ConeK {
   const StencilF* st;       // surrogate stencil: axial/ring pairs [2*kNOcc], then shadow [kNSdw]
 };
 
-// GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w).
-__device__ __forceinline__ float pyr_tri(const float* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
+// GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w) through the
+// quad layout (2 float4 loads per trilinear read).
+__device__ __forceinline__ float pyr_tri(const float4* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
                                          float w) {
   const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
-  const float* __restrict__ a = pyr + pg.off[l];
-  int x0, x1, y0, y1, z0, z1;
-  float fx, fy, fz;
-  lin_axis(u * float(nx) - 0.5f, nx, x0, x1, fx);
-  lin_axis(v * float(ny) - 0.5f, ny, y0, y1, fy);
-  lin_axis(w * float(nz) - 0.5f, nz, z0, z1, fz);
-  const int sy = nx, sz = nx * ny;
-  const int r00 = y0 * sy + z0 * sz, r10 = y1 * sy + z0 * sz, r01 = y0 * sy + z1 * sz, r11 = y1 * sy + z1 * sz;
-  const float a000 = a[r00 + x0], a100 = a[r00 + x1], a010 = a[r10 + x0], a110 = a[r10 + x1];
-  const float a001 = a[r01 + x0], a101 = a[r01 + x1], a011 = a[r11 + x0], a111 = a[r11 + x1];
-  const float c00 = a000 + fx * (a100 - a000), c10 = a010 + fx * (a110 - a010);
-  const float c01 = a001 + fx * (a101 - a001), c11 = a011 + fx * (a111 - a011);
-  const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
-  return c0 + fz * (c1 - c0);
+  const float4* __restrict__ a = pyr + pg.off[l];
+  int x0, y0;
+  float fx, fy;
+  quad_axis(u * float(nx) - 0.5f, nx, x0, fx);
+  quad_axis(v * float(ny) - 0.5f, ny, y0, fy);
+  const float zt = w * float(nz) - 0.5f;
+  const float zf = floorf(zt);
+  const float fz = zt - zf;
+  const int zi = int(zf);
+  const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
+  const int row = y0 * nx + x0, plane = nx * ny;
+  return quad_lerp(a[row + z0 * plane], a[row + z1 * plane], fx, fy, fz);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
@@ -62,7 +61,7 @@ __device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float 
 
 // textureLod with LINEAR_MIPMAP_LINEAR (lerp of levels floor(lod) and floor(lod)+1,
 // the second clamped to the chain), then the border falloff.
-__device__ __forceinline__ float gauss_ext(const float* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
+__device__ __forceinline__ float gauss_ext(const float4* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
   const float q = float(pg.levels - 1);
   lod = fminf(fmaxf(lod, 0.f), q);
   const int d1 = int(floorf(lod));
@@ -76,13 +75,13 @@ __device__ __forceinline__ float gauss_ext(const float* __restrict__ pyr, const 
 }
 
 // The same lookup at a whole level `l` (one trilinear read).
-__device__ __forceinline__ float gauss_ext_level(const float* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
+__device__ __forceinline__ float gauss_ext_level(const float4* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
   const float sg = float(1 << l);
   return pyr_tri(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
-__device__ __forceinline__ float cone_T(const float* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
+__device__ __forceinline__ float cone_T(const float4* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
                                         int n, f3 p, f3 axis) {
   float od = 0.f;
 #pragma unroll 4
@@ -102,7 +101,7 @@ __device__ __forceinline__ void occ_frame(f3 dir, f3& k, f3& up, f3& right) {
 
 // 1 axial cone + (n_occ-1) cones whose axes sit on a ring at ring angle from the axis,
 // cosine-weighted like gt_ray_marching.comp:161-168.
-__device__ __forceinline__ float ct_occlusion(const float* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
+__device__ __forceinline__ float ct_occlusion(const float4* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
                                               f3 k, f3 up, f3 right) {
   float num = cone_T(pyr, pg, K.so, K.n_steps, p, k), den = 1.f;
   const int nr = K.n_occ - 1;
@@ -119,7 +118,7 @@ __device__ __forceinline__ float ct_occlusion(const float* __restrict__ pyr, con
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
 // opacities along the light (8..14), the local opacity (15).
-__device__ __forceinline__ void nn_feats(const float* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
+__device__ __forceinline__ void nn_feats(const float4* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
                                          f3 p, f3 k, f3 L, float* f) {
 #pragma unroll
   for (int i = 0; i < 2 * kNOcc; ++i) {

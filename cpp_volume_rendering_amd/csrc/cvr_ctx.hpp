@@ -23,6 +23,7 @@ struct cvr_ctx {
   int nx = 0, ny = 0, nz = 0, dtype = 0;
   float sx = 1, sy = 1, sz = 1;
   void* d_vol = nullptr;
+  void* d_vol_quad = nullptr;  // uchar4 / ushort4 neighbour blocks (k_quad.hip)
 
   // transfer function (host FP64 table as TransferFunction1D::BuildLinear builds it)
   int tf_max_density = 255;
@@ -46,6 +47,7 @@ struct cvr_ctx {
 
   // Gaussian extinction pyramid (cone tracers, surrogate)
   float* d_pyr = nullptr;
+  float4* d_pyr_quad = nullptr;  // float4 neighbour blocks of every level
   cvr::PyrGeom pyr{};
   uint64_t pyr_built_for = ~0ull;
   void* d_cone_tab = nullptr;  // cone step schedules + surrogate stencil
@@ -79,6 +81,8 @@ float volume_diagonal(const cvr_ctx* c);
 int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st);
 int build_sat(cvr_ctx* c, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+int build_volume_quad(cvr_ctx* c, hipStream_t st);
+int build_pyramid_quad(cvr_ctx* c, size_t total, hipStream_t st);
 int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, hipStream_t st);
 int check_cone_params(const cvr_cone_params& P);
 int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);

@@ -20,7 +20,7 @@
 namespace cvr {
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, PyrGeom pg, const float* __restrict__ pyr,
+__global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, PyrGeom pg, const float4* __restrict__ pyr,
                                                      ConeK K, const float4* __restrict__ tf, int ntf, float step,
                                                      float et, float4* __restrict__ out,
                                                      unsigned long long* __restrict__ counters) {
@@ -34,7 +34,6 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
   const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
   const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
   if (px >= rf.W || py >= rf.H) return;
-  const T* vox = static_cast<const T*>(g.vox);
   const f3 dir = camera_ray(rf, px, py);
   const f3 eye = ld3(rf.eye);
   float tn, tf_;
@@ -53,7 +52,7 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      float4 src = sample_tf(s_tf, ntf, sample_volume<T>(vox, g, p));
+      float4 src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
       ++nsamp;
       if (src.w > 0.0f) {
         ++nshade;
@@ -80,7 +79,7 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
 
 // Exact occlusion / shadow and the surrogate's features at given points (texture
 // space), for given camera-ray directions and light directions.
-__global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const float* __restrict__ pyr, ConeK K, int n,
+__global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const float4* __restrict__ pyr, ConeK K, int n,
                                                      const float* __restrict__ pos, const float* __restrict__ dirs,
                                                      const float* __restrict__ lights, float* __restrict__ occ,
                                                      float* __restrict__ sdw, float* __restrict__ feat) {
@@ -224,10 +223,10 @@ int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* o
   if (shm > 64 * 1024) { set_error("cone renderer: transfer function larger than 4096 entries"); return -4; }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_cone_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K,
+    hipLaunchKernelGGL(k_cone_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
                        c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
   else
-    hipLaunchKernelGGL(k_cone_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K,
+    hipLaunchKernelGGL(k_cone_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
                        c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
@@ -239,7 +238,7 @@ int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos,
   if (rc) return rc;
   float L[3] = {0, 0, 1};
   ConeK K = cone_constants(c, P, L);
-  hipLaunchKernelGGL(k_cone_points, dim3((n + 255) / 256), dim3(256), 0, st, c->pyr, c->d_pyr, K, n, d_pos, d_dirs,
+  hipLaunchKernelGGL(k_cone_points, dim3((n + 255) / 256), dim3(256), 0, st, c->pyr, c->d_pyr_quad, K, n, d_pos, d_dirs,
                      d_lights, d_occ, d_sdw, d_feat);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -284,7 +284,6 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
   const int py = ty_ * 16 + (w >> 1) * 8 + (lane >> 3);
   if (px >= rf.W || py >= rf.H) return;
 
-  const T* vox = static_cast<const T*>(k.g.vox);
   f3 dir = camera_ray(rf, px, py);
   f3 eye = ld3(rf.eye);
   float tn, tf_;
@@ -300,7 +299,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      const float dens = sample_volume<T>(vox, k.g, p);
+      const float dens = sample_volume_quad<T>(k.g, p);
       float4 src = sample_tf(s_tf, ntf, dens);
       ++nsamp;
       if (src.w > 0.0f) {

@@ -344,19 +344,18 @@ __device__ __forceinline__ f3 unit_sphere(uint64_t& s) {
 // a light direction uniform on the sphere; outputs the 16 features and the exact
 // (occlusion, shadow) of the cone tracer minus the network-free estimate.
 template <typename T>
-__global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const float* __restrict__ pyr, ConeK K,
+__global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const float4* __restrict__ pyr, ConeK K,
                                                     const float4* __restrict__ tf, int ntf, uint64_t seed, int n,
                                                     float* __restrict__ X, float* __restrict__ Tg,
                                                     float* __restrict__ pos_out) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   uint64_t s = mix64(seed ^ (uint64_t(i) * 0xD1B54A32D192ED03ull));
-  const T* vox = static_cast<const T*>(g.vox);
   f3 p = mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
   for (int t = 0; t < 64; ++t) {
     const f3 c = mk3(u01(s) * g.G[0], u01(s) * g.G[1], u01(s) * g.G[2]);
     p = c;
-    if (sample_tf(tf, ntf, sample_volume<T>(vox, g, c)).w > 0.f) break;
+    if (sample_tf(tf, ntf, sample_volume_quad<T>(g, c)).w > 0.f) break;
   }
   const f3 dir = unit_sphere(s);
   const f3 L = unit_sphere(s);
@@ -403,7 +402,7 @@ struct NnK {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const float* __restrict__ pyr,
+__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const float4* __restrict__ pyr,
                                                    ConeK K, NnK nk, const short* __restrict__ wpack,
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
                                                    int ntf, float step, float et, float4* __restrict__ out,
@@ -424,7 +423,6 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
   const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
   const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
   const bool in_img = px < rf.W && py < rf.H;
-  const T* vox = static_cast<const T*>(g.vox);
   const f3 dir = camera_ray(rf, in_img ? px : 0, in_img ? py : 0);
   const f3 eye = ld3(rf.eye);
   float tn = 0.f, tf_ = 0.f;
@@ -454,7 +452,7 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
       } else {
         h = fminf(step, D - s);
         p = tp + dir * (s + 0.5f * h);
-        src = sample_tf(s_tf, ntf, sample_volume<T>(vox, g, p));
+        src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
         ++nsamp;
         need = src.w > 0.f;
         if (!need) ++j;
@@ -611,10 +609,10 @@ int nn_generate(cvr_ctx* c, const cvr_cone_params& P, uint64_t seed, int n, floa
   ConeK K = cone_constants(c, P, L);
   VolGeom g = geom_of(c);
   if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_nn_samples<uint8_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr, K,
+    hipLaunchKernelGGL(k_nn_samples<uint8_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr_quad, K,
                        c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
   else
-    hipLaunchKernelGGL(k_nn_samples<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr, K,
+    hipLaunchKernelGGL(k_nn_samples<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr_quad, K,
                        c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
@@ -779,10 +777,10 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   float* bp = reinterpret_cast<float*>(wp + kPackShorts);
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K, nk, wp, bp,
+    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, wp, bp,
                        c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
   else
-    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr, K, nk, wp, bp,
+    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, wp, bp,
                        c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -51,7 +51,6 @@ __global__ __launch_bounds__(256) void k_pyr_level0(PyrBuild b, VolGeom g, const
   const int y = blockIdx.y * 8 + ((threadIdx.x >> 3) & 7);
   const int z = blockIdx.z * 4 + (threadIdx.x >> 6);
   if (x >= b.bx || y >= b.by || z >= b.bz) return;
-  const T* vox = static_cast<const T*>(g.vox);
   const float gx = (float(x) + 0.5f) * b.vs[0], gy = (float(y) + 0.5f) * b.vs[1], gz = (float(z) + 0.5f) * b.vs[2];
   float acc = 0.f;
   for (int i = 0; i < 343; ++i) {
@@ -59,7 +58,7 @@ __global__ __launch_bounds__(256) void k_pyr_level0(PyrBuild b, VolGeom g, const
     const float px = gx + float(kx) * b.sigma, py = gy + float(ky) * b.sigma, pz = gz + float(kz) * b.sigma;
     const float tx = px / b.G[0], ty = py / b.G[1], tz = pz / b.G[2];
     if (outside01(tx, ty, tz)) continue;
-    const float d = sample_volume<T>(vox, g, mk3(px, py, pz));
+    const float d = sample_volume_quad<T>(g, mk3(px, py, pz));
     acc += s_w[i] * sample_tf(tf_rgba, ntf, d).w;
   }
   out_op[size_t(x) + size_t(b.bx) * (y + size_t(b.by) * z)] = acc / s_wsum;
@@ -183,6 +182,10 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   CVR_HIP_CHECK(hipFree(d_op));
   c->pyr = pg;
+  if (build_pyramid_quad(c, total, st) != CVR_OK) {
+    c->pyr_built_for = ~0ull;
+    return -5;
+  }
   c->pyr_built_for = key;
   return 0;
 }

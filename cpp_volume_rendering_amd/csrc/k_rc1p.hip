@@ -32,7 +32,6 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
   const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
   if (px >= rf.W || py >= rf.H) return;
 
-  const T* vox = static_cast<const T*>(g.vox);
   f3 dir = camera_ray(rf, px, py);
   f3 eye = ld3(rf.eye);
   float tn, tf_;
@@ -46,7 +45,7 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      const float dens = sample_volume<T>(vox, g, p);
+      const float dens = sample_volume_quad<T>(g, p);
       float4 src = sample_tf(tfp, ntf, dens);
       ++nsamp;
       if (src.w > 0.0f) {
