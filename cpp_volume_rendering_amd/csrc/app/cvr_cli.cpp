@@ -3,7 +3,7 @@
 // seeded synthetic one, loads a .tf1d transfer function, renders frames, writes the
 // eval.csv line and a PFM/PPM of the last frame.
 //
-//   cvr_cli --technique rc1p|ebs|cone|nn [--volume f.raw | --synth n[,seed[,blobs]]]
+//   cvr_cli --technique rc1p|ebs|cone|nn|gt|dos [--volume f.raw | --synth n[,seed[,blobs]]]
 //           [--tf f.tf1d] [--size WxH] [--frames N] [--csv eval.csv] [--pfm out.pfm]
 //           [--ppm out.ppm] [--light x,y,z] [--check-inputs]
 //
@@ -19,7 +19,7 @@
 
 static int usage() {
   std::fprintf(stderr,
-               "usage: cvr_cli --technique rc1p|ebs|cone|nn [--volume f.raw | --synth n[,seed[,blobs]]] "
+               "usage: cvr_cli --technique rc1p|ebs|cone|nn|gt|dos [--volume f.raw | --synth n[,seed[,blobs]]] "
                "[--tf f.tf1d] [--size WxH] [--frames N] [--csv f] [--pfm f] [--ppm f] [--light x,y,z] "
                "[--check-inputs]\n");
   return 2;
@@ -90,9 +90,11 @@ int main(int argc, char** argv) {
   rm->AddVolumeRenderer(new RC1PExtinctionBasedShading());
   rm->AddVolumeRenderer(new RC1PConeTracedOcclusion());
   rm->AddVolumeRenderer(new RC1PLearnedConeOcclusion());
-  const char* names[] = {"rc1p", "ebs", "cone", "nn"};
+  rm->AddVolumeRenderer(new RC1PConeLightGroundTruthSteps());
+  rm->AddVolumeRenderer(new RC1PConeTracingDirOcclusionShading());
+  const char* names[] = {"rc1p", "ebs", "cone", "nn", "gt", "dos"};
   int id = -1;
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 6; ++i)
     if (tech == names[i]) id = i;
   if (id < 0) return usage();
   rm->InitData(vol, tf, W, H);

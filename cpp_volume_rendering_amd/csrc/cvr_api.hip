@@ -68,6 +68,8 @@ static size_t params_size(int technique) {
   switch (technique) {
     case CVR_RC1P: return sizeof(cvr_rc1p_params);
     case CVR_EBS: return sizeof(cvr_ebs_params);
+    case CVR_CONE_GT: return sizeof(cvr_gt_params);
+    case CVR_DOS: return sizeof(cvr_dos_params);
     case CVR_CONE_TRACE: return sizeof(cvr_cone_params);
     case CVR_CONE_NN: return sizeof(cvr_nn_params);
     default: return 0;
@@ -119,6 +121,41 @@ static void defaults(int technique, void* p) {
     e->shadow_type = 0;
     e->ka = 0.5f;
     e->kd = 0.5f;
+  } else if (technique == CVR_CONE_GT) {
+    // crtgtrenderer.cpp:32-51 (both lighting terms on here: with both off the shading
+    // divides by zero)
+    cvr_gt_params* g = static_cast<cvr_gt_params*>(p);
+    g->step_size = 0.f;
+    g->et_threshold = 0.99f;
+    g->apply_occlusion = 1;
+    g->apply_shadow = 1;
+    g->occ_rays = 1;
+    g->occ_aperture_deg = 90.f;
+    g->occ_max_dist = 0.f;
+    g->sdw_rays = 1;
+    g->sdw_aperture_deg = 1.f;
+    g->sdw_max_dist = 0.f;
+    g->shadow_type = 0;
+    g->initial_gap = 1.0f;
+    g->ray_step = 0.5f;
+    g->ka = 0.5f;
+    g->kd = 0.5f;
+  } else if (technique == CVR_DOS) {
+    // dosrcrenderer.cpp:29-70, conegaussiansampler.cpp:30-47, extcoefvolumegenerator.cpp:10-15
+    cvr_dos_params* d = static_cast<cvr_dos_params*>(p);
+    d->step_size = 0.f;
+    d->et_threshold = 0.99f;
+    d->apply_occlusion = 1;
+    d->apply_shadow = 0;
+    d->shadow_type = 0;
+    d->spot_max_deg = 4.0f;
+    d->n_levels = 0;
+    d->base_res[0] = d->base_res[1] = d->base_res[2] = 128;
+    d->sigma0 = 1.0f;
+    d->occ = cvr_dos_cone{20.0f, 1, 0.35f, 3.0f, 0.f, 1.25f, 2.0f};
+    d->sdw = cvr_dos_cone{0.5f, 0, 1.0f, 3.0f, 0.f, 1.25f, 2.0f};
+    d->ka = 0.5f;
+    d->kd = 0.5f;
   } else if (technique == CVR_CONE_TRACE) {
     cone_defaults(static_cast<cvr_cone_params*>(p));
   } else if (technique == CVR_CONE_NN) {
@@ -149,6 +186,8 @@ static void free_all(cvr_ctx* c) {
   if (c->d_pyr_quad) (void)hipFree(c->d_pyr_quad);
   if (c->d_vol_quad) (void)hipFree(c->d_vol_quad);
   if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
+  if (c->d_dos_rows) (void)hipFree(c->d_dos_rows);
+  if (c->d_gt_rays) (void)hipFree(c->d_gt_rays);
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
                   static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
     if (p) (void)hipFree(p);
@@ -179,6 +218,8 @@ const char* cvr_technique_name(int technique) {
   switch (technique) {
     case CVR_RC1P: return "1-Pass - Ray Casting";
     case CVR_EBS: return "1-Pass - Ray Casting - Extinction-based";
+    case CVR_CONE_GT: return "1-Pass - Ray Casting - Cone Ground Truth (Steps)";
+    case CVR_DOS: return "1-Pass - Ray Casting - Dir. Occlusion Shading";
     case CVR_CONE_TRACE: return "1-Pass - Ray Casting - Cone-traced Directional Occlusion";
     case CVR_CONE_NN: return "1-Pass - Ray Casting - Learned Cone Occlusion";
     default: return nullptr;
@@ -201,6 +242,8 @@ int cvr_create(int technique, int device, cvr_handle* out) {
   c->device = device;
   defaults(CVR_RC1P, &c->rc1p);
   defaults(CVR_EBS, &c->ebs);
+  defaults(CVR_CONE_GT, &c->gt);
+  defaults(CVR_DOS, &c->dos);
   defaults(CVR_CONE_TRACE, &c->cone);
   defaults(CVR_CONE_NN, &c->nn);
   if (hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess ||
@@ -341,6 +384,8 @@ static void* live_params(cvr_ctx* c) {
   switch (c->technique) {
     case CVR_RC1P: return &c->rc1p;
     case CVR_EBS: return &c->ebs;
+    case CVR_CONE_GT: return &c->gt;
+    case CVR_DOS: return &c->dos;
     case CVR_CONE_TRACE: return &c->cone;
     case CVR_CONE_NN: return &c->nn;
     default: return nullptr;
@@ -356,6 +401,14 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
       set_error("cvr_set_params: et_threshold must be > 0");
       return CVR_ERR_INVALID;
     }
+  }
+  if (h->technique == CVR_CONE_GT) {
+    const int rc = check_gt_params(*static_cast<const cvr_gt_params*>(params));
+    if (rc) return rc;
+  }
+  if (h->technique == CVR_DOS) {
+    const int rc = check_dos_params(*static_cast<const cvr_dos_params*>(params));
+    if (rc) return rc;
   }
   if (h->technique == CVR_CONE_TRACE || h->technique == CVR_CONE_NN) {
     const cvr_cone_params& cp = h->technique == CVR_CONE_TRACE ? *static_cast<const cvr_cone_params*>(params)
@@ -396,6 +449,8 @@ int cvr_prepare(cvr_handle h, void* hip_stream) {
   DeviceGuard dg(h->device);
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
   if (h->technique == CVR_EBS) return build_sat(h, st);
+  if (h->technique == CVR_CONE_GT) return prepare_gt(h, h->gt);
+  if (h->technique == CVR_DOS) return prepare_dos(h, h->dos, st);
   if (h->technique == CVR_CONE_TRACE) return prepare_cone(h, h->cone, st);
   if (h->technique == CVR_CONE_NN) {
     rc = prepare_cone(h, h->nn.cone, st);
@@ -435,6 +490,8 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
   switch (h->technique) {
     case CVR_RC1P: rc = launch_rc1p(h, rf, dst, st); break;
     case CVR_EBS: rc = launch_ebs(h, rf, light, dst, st); break;
+    case CVR_CONE_GT: rc = launch_gt(h, rf, light, dst, st); break;
+    case CVR_DOS: rc = launch_dos(h, rf, light, dst, st); break;
     case CVR_CONE_TRACE: rc = launch_cone(h, rf, light, dst, st); break;
     case CVR_CONE_NN: rc = launch_nn(h, rf, light, dst, st); break;
     default: set_error("cvr_render: technique not available"); rc = CVR_ERR_UNSUPPORTED;
@@ -480,14 +537,22 @@ static const cvr_cone_params* cone_of(cvr_ctx* h) {
   return nullptr;
 }
 
-int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels) {
-  if (!h || !levels) { set_error("cvr_pyramid_info: null argument"); return CVR_ERR_INVALID; }
+// Builds the pyramid of a pyramid-based technique.
+static int prepare_pyramid(cvr_ctx* h, const char* who) {
   const cvr_cone_params* cp = cone_of(h);
-  if (!cp) { set_error("cvr_pyramid_info: technique has no pyramid"); return CVR_ERR_UNSUPPORTED; }
+  if (!cp && h->technique != CVR_DOS) {
+    set_error(std::string(who) + ": technique has no pyramid");
+    return CVR_ERR_UNSUPPORTED;
+  }
   int rc = check_inputs(h);
   if (rc) return rc;
+  return cp ? prepare_cone(h, *cp, nullptr) : prepare_dos(h, h->dos, nullptr);
+}
+
+int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels) {
+  if (!h || !levels) { set_error("cvr_pyramid_info: null argument"); return CVR_ERR_INVALID; }
   DeviceGuard dg(h->device);
-  rc = prepare_cone(h, *cp, nullptr);
+  int rc = prepare_pyramid(h, "cvr_pyramid_info");
   if (rc) return rc;
   *levels = h->pyr.levels;
   if (dims)
@@ -498,12 +563,8 @@ int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels) {
 
 int cvr_get_pyramid_level(cvr_handle h, int level, float* out_host, size_t count) {
   if (!h || !out_host) { set_error("cvr_get_pyramid_level: null argument"); return CVR_ERR_INVALID; }
-  const cvr_cone_params* cp = cone_of(h);
-  if (!cp) { set_error("cvr_get_pyramid_level: technique has no pyramid"); return CVR_ERR_UNSUPPORTED; }
-  int rc = check_inputs(h);
-  if (rc) return rc;
   DeviceGuard dg(h->device);
-  rc = prepare_cone(h, *cp, nullptr);
+  int rc = prepare_pyramid(h, "cvr_get_pyramid_level");
   if (rc) return rc;
   if (level < 0 || level >= h->pyr.levels) { set_error("cvr_get_pyramid_level: level out of range"); return CVR_ERR_INVALID; }
   const size_t n = size_t(h->pyr.dims[level][0]) * h->pyr.dims[level][1] * h->pyr.dims[level][2];
@@ -695,6 +756,49 @@ int cvr_nn_load(cvr_handle h, const void* in, size_t bytes) {
   if (rc) return rc;
   CVR_HIP_CHECK(hipDeviceSynchronize());
   h->nn_trained = true;
+  return CVR_OK;
+}
+
+int cvr_dos_sections(cvr_handle h, int which, float* rows, int max_rows, int* n_sections, int* counts, float* axes,
+                     float* adj7) {
+  if (!h) { set_error("cvr_dos_sections: null handle"); return CVR_ERR_INVALID; }
+  if (h->technique != CVR_DOS) { set_error("cvr_dos_sections: technique has no section tables"); return CVR_ERR_UNSUPPORTED; }
+  if (which != 0 && which != 1) { set_error("cvr_dos_sections: which must be 0 (occlusion) or 1 (shadow)"); return CVR_ERR_INVALID; }
+  if (rows && max_rows < 0) { set_error("cvr_dos_sections: max_rows < 0"); return CVR_ERR_INVALID; }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  DeviceGuard dg(h->device);
+  rc = prepare_dos(h, h->dos, nullptr);
+  if (rc) return rc;
+  const DosTable& T = which == 0 ? h->dos_occ : h->dos_sdw;
+  const int n = int(T.rows.size());
+  if (n_sections) *n_sections = n;
+  if (rows)
+    for (int i = 0; i < std::min(n, max_rows); ++i) {
+      rows[4 * i] = T.rows[i].x;
+      rows[4 * i + 1] = T.rows[i].y;
+      rows[4 * i + 2] = T.rows[i].z;
+      rows[4 * i + 3] = T.rows[i].w;
+    }
+  if (counts)
+    for (int k = 0; k < 3; ++k) counts[k] = T.n[k];
+  if (axes) std::memcpy(axes, T.axes, sizeof(T.axes));
+  if (adj7) *adj7 = T.adj7;
+  return CVR_OK;
+}
+
+int cvr_gt_rays(cvr_handle h, int which, float* out, int max_rays, int* n) {
+  if (!h) { set_error("cvr_gt_rays: null handle"); return CVR_ERR_INVALID; }
+  if (h->technique != CVR_CONE_GT) { set_error("cvr_gt_rays: technique has no ray tables"); return CVR_ERR_UNSUPPORTED; }
+  if (which != 0 && which != 1) { set_error("cvr_gt_rays: which must be 0 (occlusion) or 1 (shadow)"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  int rc = prepare_gt(h, h->gt);
+  if (rc) return rc;
+  const std::vector<float>& R = which == 0 ? h->gt_occ : h->gt_sdw;
+  const int m = int(R.size() / 3);
+  if (n) *n = m;
+  if (out)
+    for (int i = 0; i < std::min(m, max_rays) * 3; ++i) out[i] = R[i];
   return CVR_OK;
 }
 

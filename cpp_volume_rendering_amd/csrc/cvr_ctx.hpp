@@ -13,6 +13,14 @@ struct PyrGeom {
   size_t off[kMaxLevels];
   float G[3];
 };
+// Section table of one directional-occlusion cone: rows (interval, level, d_integral,
+// amplitude), section counts with 1/3/7 Gaussians, sub-cone axes.
+struct DosTable {
+  std::vector<float4> rows;
+  int n[3] = {0, 0, 0};
+  float axes[10][3] = {};
+  float adj7 = 0.f, init = 0.f, w = 0.f;
+};
 }  // namespace cvr
 
 struct cvr_ctx {
@@ -37,6 +45,8 @@ struct cvr_ctx {
   // parameters (one of these is live, chosen by `technique`)
   cvr_rc1p_params rc1p{};
   cvr_ebs_params ebs{};
+  cvr_gt_params gt{};
+  cvr_dos_params dos{};
   cvr_cone_params cone{};
   cvr_nn_params nn{};
 
@@ -51,6 +61,16 @@ struct cvr_ctx {
   cvr::PyrGeom pyr{};
   uint64_t pyr_built_for = ~0ull;
   void* d_cone_tab = nullptr;  // cone step schedules + surrogate stencil
+
+  // directional occlusion shading: section tables (occlusion rows, then shadow rows)
+  cvr::DosTable dos_occ, dos_sdw;
+  float4* d_dos_rows = nullptr;
+  size_t dos_rows_cap = 0;
+
+  // cone ground truth: sampled ray directions (occlusion, then shadow)
+  std::vector<float> gt_occ, gt_sdw;
+  float* d_gt_rays = nullptr;
+  cvr_gt_params gt_rays_for{};
   int cone_n_steps = 0;
 
   // surrogate network: fp32 master weights, gradient, Adam moments, packed bf16 copy
@@ -89,6 +109,12 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);
 int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos, const float* d_dirs,
                 const float* d_lights, float* d_occ, float* d_sdw, float* d_feat, hipStream_t st);
+int check_gt_params(const cvr_gt_params& P);
+int prepare_gt(cvr_ctx* c, const cvr_gt_params& P);
+int launch_gt(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+int check_dos_params(const cvr_dos_params& P);
+int prepare_dos(cvr_ctx* c, const cvr_dos_params& P, hipStream_t st);
+int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int nn_alloc(cvr_ctx* c);
 int nn_pack(cvr_ctx* c, hipStream_t st);
 int nn_train(cvr_ctx* c, hipStream_t st);

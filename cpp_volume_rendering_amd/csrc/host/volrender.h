@@ -90,6 +90,23 @@ class RC1PExtinctionBasedShading : public GpuVolumeRenderer {
   const char* GetAbbreviationName() override { return "s_1rc_eb"; }
 };
 
+// RC1PConeLightGroundTruthSteps (crtgtrenderer.h:50-120): ray-sampled cone ground truth
+class RC1PConeLightGroundTruthSteps : public GpuVolumeRenderer {
+ public:
+  RC1PConeLightGroundTruthSteps() : GpuVolumeRenderer(CVR_CONE_GT) {}
+  const char* GetName() override { return "1-Pass - Ray Casting - Cone Ground Truth (Steps)"; }
+  const char* GetAbbreviationName() override { return "s_1rc_gt_c"; }
+};
+
+// RC1PConeTracingDirOcclusionShading (dosrcrenderer.h:38-121): cone section tables
+// integrated over the Gaussian extinction pyramid
+class RC1PConeTracingDirOcclusionShading : public GpuVolumeRenderer {
+ public:
+  RC1PConeTracingDirOcclusionShading() : GpuVolumeRenderer(CVR_DOS) {}
+  const char* GetName() override { return "1-Pass - Ray Casting - Dir. Occlusion Shading"; }
+  const char* GetAbbreviationName() override { return "s_1rc_dos"; }
+};
+
 // Cone-traced directional occlusion + shadows over the Gaussian extinction pyramid
 // (the pyramid and shading of RC1PConeTracingDirOcclusionShading, dosrcrenderer.h:38-121)
 class RC1PConeTracedOcclusion : public GpuVolumeRenderer {

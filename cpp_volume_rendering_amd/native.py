@@ -47,6 +47,27 @@ class EbsParams(C.Structure):
                 ("ka", C.c_float), ("kd", C.c_float)]
 
 
+class GtParams(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
+                ("apply_shadow", C.c_int), ("occ_rays", C.c_int), ("occ_aperture_deg", C.c_float),
+                ("occ_max_dist", C.c_float), ("sdw_rays", C.c_int), ("sdw_aperture_deg", C.c_float),
+                ("sdw_max_dist", C.c_float), ("shadow_type", C.c_int), ("initial_gap", C.c_float),
+                ("ray_step", C.c_float), ("ka", C.c_float), ("kd", C.c_float)]
+
+
+class DosCone(C.Structure):
+    _fields_ = [("half_deg", C.c_float), ("max_packing", C.c_int), ("ui_weight", C.c_float),
+                ("initial_step", C.c_float), ("covered_dist", C.c_float), ("d_sigma", C.c_float),
+                ("r_sigma", C.c_float)]
+
+
+class DosParams(C.Structure):
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
+                ("apply_shadow", C.c_int), ("shadow_type", C.c_int), ("spot_max_deg", C.c_float),
+                ("n_levels", C.c_int), ("base_res", C.c_int * 3), ("sigma0", C.c_float), ("occ", DosCone),
+                ("sdw", DosCone), ("ka", C.c_float), ("kd", C.c_float)]
+
+
 class ConeParams(C.Structure):
     _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
                 ("apply_shadow", C.c_int), ("n_levels", C.c_int), ("base_res", C.c_int * 3), ("sigma0", C.c_float),
@@ -73,7 +94,8 @@ class Stats(C.Structure):
                 ("last_ms", C.c_double)]
 
 
-PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_TRACE: ConeParams, CONE_NN: NnParams}
+PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, CONE_TRACE: ConeParams,
+          CONE_NN: NnParams}
 
 # every symbol include/cvr.h and include/cvr_synth.h declare
 EXPORTS = [
@@ -82,7 +104,7 @@ EXPORTS = [
     "cvr_render", "cvr_get_sat", "cvr_get_stats", "cvr_set_collect_stats",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
-    "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load",
+    "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load", "cvr_dos_sections", "cvr_gt_rays",
     "cvr_synth_blobs", "cvr_synth_noise",
 ]
 
@@ -135,6 +157,8 @@ def lib() -> C.CDLL:
         "cvr_nn_checkpoint_size": (sz, []),
         "cvr_nn_save": (i, [vp, vp, sz]),
         "cvr_nn_load": (i, [vp, vp, sz]),
+        "cvr_dos_sections": (i, [vp, i, vp, i, vp, vp, vp, vp]),
+        "cvr_gt_rays": (i, [vp, i, vp, i, vp]),
         "cvr_synth_blobs": (i, [i, i, i, i, i, d, i, C.c_uint64, vp]),
         "cvr_synth_noise": (i, [i, i, i, i, i, C.c_uint64, i, vp, vp]),
     }
@@ -275,6 +299,26 @@ class Renderer:
         check(lib().cvr_cone_eval(self.h, n, pos.ctypes.data, dirs.ctypes.data, lights.ctypes.data,
                                   occ.ctypes.data, sdw.ctypes.data, feat.ctypes.data))
         return occ, sdw, feat
+
+    def dos_sections(self, which: int):
+        """Section table of the occlusion (0) or shadow (1) cone: (rows (n, 4) float32,
+        counts (3,), axes (10, 3), adj7)."""
+        n = C.c_int()
+        check(lib().cvr_dos_sections(self.h, which, None, 0, C.byref(n), None, None, None))
+        rows = np.zeros((n.value, 4), np.float32)
+        counts = np.zeros(3, np.int32)
+        axes = np.zeros((10, 3), np.float32)
+        adj7 = C.c_float()
+        check(lib().cvr_dos_sections(self.h, which, rows.ctypes.data, n.value, C.byref(n), counts.ctypes.data,
+                                     axes.ctypes.data, C.byref(adj7)))
+        return rows, counts, axes, adj7.value
+
+    def gt_rays(self, which: int) -> np.ndarray:
+        n = C.c_int()
+        check(lib().cvr_gt_rays(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3), np.float32)
+        check(lib().cvr_gt_rays(self.h, which, out.ctypes.data, n.value, C.byref(n)))
+        return out
 
     def nn_train(self, stream=None):
         check(lib().cvr_nn_train(self.h, stream))

@@ -34,6 +34,11 @@ extern "C" {
 /* Techniques.  Each replaces one reference renderer class:
  *   CVR_RC1P        RayCasting1Pass            cppvolrend/structured/rc1pass/rc1prenderer.h:32-78
  *   CVR_EBS         RC1PExtinctionBasedShading cppvolrend/structured/rc1pextbsd/ebsrenderer.h:45-119
+ *   CVR_CONE_GT     RC1PConeLightGroundTruthSteps (ray-sampled cone ground truth)
+ *                   cppvolrend/structured/rc1pcrtgt/crtgtrenderer.h:50-120
+ *   CVR_DOS         RC1PConeTracingDirOcclusionShading (section tables of
+ *                   ConeGaussianSampler integrated over the Gaussian extinction
+ *                   pyramid) cppvolrend/structured/rc1pdosct/dosrcrenderer.h:38-121
  *   CVR_CONE_TRACE  cone-traced directional occlusion + shadows over the Gaussian
  *                   extinction pyramid of RC1PConeTracingDirOcclusionShading
  *                   (cppvolrend/structured/rc1pdosct/dosrcrenderer.h:38-121 and
@@ -42,7 +47,7 @@ extern "C" {
  *   CVR_CONE_NN     the same renderer with the per-sample cone integration replaced
  *                   by a learned bf16 surrogate; no reference counterpart
  */
-enum cvr_technique { CVR_RC1P = 0, CVR_EBS = 1, CVR_CONE_TRACE = 4, CVR_CONE_NN = 5 };
+enum cvr_technique { CVR_RC1P = 0, CVR_EBS = 1, CVR_CONE_GT = 2, CVR_DOS = 3, CVR_CONE_TRACE = 4, CVR_CONE_NN = 5 };
 
 enum cvr_dtype { CVR_U8 = 0, CVR_U16 = 1 };
 
@@ -89,6 +94,57 @@ typedef struct cvr_ebs_params {
   int shadow_type;       /* TypeOfShadow: 0 point light (position), 1 directional (forward) */
   float ka, kd;          /* Kambient, Kdiffuse: 0.5, 0.5 */
 } cvr_ebs_params;
+
+/* RC1PConeLightGroundTruthSteps parameters (crtgtrenderer.cpp:23-56, 106-113).  Each
+ * opaque sample casts n rays inside a cone (theta ~ U[0, aperture], phi ~ U[0, 360)
+ * drawn once per parameter change from a default-seeded std::default_random_engine,
+ * :131-185) and marches them with the trapezoid rule from `initial_gap` in steps of
+ * `ray_step` (gt_ray_marching.comp:109-252). */
+typedef struct cvr_gt_params {
+  float step_size;        /* <= 0: reference default 0.5/sqrt(3)*|voxel scale| */
+  float et_threshold;     /* 0.99 (gt_ray_marching.comp:450) */
+  int apply_occlusion;    /* m_apply_occlusion (reference default off; here 1) */
+  int apply_shadow;       /* m_apply_shadows (reference default off; here 1) */
+  int occ_rays;           /* OccNumberOfSampledRays, 1 */
+  float occ_aperture_deg; /* OccConeApertureAngle, 90 */
+  float occ_max_dist;     /* <= 0: 0.5 * volume diagonal */
+  int sdw_rays;           /* SdwNumberOfSampledRays, 1 */
+  float sdw_aperture_deg; /* SdwConeApertureAngle, 1 */
+  float sdw_max_dist;     /* <= 0: 0.75 * volume diagonal */
+  int shadow_type;        /* SdwShadowType: 0 point light (position), 1 spot, 2 directional */
+  float initial_gap;      /* LightRayInitialGap, 1.0 */
+  float ray_step;         /* LightRayStepSize, 0.5 */
+  float ka, kd;           /* 0.5, 0.5 */
+} cvr_gt_params;
+
+/* One cone of RC1PConeTracingDirOcclusionShading: the ConeGaussianSampler settings
+ * (conegaussiansampler.cpp:30-47, 52-172; out-of-range values are clamped as the
+ * reference's setters clamp them). */
+typedef struct cvr_dos_cone {
+  float half_deg;       /* cone half angle, [0.5, 89.5] */
+  int max_packing;      /* 0: one Gaussian per section, 1: up to 3, 2: up to 7 */
+  float ui_weight;      /* UIWeight multiplying the optical depth */
+  float initial_step;   /* distance of the first section, 3.0 */
+  float covered_dist;   /* <= 0: the reference's fraction of the diagonal; at least 10 */
+  float d_sigma;        /* integration half-step multiplier, [1, 3], 1.25 */
+  float r_sigma;        /* Gaussian sigma limit multiplier, [0.5, 3], 2.0 */
+} cvr_dos_cone;
+
+/* RC1PConeTracingDirOcclusionShading parameters (dosrcrenderer.cpp:29-70, 101-132). */
+typedef struct cvr_dos_params {
+  float step_size;       /* <= 0: reference default */
+  float et_threshold;    /* 0.99 (ray_bbox_marching.comp:725) */
+  int apply_occlusion;   /* 1 */
+  int apply_shadow;      /* 0 */
+  int shadow_type;       /* type_of_shadow: 0 point light, 1 spot light, 2 directional */
+  float spot_max_deg;    /* spot light half angle (lightsourcelist.cpp:32: 4) */
+  int n_levels;          /* pyramid levels kept; <= 0: the full chain down to 1^3 */
+  int base_res[3];       /* pyramid level-0 resolution, 128^3 (extcoefvolumegenerator.cpp:10-15); 0: the volume's */
+  float sigma0;          /* level-0 Gaussian sigma, 1.0 */
+  cvr_dos_cone occ;      /* 20 deg, packing 1 (3 cones), weight 0.35, covered 0.5 * diagonal */
+  cvr_dos_cone sdw;      /* 0.5 deg, packing 0, weight 1.0, covered 0.75 * diagonal */
+  float ka, kd;          /* 0.5, 0.5 */
+} cvr_dos_params;
 
 /* Cone-traced lighting over the Gaussian extinction pyramid.  The pyramid follows
  * ExtinctionCoefficientVolume (extcoefvolumegenerator.cpp:10-15, 230-408: 7^3
@@ -190,11 +246,26 @@ typedef struct cvr_stats {
 int cvr_get_stats(cvr_handle h, cvr_stats* out);
 int cvr_set_collect_stats(cvr_handle h, int enable);
 
-/* The extinction pyramid of CVR_CONE_TRACE / CVR_CONE_NN (built if needed): level
+/* The extinction pyramid of CVR_DOS / CVR_CONE_TRACE / CVR_CONE_NN (built if needed): level
  * count and dims (3 ints per level, x first), and one level's extinction (fp32,
  * x fastest).  Replaces reading back ExtinctionCoefficientVolume's texture. */
 int cvr_pyramid_info(cvr_handle h, int* levels, int* dims, int max_levels);
 int cvr_get_pyramid_level(cvr_handle h, int level, float* out_host, size_t count);
+
+/* CVR_DOS: the section table of the occlusion (which = 0) or shadow (which = 1) cone
+ * as ConeGaussianSampler::ComputeConeIntegrationSteps builds it in FP64
+ * (conegaussiansampler.cpp:212-414) and GetConeSectionsInfoTex lays it out (:179-205):
+ * rows (interval to the next section, mip level, d_integral, amplitude) as float.
+ * `n_sections` receives the row count (at most max_rows are copied), `counts` the
+ * sections with 1, 3 and 7 Gaussians, `axes` the 3 + 7 sub-cone axes (x, y, z) and
+ * `adj7` the 7-cone adjacent weight (each output may be NULL). */
+int cvr_dos_sections(cvr_handle h, int which, float* rows, int max_rows, int* n_sections, int* counts, float* axes,
+                     float* adj7);
+
+/* CVR_CONE_GT: the sampled ray directions of the occlusion (which = 0) or shadow
+ * (which = 1) cone, 3 floats each in the cone's (right, up, axis) frame
+ * (crtgtrenderer.cpp:131-185).  `n` receives the count. */
+int cvr_gt_rays(cvr_handle h, int which, float* out, int max_rays, int* n);
 
 /* Exact cone-traced occlusion and shadow at n points (host arrays): positions in
  * texture space [0, G], camera-ray directions, light directions (3 floats each);

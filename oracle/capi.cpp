@@ -3,6 +3,7 @@
 // __graft_entry__.smoke() and bench.py's CPU-baseline leg.
 #include "oracle.hpp"
 #include "oracle_cone.hpp"
+#include "oracle_dos.hpp"
 #include <omp.h>
 
 using namespace orc;
@@ -283,6 +284,125 @@ void orc_cone_points(void* pyr, const ConeC* c, int n, const double* pos, const 
     if (feat) nn_features<double>(P, cp, p, k, L, feat + size_t(i) * kNFeat);
   }
 }
+// ---------------------------------------------------------------- directional occlusion shading
+// Cone spec after the reference's clamps (the caller applies them): half_deg, packing,
+// ui_weight, initial_step, covered, d_sigma, r_sigma.
+struct DosConeC {
+  double half_deg;
+  int max_packing;
+  double ui_weight, initial_step, covered, d_sigma, r_sigma;
+};
+struct DosC {
+  int apply_occ, apply_sdw, shadow_type;
+  double spot_cos, ka, kd;
+  DosConeC occ, sdw;
+  double lpos[3], lfwd[3], lup[3], lright[3];
+};
+static DosConeSpec spec_from(const DosConeC& c) {
+  DosConeSpec s;
+  s.half_deg = c.half_deg; s.max_packing = c.max_packing; s.ui_weight = c.ui_weight;
+  s.initial_step = c.initial_step; s.covered = c.covered; s.d_sigma = c.d_sigma; s.r_sigma = c.r_sigma;
+  return s;
+}
+static DosParamsO dos_from(const DosC* c) {
+  DosParamsO p;
+  p.apply_occ = c->apply_occ; p.apply_sdw = c->apply_sdw; p.shadow_type = c->shadow_type;
+  p.spot_cos = c->spot_cos; p.ka = c->ka; p.kd = c->kd;
+  p.occ = spec_from(c->occ);
+  p.sdw = spec_from(c->sdw);
+  for (int i = 0; i < 3; ++i) {
+    p.lpos[i] = c->lpos[i]; p.lfwd[i] = c->lfwd[i]; p.lup[i] = c->lup[i]; p.lright[i] = c->lright[i];
+  }
+  return p;
+}
+// Section table: returns the row count (or -1); rows[8 i ..] = interval, level, d_integral,
+// amplitude, pos, radius, sigma, n_gaussians (up to max_rows); counts[3], axes[30], adj7[1].
+int orc_dos_sections(const DosConeC* c, double sigma0, double* rows, int max_rows, int* counts, double* axes,
+                     double* adj7) {
+  DosTableO T;
+  if (!dos_build(spec_from(*c), sigma0, T)) return -1;
+  for (int i = 0; i < int(T.s.size()) && i < max_rows; ++i) {
+    const DosSection& s = T.s[i];
+    double r[8] = {s.interval, s.level, s.d_integral, s.amplitude, s.pos, s.radius, s.sigma, double(s.ng)};
+    for (int k = 0; k < 8; ++k) rows[8 * i + k] = r[k];
+  }
+  for (int k = 0; k < 3; ++k) counts[k] = T.n[k];
+  for (int j = 0; j < 10; ++j)
+    for (int k = 0; k < 3; ++k) axes[3 * j + k] = T.axes[j][k];
+  *adj7 = T.adj7;
+  return int(T.s.size());
+}
+// One cone through a table at texture-space point pos with frame (k, u, v).
+double orc_dos_cone(void* pyr, const DosConeC* c, double sigma0, const double* pos, const double* k, const double* u,
+                    const double* v) {
+  DosTableO T;
+  DosConeSpec sp = spec_from(*c);
+  if (!dos_build(sp, sigma0, T)) return -1.0;
+  return dos_cone<double>(*static_cast<Pyramid*>(pyr), T, sp, V3<double>(pos[0], pos[1], pos[2]),
+                          V3<double>(k[0], k[1], k[2]), V3<double>(u[0], u[1], u[2]), V3<double>(v[0], v[1], v[2]));
+}
+void orc_render_dos_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                        void* tf, void* pyr, const DosC* c, double sigma0, const float* cam, int W, int H,
+                        double step, double et, double* out, double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  render_dos<double>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr),
+                     dos_from(c), sigma0, mkcam(cam), W, H, step, et, out, margin, stride);
+}
+void orc_render_dos_f32(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                        void* tf, void* pyr, const DosC* c, double sigma0, const float* cam, int W, int H, float step,
+                        float et, float* out, int nthreads, int stride) {
+  threads(nthreads);
+  render_dos<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr),
+                    dos_from(c), sigma0, mkcam(cam), W, H, step, et, out, nullptr, stride);
+}
+
+// ---------------------------------------------------------------- cone ground truth
+struct GtC {
+  int apply_occ, apply_sdw, shadow_type, n_occ, n_sdw;
+  double occ_aperture_deg, sdw_aperture_deg, occ_dist, sdw_dist, gap, rstep, ka, kd;
+  double lpos[3], lfwd[3], lup[3], lright[3];
+};
+static GtParamsO gt_from(const GtC* c) {
+  GtParamsO q;
+  q.apply_occ = c->apply_occ; q.apply_sdw = c->apply_sdw; q.shadow_type = c->shadow_type;
+  std::default_random_engine gen;
+  gt_draw(gen, c->n_occ, float(c->occ_aperture_deg), q.occ_rays);
+  gt_draw(gen, c->n_sdw, float(c->sdw_aperture_deg), q.sdw_rays);
+  q.occ_dist = c->occ_dist; q.sdw_dist = c->sdw_dist; q.gap = c->gap; q.rstep = c->rstep;
+  q.ka = c->ka; q.kd = c->kd;
+  for (int i = 0; i < 3; ++i) {
+    q.lpos[i] = c->lpos[i]; q.lfwd[i] = c->lfwd[i]; q.lup[i] = c->lup[i]; q.lright[i] = c->lright[i];
+  }
+  return q;
+}
+void orc_gt_rays(const GtC* c, double* occ, double* sdw) {
+  GtParamsO q = gt_from(c);
+  std::copy(q.occ_rays.begin(), q.occ_rays.end(), occ);
+  std::copy(q.sdw_rays.begin(), q.sdw_rays.end(), sdw);
+}
+// Transmittance of one ray from texture-space point pos along unit w.
+double orc_gt_ray(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                  const double* pos, const double* w, double dist, double gap, double rstep) {
+  Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  V3<double> G(nx * sx, ny * sy, nz * sz);
+  return gt_ray<double>(v, *static_cast<TF*>(tf), G, V3<double>(pos[0], pos[1], pos[2]), V3<double>(w[0], w[1], w[2]),
+                        dist, gap, rstep, nullptr);
+}
+void orc_render_gt_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                       const GtC* c, const float* cam, int W, int H, double step, double et, double* out,
+                       double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  render_gt<double>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), gt_from(c), mkcam(cam), W, H,
+                    step, et, out, margin, stride);
+}
+void orc_render_gt_f32(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                       const GtC* c, const float* cam, int W, int H, float step, float et, float* out, int nthreads,
+                       int stride) {
+  threads(nthreads);
+  render_gt<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), gt_from(c), mkcam(cam), W, H,
+                   step, et, out, nullptr, stride);
+}
+
 int orc_mlp_nparams() { return kNParams; }
 void orc_mlp_forward(const double* prm, int n, const double* X, double* Y) {
   for (int i = 0; i < n; ++i) mlp_forward(prm, X + size_t(i) * kNFeat, Y + size_t(i) * kOut);

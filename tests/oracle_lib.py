@@ -66,6 +66,14 @@ def lib():
         "orc_render_cone_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
         "orc_render_cone_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, f, f, vp, i, i]),
         "orc_cone_points": (None, [vp, vp, i, vp, vp, vp, vp, vp, i]),
+        "orc_dos_sections": (i, [vp, d, vp, i, vp, vp, vp]),
+        "orc_dos_cone": (d, [vp, vp, d, vp, vp, vp, vp]),
+        "orc_render_dos_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_dos_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, vp, i, i, f, f, vp, i, i]),
+        "orc_gt_rays": (None, [vp, vp, vp]),
+        "orc_gt_ray": (d, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, d, d]),
+        "orc_render_gt_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_gt_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, i, i, f, f, vp, i, i]),
         "orc_mlp_nparams": (i, []),
         "orc_mlp_forward": (None, [vp, i, vp, vp]),
         "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
@@ -344,3 +352,148 @@ def mlp_loss_grad(prm, X, T):
     g = np.zeros_like(prm)
     loss = lib().orc_mlp_loss_grad(prm.ctypes.data, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
     return loss, g
+
+
+# ---------------------------------------------------------------------------- section tables (DOS)
+class DosConeC(C.Structure):
+    _fields_ = [("half_deg", C.c_double), ("max_packing", C.c_int), ("ui_weight", C.c_double),
+                ("initial_step", C.c_double), ("covered", C.c_double), ("d_sigma", C.c_double),
+                ("r_sigma", C.c_double)]
+
+
+class DosC(C.Structure):
+    _fields_ = [("apply_occ", C.c_int), ("apply_sdw", C.c_int), ("shadow_type", C.c_int), ("spot_cos", C.c_double),
+                ("ka", C.c_double), ("kd", C.c_double), ("occ", DosConeC), ("sdw", DosConeC),
+                ("lpos", C.c_double * 3), ("lfwd", C.c_double * 3), ("lup", C.c_double * 3),
+                ("lright", C.c_double * 3)]
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def diagonal_f32(grid):
+    g = np.asarray(grid, np.float32)
+    return np.float32(np.sqrt(np.float32(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])))
+
+
+def dos_cone_spec(native_cone, covered_default) -> DosConeC:
+    """A native.DosCone after ConeGaussianSampler's setter clamps
+    (conegaussiansampler.cpp:57-172); covered_default is used when covered_dist <= 0."""
+    c = DosConeC()
+    c.half_deg = _f32(min(max(native_cone.half_deg, 0.5), 89.5))
+    c.max_packing = min(max(native_cone.max_packing, 0), 2)
+    c.ui_weight = _f32(native_cone.ui_weight)
+    c.initial_step = _f32(native_cone.initial_step)
+    cov = native_cone.covered_dist if native_cone.covered_dist > 0 else covered_default
+    c.covered = _f32(max(_f32(cov), 10.0))
+    c.d_sigma = _f32(min(max(native_cone.d_sigma, 1.0), 3.0))
+    c.r_sigma = _f32(min(max(native_cone.r_sigma, 0.5), 3.0))
+    return c
+
+
+def dos_params(p, grid, light=None, cam=None) -> DosC:
+    """From a native.DosParams, the volume's world size and the light (or, without one,
+    the camera: a light at the eye along the view axes)."""
+    d = DosC()
+    d.apply_occ, d.apply_sdw, d.shadow_type = p.apply_occlusion, p.apply_shadow, p.shadow_type
+    d.spot_cos = float(np.cos(p.spot_max_deg * np.pi / 180.0))
+    d.ka, d.kd = _f32(p.ka), _f32(p.kd)
+    diag = diagonal_f32(grid)
+    d.occ = dos_cone_spec(p.occ, np.float32(diag * np.float32(0.5)))
+    d.sdw = dos_cone_spec(p.sdw, np.float32(diag * np.float32(0.75)))
+    if light is not None:
+        d.lpos[:], d.lfwd[:], d.lup[:], d.lright[:] = (list(light.position), list(light.forward), list(light.up),
+                                                       list(light.right))
+    return d
+
+
+def dos_sections(cone: DosConeC, sigma0=1.0, max_rows=1 << 16):
+    rows = np.zeros((max_rows, 8))
+    counts = np.zeros(3, np.int32)
+    axes = np.zeros((10, 3))
+    adj7 = C.c_double()
+    n = lib().orc_dos_sections(C.byref(cone), sigma0, rows.ctypes.data, max_rows, counts.ctypes.data,
+                               axes.ctypes.data, C.byref(adj7))
+    if n < 0:
+        raise ValueError("section table could not be built")
+    return rows[:n], counts, axes, adj7.value
+
+
+def dos_cone(pyr: OPyr, cone: DosConeC, pos, k, u, v, sigma0=1.0):
+    a = [np.ascontiguousarray(x, np.float64) for x in (pos, k, u, v)]
+    return lib().orc_dos_cone(pyr.h, C.byref(cone), sigma0, *[x.ctypes.data for x in a])
+
+
+def render_dos(vol, tf: OTF, pyr: OPyr, prm: DosC, cam, W, H, sigma0=1.0, step=0.5, et=0.99, scale=(1, 1, 1),
+               fp32=False, threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    if fp32:
+        out = np.zeros((H, W, 4), np.float32)
+        lib().orc_render_dos_f32(*va, tf.h, pyr.h, C.byref(prm), sigma0, c.ctypes.data, W, H, step, et,
+                                 out.ctypes.data, threads, stride)
+        return out
+    out = np.zeros((H, W, 4), np.float64)
+    margin = np.zeros((H, W, 2), np.float64)
+    lib().orc_render_dos_f64(*va, tf.h, pyr.h, C.byref(prm), sigma0, c.ctypes.data, W, H, step, et, out.ctypes.data,
+                             margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+# ---------------------------------------------------------------------------- cone ground truth
+class GtC(C.Structure):
+    _fields_ = [("apply_occ", C.c_int), ("apply_sdw", C.c_int), ("shadow_type", C.c_int), ("n_occ", C.c_int),
+                ("n_sdw", C.c_int), ("occ_aperture_deg", C.c_double), ("sdw_aperture_deg", C.c_double),
+                ("occ_dist", C.c_double), ("sdw_dist", C.c_double), ("gap", C.c_double), ("rstep", C.c_double),
+                ("ka", C.c_double), ("kd", C.c_double), ("lpos", C.c_double * 3), ("lfwd", C.c_double * 3),
+                ("lup", C.c_double * 3), ("lright", C.c_double * 3)]
+
+
+def gt_params(p, grid, light=None) -> GtC:
+    """From a native.GtParams; the shader's LightCamForward is minus the light's forward
+    (crtgtrenderer.cpp:224-234)."""
+    g = GtC()
+    g.apply_occ, g.apply_sdw, g.shadow_type = p.apply_occlusion, p.apply_shadow, p.shadow_type
+    g.n_occ, g.n_sdw = p.occ_rays, p.sdw_rays
+    g.occ_aperture_deg, g.sdw_aperture_deg = _f32(p.occ_aperture_deg), _f32(p.sdw_aperture_deg)
+    diag = diagonal_f32(grid)
+    g.occ_dist = _f32(p.occ_max_dist) if p.occ_max_dist > 0 else float(np.float32(diag * np.float32(0.5)))
+    g.sdw_dist = _f32(p.sdw_max_dist) if p.sdw_max_dist > 0 else float(np.float32(diag * np.float32(0.75)))
+    g.gap, g.rstep, g.ka, g.kd = _f32(p.initial_gap), _f32(p.ray_step), _f32(p.ka), _f32(p.kd)
+    if light is not None:
+        g.lpos[:] = list(light.position)
+        g.lfwd[:] = [-float(x) for x in light.forward]
+        g.lup[:] = list(light.up)
+        g.lright[:] = list(light.right)
+    return g
+
+
+def gt_rays(prm: GtC):
+    occ = np.zeros((prm.n_occ, 3))
+    sdw = np.zeros((prm.n_sdw, 3))
+    lib().orc_gt_rays(C.byref(prm), occ.ctypes.data, sdw.ctypes.data)
+    return occ, sdw
+
+
+def gt_ray(vol, tf: OTF, pos, w, dist, gap=1.0, rstep=0.5, scale=(1, 1, 1)):
+    vol, va = _vol_args(vol, scale)
+    p = np.ascontiguousarray(pos, np.float64)
+    ww = np.ascontiguousarray(w, np.float64)
+    return lib().orc_gt_ray(*va, tf.h, p.ctypes.data, ww.ctypes.data, dist, gap, rstep)
+
+
+def render_gt(vol, tf: OTF, prm: GtC, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1), fp32=False, threads=0,
+              stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    if fp32:
+        out = np.zeros((H, W, 4), np.float32)
+        lib().orc_render_gt_f32(*va, tf.h, C.byref(prm), c.ctypes.data, W, H, step, et, out.ctypes.data, threads,
+                                stride)
+        return out
+    out = np.zeros((H, W, 4), np.float64)
+    margin = np.zeros((H, W, 2), np.float64)
+    lib().orc_render_gt_f64(*va, tf.h, C.byref(prm), c.ctypes.data, W, H, step, et, out.ctypes.data,
+                            margin.ctypes.data, threads, stride)
+    return out, margin
