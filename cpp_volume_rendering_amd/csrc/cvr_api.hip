@@ -98,6 +98,7 @@ static void cone_defaults(cvr_cone_params* c) {
   c->r_sigma = 2.0f;
   c->ka = 0.5f;
   c->kd = 0.5f;
+  c->density_precision = 0;
 }
 
 static void defaults(int technique, void* p) {
@@ -156,6 +157,7 @@ static void defaults(int technique, void* p) {
     d->sdw = cvr_dos_cone{0.5f, 0, 1.0f, 3.0f, 0.f, 1.25f, 2.0f};
     d->ka = 0.5f;
     d->kd = 0.5f;
+    d->density_precision = 0;
   } else if (technique == CVR_CONE_TRACE) {
     cone_defaults(static_cast<cvr_cone_params*>(p));
   } else if (technique == CVR_CONE_NN) {
@@ -171,6 +173,8 @@ static void defaults(int technique, void* p) {
     n->ood_margin = 0.02f;
     n->force_exact = 0;
     n->seed = 4;
+    n->tile_w = 8;
+    n->shade_batch = 0.5f;
   }
 }
 
@@ -415,6 +419,14 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
                                                                 : static_cast<const cvr_nn_params*>(params)->cone;
     int rc = check_cone_params(cp);
     if (rc) return rc;
+  }
+  if (h->technique == CVR_CONE_NN) {
+    const cvr_nn_params* q = static_cast<const cvr_nn_params*>(params);
+    if ((q->tile_w != 4 && q->tile_w != 8 && q->tile_w != 16 && q->tile_w != 32) ||
+        !(q->shade_batch >= 0.f && q->shade_batch <= 1.f)) {
+      set_error("cvr_set_params: tile_w must be 4, 8, 16 or 32 and shade_batch in [0, 1]");
+      return CVR_ERR_INVALID;
+    }
   }
   if (h->technique == CVR_CONE_NN) {
     // training inputs changed: the network is retrained on the next prepare/render

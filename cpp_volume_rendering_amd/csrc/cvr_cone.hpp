@@ -2,6 +2,7 @@
 // directional occlusion / shadow, and the surrogate's input features.
 #pragma once
 #include "cvr_ctx.hpp"
+#include <hip/hip_fp16.h>
 
 namespace cvr {
 
@@ -29,12 +30,23 @@ struct ConeK {
   const StencilF* st;       // surrogate stencil: axial/ring pairs [2*kNOcc], then shadow [kNSdw]
 };
 
+// One neighbour block of the pyramid in its storage format: fp32 (float4), or four
+// 16-bit values (fp16 or bf16) in 8 bytes.
+__device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, int fmt, size_t i) {
+  if (fmt == 0) return static_cast<const float4*>(pyr)[i];
+  const uint2 r = static_cast<const uint2*>(pyr)[i];
+  if (fmt == 1)
+    return make_float4(__half2float(__ushort_as_half(ushort(r.x & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.x >> 16))),
+                       __half2float(__ushort_as_half(ushort(r.y & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.y >> 16))));
+  return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xFFFF0000u), __uint_as_float(r.y << 16),
+                     __uint_as_float(r.y & 0xFFFF0000u));
+}
+
 // GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w) through the
-// quad layout (2 float4 loads per trilinear read).
-__device__ __forceinline__ float pyr_tri(const float4* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
+// neighbour blocks (2 vector loads per trilinear read).
+__device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
                                          float w) {
   const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
-  const float4* __restrict__ a = pyr + pg.off[l];
   int x0, y0;
   float fx, fy;
   quad_axis(u * float(nx) - 0.5f, nx, x0, fx);
@@ -44,8 +56,8 @@ __device__ __forceinline__ float pyr_tri(const float4* __restrict__ pyr, const P
   const float fz = zt - zf;
   const int zi = int(zf);
   const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
-  const int row = y0 * nx + x0, plane = nx * ny;
-  return quad_lerp(a[row + z0 * plane], a[row + z1 * plane], fx, fy, fz);
+  const size_t row = pg.off[l] + size_t(y0 * nx + x0), plane = size_t(nx * ny);
+  return quad_lerp(pyr_block(pyr, pg.fmt, row + z0 * plane), pyr_block(pyr, pg.fmt, row + z1 * plane), fx, fy, fz);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
@@ -61,7 +73,7 @@ __device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float 
 
 // textureLod with LINEAR_MIPMAP_LINEAR (lerp of levels floor(lod) and floor(lod)+1,
 // the second clamped to the chain), then the border falloff.
-__device__ __forceinline__ float gauss_ext(const float4* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
+__device__ __forceinline__ float gauss_ext(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
   const float q = float(pg.levels - 1);
   lod = fminf(fmaxf(lod, 0.f), q);
   const int d1 = int(floorf(lod));
@@ -75,13 +87,13 @@ __device__ __forceinline__ float gauss_ext(const float4* __restrict__ pyr, const
 }
 
 // The same lookup at a whole level `l` (one trilinear read).
-__device__ __forceinline__ float gauss_ext_level(const float4* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
+__device__ __forceinline__ float gauss_ext_level(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
   const float sg = float(1 << l);
   return pyr_tri(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
-__device__ __forceinline__ float cone_T(const float4* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
+__device__ __forceinline__ float cone_T(const void* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
                                         int n, f3 p, f3 axis) {
   float od = 0.f;
 #pragma unroll 4
@@ -101,7 +113,7 @@ __device__ __forceinline__ void occ_frame(f3 dir, f3& k, f3& up, f3& right) {
 
 // 1 axial cone + (n_occ-1) cones whose axes sit on a ring at ring angle from the axis,
 // cosine-weighted like gt_ray_marching.comp:161-168.
-__device__ __forceinline__ float ct_occlusion(const float4* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
+__device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
                                               f3 k, f3 up, f3 right) {
   float num = cone_T(pyr, pg, K.so, K.n_steps, p, k), den = 1.f;
   const int nr = K.n_occ - 1;
@@ -118,7 +130,7 @@ __device__ __forceinline__ float ct_occlusion(const float4* __restrict__ pyr, co
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
 // opacities along the light (8..14), the local opacity (15).
-__device__ __forceinline__ void nn_feats(const float4* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
+__device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
                                          f3 p, f3 k, f3 L, float* f) {
 #pragma unroll
   for (int i = 0; i < 2 * kNOcc; ++i) {

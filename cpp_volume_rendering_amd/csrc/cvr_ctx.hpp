@@ -9,6 +9,7 @@ constexpr int kMaxLevels = 12;
 // Mip chain of the extinction pyramid: level dims and offsets into one fp32 buffer.
 struct PyrGeom {
   int levels;
+  int fmt;  // storage of the neighbour blocks: 0 fp32 (float4), 1 fp16, 2 bf16 (4 x 16 bit)
   int dims[kMaxLevels][3];
   size_t off[kMaxLevels];
   float G[3];
@@ -57,7 +58,7 @@ struct cvr_ctx {
 
   // Gaussian extinction pyramid (cone tracers, surrogate)
   float* d_pyr = nullptr;
-  float4* d_pyr_quad = nullptr;  // float4 neighbour blocks of every level
+  void* d_pyr_quad = nullptr;  // neighbour blocks of every level (PyrGeom::fmt)
   cvr::PyrGeom pyr{};
   uint64_t pyr_built_for = ~0ull;
   void* d_cone_tab = nullptr;  // cone step schedules + surrogate stencil
@@ -102,8 +103,8 @@ int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st);
 int build_sat(cvr_ctx* c, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int build_volume_quad(cvr_ctx* c, hipStream_t st);
-int build_pyramid_quad(cvr_ctx* c, size_t total, hipStream_t st);
-int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, hipStream_t st);
+int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st);
+int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st);
 int check_cone_params(const cvr_cone_params& P);
 int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);
 int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);

@@ -1,0 +1,230 @@
+// Surrogate network inference shared by the single-frame and batched renderers:
+// bf16 MFMA types and layout constants, the wavefront-cooperative forward pass,
+// and the per-tile surrogate march.
+#pragma once
+#include "cvr_cone.hpp"
+
+namespace cvr {
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kH = 64, kOut = 16;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kNFeat;
+constexpr int kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kH;
+constexpr int kOffW4 = kOffB3 + kH, kOffB4 = kOffW4 + kOut * kH;
+constexpr int kNParams = kOffB4 + kOut;  // 10448
+// Packed inference weights: 40 A-operand tiles of 64 lanes x 4 bf16.
+constexpr int kTiles1 = 4, kTiles2 = 16, kTiles3 = 16, kTiles4 = 4;
+constexpr int kNTiles = kTiles1 + kTiles2 + kTiles3 + kTiles4;
+constexpr int kPackShorts = kNTiles * 64 * 4;
+constexpr int kNBias = kH + kH + kH + kOut;
+constexpr int kWaveScratch = kNFeat * 64 + 2 * 64;
+
+// fp32 -> bf16, round to nearest even (inputs are finite)
+__device__ __forceinline__ short f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(uint32_t(uint16_t(b)) << 16); }
+
+__device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Wavefront-cooperative inference: every lane brings one sample's 16 features; the
+// 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
+// and biases in LDS; sX: this wave's LDS scratch of kWaveScratch floats (16x64 inputs,
+// then 64x2 outputs).  Returns outputs 0 and 1.
+__device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const float* __restrict__ sB, float* sX,
+                                         const float* f, float& o0, float& o1) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < kNFeat; ++i) sX[i * 64 + lane] = f[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const bf16x4_t* W = reinterpret_cast<const bf16x4_t*>(sW);
+  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
+  // Sample tiles are independent through the network: each 16-sample column tile goes
+  // through all four layers before the next, so only its 4 activation tiles are live.
+#pragma unroll 1
+  for (int nt = 0; nt < 4; ++nt) {
+    bf16x4_t xb, hb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xb[e] = f2bf(sX[(4 * q + e) * 64 + nt * 16 + r]);
+    // layer 1: K = 16
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4_t acc;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[e] = sB[mt * 16 + 4 * q + e];
+      acc = mfma16(W[mt * 64 + lane], xb, acc);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) hb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
+    }
+    // layers 2 and 3: K = 64
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
+      const float* bl = sB + kH * (l + 1);
+      bf16x4_t nb[4];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        f32x4_t acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[e] = bl[mt * 16 + 4 * q + e];
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt], acc);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) nb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) hb[mt] = nb[mt];
+    }
+    // layer 4: 16 outputs (rows 0, 1 used)
+    f32x4_t acc;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[e] = sB[3 * kH + 4 * q + e];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt], acc);
+    // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r
+    if (q == 0) {
+      sX[kNFeat * 64 + (nt * 16 + r) * 2] = acc[0];
+      sX[kNFeat * 64 + (nt * 16 + r) * 2 + 1] = acc[1];
+    }
+  }
+  // lanes 0..15 wrote outputs 0 and 1 of every sample; hand them back
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  o0 = sX[kNFeat * 64 + lane * 2];
+  o1 = sX[kNFeat * 64 + lane * 2 + 1];
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------- surrogate renderer
+struct NnK {
+  float lo[kNFeat], hi[kNFeat];  // training range (+- margin): outside -> exact tracer
+  int exact_all;                 // validation gate failed: every sample uses the exact tracer
+};
+
+// Shape of the work of one wavefront: an (1 << tw_log2) x (64 >> tw_log2) pixel tile
+// (four of them, 2 x 2, per workgroup), and how many of the wave's marching lanes
+// must be waiting on a shaded sample before the wave gathers their features and runs
+// the network together (`park`: a fraction of the active lanes; 0 = as soon as one
+// waits).  Neither changes any pixel: every ray takes the same samples in the same
+// order and the network maps each sample independently of the others.
+struct NnSched {
+  int tw_log2;
+  float park;
+};
+
+NnK nn_consts(cvr_ctx* c);
+ConeK cone_constants(const cvr_ctx* c, const cvr_cone_params& P, const float L[3]);
+void light_dir(const RayFrame& rf, const cvr_light* light, float L[3]);
+
+// Renders one workgroup tile (2 x 2 wave tiles of the NnSched shape) of frame `rf`.
+// sW/sB/s_tf: the network and transfer function staged in LDS; sXw: this wave's LDS
+// scratch.  Counters (samples, shaded, exact fallbacks) accumulate into cnt.
+template <typename T>
+__device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, const PyrGeom& pg,
+                                        const void* __restrict__ pyr, const ConeK& K, const NnK& nk,
+                                        const NnSched& sc, const short* sW, const float* sB, float* sXw,
+                                        const float4* s_tf, int ntf, float step, float et, float4* __restrict__ out,
+                                        int tx, int ty, unsigned int* cnt) {
+  const int tw = 1 << sc.tw_log2, th = 64 >> sc.tw_log2;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx * 2 * tw + (w & 1) * tw + (lane & (tw - 1));
+  const int py = ty * 2 * th + (w >> 1) * th + (lane >> sc.tw_log2);
+  const bool in_img = px < rf.W && py < rf.H;
+  const f3 dir = camera_ray(rf, in_img ? px : 0, in_img ? py : 0);
+  const f3 eye = ld3(rf.eye);
+  float tn = 0.f, tf_ = 0.f;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float ka = K.apply_occ ? K.ka : 0.f, kd = K.apply_sdw ? K.kd : 0.f;
+  const float inv_k = 1.0f / (ka + kd);
+  bool active = in_img && ray_box(eye, dir, g.G, tn, tf_);
+  f3 k, up, right;
+  occ_frame(dir, k, up, right);
+  const f3 L = ld3(K.L);
+  const float D = fabsf(tf_ - tn);
+  const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
+  int j = 0;
+  // A lane whose sample needs shading parks on it (keeps p, h, src) while the other
+  // lanes march on; the wave shades all parked samples together once enough lanes
+  // wait, or when every marching lane waits.  The loop runs while any lane marches.
+  bool parked = false;
+  float h = 0.f;
+  f3 p = tp;
+  float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
+  while (__any(active)) {
+    if (active && !parked) {
+      const float s = float(j) * step;
+      if (!(s < D) || j >= kMaxSamples) {
+        active = false;
+      } else {
+        h = fminf(step, D - s);
+        p = tp + dir * (s + 0.5f * h);
+        src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
+        ++cnt[0];
+        if (src.w > 0.f) parked = true;
+        else ++j;
+      }
+    }
+    const int np = __popcll(__ballot(parked)), na = __popcll(__ballot(active));
+    if (np > 0 && (np == na || float(np) >= sc.park * float(na))) {
+      float f[kNFeat];
+      bool ood = false;
+      if (parked) {
+        nn_feats(pyr, pg, K.st, p, k, L, f);
+#pragma unroll
+        for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
+      } else {
+#pragma unroll
+        for (int c = 0; c < kNFeat; ++c) f[c] = 0.f;
+      }
+      float Io, Is;
+      mlp_wave(sW, sB, sXw, f, Io, Is);
+      if (parked) {
+        float bo, bs;
+        nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
+        Io += bo;
+        Is += bs;
+        ++cnt[1];
+        if (ood || nk.exact_all) {
+          ++cnt[2];
+          Io = K.apply_occ ? ct_occlusion(pyr, pg, K, p, k, up, right) : 0.f;
+          Is = K.apply_sdw ? cone_T(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
+        }
+        Io = fminf(fmaxf(Io, 0.f), 1.f);
+        Is = fminf(fmaxf(Is, 0.f), 1.f);
+        const float m = inv_k * (Io * ka + Is * kd);
+        const float a = 1.0f - __expf(-src.w * h);
+        const float wgt = (1.0f - dst.w) * a;
+        dst.x += wgt * src.x * m;
+        dst.y += wgt * src.y * m;
+        dst.z += wgt * src.z * m;
+        dst.w += wgt;
+        ++j;
+        parked = false;
+        if (dst.w > et) active = false;
+      }
+    }
+  }
+  if (in_img) out[size_t(py) * rf.W + px] = dst;
+}
+
+// Stages the packed network and the transfer function of one volume in LDS (all
+// threads of the workgroup take part; the caller synchronises).
+__device__ __forceinline__ void nn_stage(const short* __restrict__ wpack, const float* __restrict__ bpack,
+                                         const float4* __restrict__ tf, int ntf, short* sW, float* sB, float4* s_tf) {
+  for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
+    reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
+  for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
+  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+}
+
+}  // namespace cvr

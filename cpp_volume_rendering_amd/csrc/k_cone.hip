@@ -20,7 +20,7 @@
 namespace cvr {
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, PyrGeom pg, const float4* __restrict__ pyr,
+__global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                      ConeK K, const float4* __restrict__ tf, int ntf, float step,
                                                      float et, float4* __restrict__ out,
                                                      unsigned long long* __restrict__ counters) {
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
 
 // Exact occlusion / shadow and the surrogate's features at given points (texture
 // space), for given camera-ray directions and light directions.
-__global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const float4* __restrict__ pyr, ConeK K, int n,
+__global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const void* __restrict__ pyr, ConeK K, int n,
                                                      const float* __restrict__ pos, const float* __restrict__ dirs,
                                                      const float* __restrict__ lights, float* __restrict__ occ,
                                                      float* __restrict__ sdw, float* __restrict__ feat) {
@@ -142,8 +142,10 @@ static void stencil(const cvr_cone_params& P, double occ_max, double sdw_max, in
 int check_cone_params(const cvr_cone_params& P) {
   if (P.n_steps < 1 || P.n_steps > kMaxSteps || P.n_occ_cones < 1 || P.n_occ_cones > 64 || !(P.t0 > 0) ||
       !(P.r_sigma > 0) || !(P.sigma0 > 0) || !(P.occ_half_deg > 0 && P.occ_half_deg < 90) ||
-      !(P.sdw_half_deg > 0 && P.sdw_half_deg < 90) || P.n_levels > kMaxLevels) {
-    set_error("cone params: n_steps in [1,256], n_occ_cones in [1,64], t0, r_sigma, sigma0 > 0, half angles in (0,90)");
+      !(P.sdw_half_deg > 0 && P.sdw_half_deg < 90) || P.n_levels > kMaxLevels || P.density_precision < 0 ||
+      P.density_precision > 2) {
+    set_error("cone params: n_steps in [1,256], n_occ_cones in [1,64], t0, r_sigma, sigma0 > 0, half angles in (0,90), "
+              "density_precision 0, 1 or 2");
     return -1;
   }
   if (P.apply_occlusion == 0 && P.apply_shadow == 0) {
@@ -160,7 +162,7 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   const int bx = P.base_res[0] > 0 ? P.base_res[0] : c->nx;
   const int by = P.base_res[1] > 0 ? P.base_res[1] : c->ny;
   const int bz = P.base_res[2] > 0 ? P.base_res[2] : c->nz;
-  rc = build_pyramid(c, bx, by, bz, P.sigma0, P.n_levels, st);
+  rc = build_pyramid(c, bx, by, bz, P.sigma0, P.n_levels, P.density_precision, st);
   if (rc) return rc;
   const double diag = volume_diagonal(c);
   const double occ_max = P.occ_max_dist > 0 ? P.occ_max_dist : 0.5 * float(diag);

@@ -43,7 +43,7 @@ struct DosK {
 
 // GetGaussianExtinction (ray_bbox_marching.comp:92-112) at a section's level: textureLod
 // clamps the level to the chain, the border falloff uses the level as given.
-__device__ __forceinline__ float dos_ext(const float4* __restrict__ pyr, const PyrGeom& pg, f3 tp, float level) {
+__device__ __forceinline__ float dos_ext(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, float level) {
   const float lc = fminf(fmaxf(level, 0.f), float(pg.levels - 1));
   const int d1 = int(floorf(lc));
   const float fr = lc - float(d1);
@@ -55,7 +55,7 @@ __device__ __forceinline__ float dos_ext(const float4* __restrict__ pyr, const P
 }
 
 // One cone through its section table; sub-cone axes are k*z + uy*y + vx*x.
-__device__ __forceinline__ float dos_cone(const float4* __restrict__ pyr, const PyrGeom& pg, const DosConeK& C,
+__device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const PyrGeom& pg, const DosConeK& C,
                                           const float4* __restrict__ rows, f3 p, f3 k, f3 uy, f3 vx) {
   float od[7], lt[7];
   float track = C.init;
@@ -115,7 +115,7 @@ __device__ __forceinline__ float dos_cone(const float4* __restrict__ pyr, const 
 }
 
 // Shadow cone of ShadowEvaluationKernel (:533-562).  `wp` is the world position.
-__device__ __forceinline__ float dos_shadow(const float4* __restrict__ pyr, const PyrGeom& pg, const DosK& K, f3 p,
+__device__ __forceinline__ float dos_shadow(const void* __restrict__ pyr, const PyrGeom& pg, const DosK& K, f3 p,
                                             f3 wp) {
   if (K.shadow_type == 2) return dos_cone(pyr, pg, K.sdw, K.sdw_rows, p, ld3(K.lfwd), ld3(K.lup), ld3(K.lright));
   const f3 k = norm3(ld3(K.lpos) - wp);
@@ -126,7 +126,7 @@ __device__ __forceinline__ float dos_shadow(const float4* __restrict__ pyr, cons
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrGeom pg, const float4* __restrict__ pyr,
+__global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                     DosK K, const float4* __restrict__ tf, int ntf, float step, float et,
                                                     float4* __restrict__ out, unsigned long long* __restrict__ counters) {
   extern __shared__ float4 s_tf[];
@@ -316,8 +316,9 @@ int check_dos_params(const cvr_dos_params& P) {
     return CVR_ERR_INVALID;
   }
   if (!(P.sigma0 > 0) || P.n_levels > kMaxLevels || P.shadow_type < 0 || P.shadow_type > 2 || !(P.et_threshold > 0) ||
-      !(P.occ.initial_step > 0) || !(P.sdw.initial_step > 0)) {
-    set_error("dos params: sigma0 > 0, n_levels <= 12, shadow_type in {0,1,2}, et_threshold > 0, initial steps > 0");
+      !(P.occ.initial_step > 0) || !(P.sdw.initial_step > 0) || P.density_precision < 0 || P.density_precision > 2) {
+    set_error("dos params: sigma0 > 0, n_levels <= 12, shadow_type in {0,1,2}, et_threshold > 0, initial steps > 0, "
+              "density_precision 0, 1 or 2");
     return CVR_ERR_INVALID;
   }
   return CVR_OK;
@@ -329,7 +330,7 @@ int prepare_dos(cvr_ctx* c, const cvr_dos_params& P, hipStream_t st) {
   const int bx = P.base_res[0] > 0 ? P.base_res[0] : c->nx;
   const int by = P.base_res[1] > 0 ? P.base_res[1] : c->ny;
   const int bz = P.base_res[2] > 0 ? P.base_res[2] : c->nz;
-  rc = build_pyramid(c, bx, by, bz, P.sigma0, P.n_levels, st);
+  rc = build_pyramid(c, bx, by, bz, P.sigma0, P.n_levels, P.density_precision, st);
   if (rc) return rc;
   // SetCoveredDistance(diagonal * 0.50f / 0.75f) in float (dosrcrenderer.cpp:112-113)
   const float diag = volume_diagonal(c);

@@ -15,40 +15,12 @@
 // samples, keeps the activations of one group in LDS, runs the backward GEMMs
 // (W^T dY and dY H^T, operands gathered from LDS with transposed strides) and
 // accumulates the weight gradients in LDS; one atomic flush per workgroup.
-#include "cvr_cone.hpp"
+#include "cvr_nn.hpp"
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 namespace cvr {
-
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
-constexpr int kH = 64, kOut = 16;
-constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kNFeat;
-constexpr int kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH;
-constexpr int kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kH;
-constexpr int kOffW4 = kOffB3 + kH, kOffB4 = kOffW4 + kOut * kH;
-constexpr int kNParams = kOffB4 + kOut;  // 10448
-// Packed inference weights: 40 A-operand tiles of 64 lanes x 4 bf16.
-constexpr int kTiles1 = 4, kTiles2 = 16, kTiles3 = 16, kTiles4 = 4;
-constexpr int kNTiles = kTiles1 + kTiles2 + kTiles3 + kTiles4;
-constexpr int kPackShorts = kNTiles * 64 * 4;
-constexpr int kNBias = kH + kH + kH + kOut;
-constexpr int kWaveScratch = kNFeat * 64 + 2 * 64;
-
-// fp32 -> bf16, round to nearest even (inputs are finite)
-__device__ __forceinline__ short f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return short(u >> 16);
-}
-__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(uint32_t(uint16_t(b)) << 16); }
-
-__device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
 
 // Tile (m0.., n0..) += sum_k A(m,k) B(k,n) with A(m,k) = a[m*am + k*ak], B(k,n) = b[k*bk + n*bn].
 __device__ __forceinline__ f32x4_t tile_gemm(const float* a, int am, int ak, const float* b, int bk, int bn, int m0,
@@ -230,77 +202,6 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, short* __restrict__ wp
   }
 }
 
-// Wavefront-cooperative inference: every lane brings one sample's 16 features; the
-// 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
-// and biases in LDS; sX: this wave's LDS scratch of kWaveScratch floats (16x64 inputs,
-// then 64x2 outputs).  Returns outputs 0 and 1.
-__device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const float* __restrict__ sB, float* sX,
-                                         const float* f, float& o0, float& o1) {
-  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-#pragma unroll
-  for (int i = 0; i < kNFeat; ++i) sX[i * 64 + lane] = f[i];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const bf16x4_t* W = reinterpret_cast<const bf16x4_t*>(sW);
-  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
-  // Sample tiles are independent through the network: each 16-sample column tile goes
-  // through all four layers before the next, so only its 4 activation tiles are live.
-#pragma unroll 1
-  for (int nt = 0; nt < 4; ++nt) {
-    bf16x4_t xb, hb[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) xb[e] = f2bf(sX[(4 * q + e) * 64 + nt * 16 + r]);
-    // layer 1: K = 16
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      f32x4_t acc;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] = sB[mt * 16 + 4 * q + e];
-      acc = mfma16(W[mt * 64 + lane], xb, acc);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
-    }
-    // layers 2 and 3: K = 64
-#pragma unroll
-    for (int l = 0; l < 2; ++l) {
-      const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
-      const float* bl = sB + kH * (l + 1);
-      bf16x4_t nb[4];
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        f32x4_t acc;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e] = bl[mt * 16 + 4 * q + e];
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt], acc);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) nb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
-      }
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) hb[mt] = nb[mt];
-    }
-    // layer 4: 16 outputs (rows 0, 1 used)
-    f32x4_t acc;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = sB[3 * kH + 4 * q + e];
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt], acc);
-    // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r
-    if (q == 0) {
-      sX[kNFeat * 64 + (nt * 16 + r) * 2] = acc[0];
-      sX[kNFeat * 64 + (nt * 16 + r) * 2 + 1] = acc[1];
-    }
-  }
-  // lanes 0..15 wrote outputs 0 and 1 of every sample; hand them back
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  o0 = sX[kNFeat * 64 + lane * 2];
-  o1 = sX[kNFeat * 64 + lane * 2 + 1];
-  __builtin_amdgcn_wave_barrier();
-}
-
 // Batch inference through the register path (the one the renderer uses).
 __global__ __launch_bounds__(256) void k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ bpack,
                                                    const float* __restrict__ X, int n, float* __restrict__ Y) {
@@ -344,7 +245,7 @@ __device__ __forceinline__ f3 unit_sphere(uint64_t& s) {
 // a light direction uniform on the sphere; outputs the 16 features and the exact
 // (occlusion, shadow) of the cone tracer minus the network-free estimate.
 template <typename T>
-__global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const float4* __restrict__ pyr, ConeK K,
+__global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const void* __restrict__ pyr, ConeK K,
                                                     const float4* __restrict__ tf, int ntf, uint64_t seed, int n,
                                                     float* __restrict__ X, float* __restrict__ Tg,
                                                     float* __restrict__ pos_out) {
@@ -396,14 +297,9 @@ __global__ void k_feat_range(const float* __restrict__ X, int n, unsigned int* _
 }
 
 // ---------------------------------------------------------------- surrogate renderer
-struct NnK {
-  float lo[kNFeat], hi[kNFeat];  // training range (+- margin): outside -> exact tracer
-  int exact_all;                 // validation gate failed: every sample uses the exact tracer
-};
-
 template <typename T>
-__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const float4* __restrict__ pyr,
-                                                   ConeK K, NnK nk, const short* __restrict__ wpack,
+__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
+                                                   ConeK K, NnK nk, NnSched sc, const short* __restrict__ wpack,
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
                                                    int ntf, float step, float et, float4* __restrict__ out,
                                                    unsigned long long* __restrict__ counters) {
@@ -411,96 +307,18 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
   __shared__ float sB[kNBias];
   __shared__ float sX[4][kWaveScratch];
   extern __shared__ float4 s_tf[];
-  for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
-    reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
-  for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
-  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+  nn_stage(wpack, bpack, tf, ntf, sW, sB, s_tf);
   __syncthreads();
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
-  const int tiles_x = (rf.W + 15) >> 4;
-  const int tx = bid % tiles_x, ty = bid / tiles_x;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
-  const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
-  const bool in_img = px < rf.W && py < rf.H;
-  const f3 dir = camera_ray(rf, in_img ? px : 0, in_img ? py : 0);
-  const f3 eye = ld3(rf.eye);
-  float tn = 0.f, tf_ = 0.f;
-  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
-  unsigned int nsamp = 0, nshade = 0, nfall = 0;
-  const float ka = K.apply_occ ? K.ka : 0.f, kd = K.apply_sdw ? K.kd : 0.f;
-  const float inv_k = 1.0f / (ka + kd);
-  bool active = in_img && ray_box(eye, dir, g.G, tn, tf_);
-  f3 k, up, right;
-  occ_frame(dir, k, up, right);
-  const f3 L = ld3(K.L);
-  const float D = fabsf(tf_ - tn);
-  const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-  int j = 0;
-  // The loop runs while any lane of the wave marches; each iteration every marching
-  // lane takes its next sample, and when any lane must be shaded the whole wave
-  // evaluates the network together.
-  while (__any(active)) {
-    bool need = false;
-    float h = 0.f;
-    f3 p = tp;
-    float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (active) {
-      const float s = float(j) * step;
-      if (!(s < D) || j >= kMaxSamples) {
-        active = false;
-      } else {
-        h = fminf(step, D - s);
-        p = tp + dir * (s + 0.5f * h);
-        src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
-        ++nsamp;
-        need = src.w > 0.f;
-        if (!need) ++j;
-      }
-    }
-    if (__any(need)) {
-      float f[kNFeat];
-      bool ood = false;
-      if (need) {
-        nn_feats(pyr, pg, K.st, p, k, L, f);
-#pragma unroll
-        for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
-      } else {
-#pragma unroll
-        for (int c = 0; c < kNFeat; ++c) f[c] = 0.f;
-      }
-      float Io, Is;
-      mlp_wave(sW, sB, sX[w], f, Io, Is);
-      if (need) {
-        float bo, bs;
-        nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
-        Io += bo;
-        Is += bs;
-        ++nshade;
-        if (ood || nk.exact_all) {
-          ++nfall;
-          Io = K.apply_occ ? ct_occlusion(pyr, pg, K, p, k, up, right) : 0.f;
-          Is = K.apply_sdw ? cone_T(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
-        }
-        Io = fminf(fmaxf(Io, 0.f), 1.f);
-        Is = fminf(fmaxf(Is, 0.f), 1.f);
-        const float m = inv_k * (Io * ka + Is * kd);
-        const float a = 1.0f - __expf(-src.w * h);
-        const float wgt = (1.0f - dst.w) * a;
-        dst.x += wgt * src.x * m;
-        dst.y += wgt * src.y * m;
-        dst.z += wgt * src.z * m;
-        dst.w += wgt;
-        ++j;
-        if (dst.w > et) active = false;
-      }
-    }
-  }
-  if (in_img) out[size_t(py) * rf.W + px] = dst;
-  if (counters && in_img) {
-    atomicAdd(&counters[0], (unsigned long long)nsamp);
-    atomicAdd(&counters[1], (unsigned long long)nshade);
-    atomicAdd(&counters[2], (unsigned long long)nfall);
+  const int tw = 1 << sc.tw_log2;
+  const int tiles_x = (rf.W + 2 * tw - 1) / (2 * tw);
+  unsigned int cnt[3] = {0, 0, 0};
+  nn_tile<T>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step, et, out, bid % tiles_x,
+             bid / tiles_x, cnt);
+  if (counters) {
+    atomicAdd(&counters[0], (unsigned long long)cnt[0]);
+    atomicAdd(&counters[1], (unsigned long long)cnt[1]);
+    atomicAdd(&counters[2], (unsigned long long)cnt[2]);
   }
 }
 
@@ -744,6 +562,18 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   return 0;
 }
 
+// Render-time constants of the surrogate: the training range widened by the margin,
+// and whether the validation gate (or force_exact) sends every sample to the exact tracer.
+NnK nn_consts(cvr_ctx* c) {
+  const cvr_nn_params& N = c->nn;
+  NnK nk{};
+  for (int i = 0; i < kNFeat; ++i) { nk.lo[i] = c->nn_lo[i] - N.ood_margin; nk.hi[i] = c->nn_hi[i] + N.ood_margin; }
+  c->nn_report.exact_fallback =
+      (std::max(c->nn_report.val_rmse_occ, c->nn_report.val_rmse_sdw) > N.rmse_gate) ? 1 : 0;
+  nk.exact_all = (N.force_exact || c->nn_report.exact_fallback) ? 1 : 0;
+  return nk;
+}
+
 int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
   const cvr_nn_params& N = c->nn;
   int rc = prepare_cone(c, N.cone, st);
@@ -755,11 +585,7 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   float L[3];
   light_dir(rf, light, L);
   ConeK K = cone_constants(c, N.cone, L);
-  NnK nk{};
-  for (int i = 0; i < kNFeat; ++i) { nk.lo[i] = c->nn_lo[i] - N.ood_margin; nk.hi[i] = c->nn_hi[i] + N.ood_margin; }
-  c->nn_report.exact_fallback =
-      (std::max(c->nn_report.val_rmse_occ, c->nn_report.val_rmse_sdw) > N.rmse_gate) ? 1 : 0;
-  nk.exact_all = (N.force_exact || c->nn_report.exact_fallback) ? 1 : 0;
+  const NnK nk = nn_consts(c);
   if (nk.exact_all) {
     // the validation gate failed: the exact tracer renders the frame
     cvr_cone_params saved = c->cone;
@@ -770,18 +596,21 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   }
   VolGeom g = geom_of(c);
   const float step = N.cone.step_size > 0 ? N.cone.step_size : default_step(c);
-  const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
+  NnSched sc{0, N.shade_batch};
+  while ((1 << sc.tw_log2) < N.tile_w) ++sc.tw_log2;
+  const int tw = 1 << sc.tw_log2, th = 64 / tw;
+  const int tiles = ((rf.W + 2 * tw - 1) / (2 * tw)) * ((rf.H + 2 * th - 1) / (2 * th));
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 32 * 1024) { set_error("surrogate renderer: transfer function larger than 2048 entries"); return -4; }
   short* wp = static_cast<short*>(c->d_nn_pack);
   float* bp = reinterpret_cast<float*>(wp + kPackShorts);
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, wp, bp,
-                       c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
+    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, sc, wp,
+                       bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
   else
-    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, wp, bp,
-                       c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
+    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, sc,
+                       wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }

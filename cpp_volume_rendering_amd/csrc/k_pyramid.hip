@@ -127,17 +127,19 @@ static int mip_count(int bx, int by, int bz) {
   return c;
 }
 
-int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, hipStream_t st) {
+int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st) {
   if (base_x < 1 || base_y < 1 || base_z < 1) { set_error("pyramid: base resolution must be >= 1"); return -1; }
   if (!(sigma0 > 0)) { set_error("pyramid: sigma0 must be > 0"); return -1; }
   const uint64_t key = (c->vol_version << 32) ^ (c->tf_version << 8) ^ uint64_t(base_x * 131 + base_y * 7 + base_z) ^
-                       (uint64_t(nlev) << 56) ^ uint64_t(double(sigma0) * 1e6);
+                       (uint64_t(nlev) << 56) ^ uint64_t(double(sigma0) * 1e6) ^ (uint64_t(fmt) << 60);
+  if (fmt < 0 || fmt > 2) { set_error("pyramid: density_precision must be 0 (fp32), 1 (fp16) or 2 (bf16)"); return -1; }
   if (c->d_pyr && c->pyr_built_for == key) return 0;
   const int full = mip_count(base_x, base_y, base_z);
   const int L = nlev > 0 ? std::min(nlev, full) : full;
   if (L > kMaxLevels) { set_error("pyramid: too many levels"); return -4; }
   PyrGeom pg{};
   pg.levels = L;
+  pg.fmt = fmt;
   size_t total = 0;
   for (int i = 0; i < L; ++i) {
     pg.dims[i][0] = std::max(1, base_x >> i);
@@ -182,7 +184,7 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   CVR_HIP_CHECK(hipFree(d_op));
   c->pyr = pg;
-  if (build_pyramid_quad(c, total, st) != CVR_OK) {
+  if (build_pyramid_quad(c, total, fmt, st) != CVR_OK) {
     c->pyr_built_for = ~0ull;
     return -5;
   }

@@ -2,8 +2,10 @@
 // holds the values at (x, y), (x+1, y), (x, y+1), (x+1, y+1) of plane z, with the +1
 // neighbours clamped to the edge, so a GL_LINEAR trilinear read is two vector loads
 // (planes z0 and z1) instead of eight scalar gathers.  Used for the raw volume
-// (uchar4 / ushort4 per voxel) and for every level of the extinction pyramid (float4).
+// (uchar4 / ushort4 per voxel) and for every level of the extinction pyramid (float4,
+// or four fp16 / bf16 values in 8 bytes).
 #include "cvr_ctx.hpp"
+#include <hip/hip_fp16.h>
 
 namespace cvr {
 
@@ -54,21 +56,56 @@ int build_volume_quad(cvr_ctx* c, hipStream_t st) {
   return CVR_OK;
 }
 
-int build_pyramid_quad(cvr_ctx* c, size_t total, hipStream_t st) {
+// fp32 -> 16-bit storage, round to nearest even: fp16 (fmt 1) or bf16 (fmt 2).
+__device__ __forceinline__ unsigned int to16(float x, int fmt) {
+  if (fmt == 1) return __half_as_ushort(__float2half_rn(x));
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// Narrows fp32 neighbour blocks to 4 x 16 bit.
+__global__ __launch_bounds__(256) void k_pack_quad16(const float4* __restrict__ src, uint2* __restrict__ dst, size_t n,
+                                                     int fmt) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = src[i];
+  dst[i] = make_uint2(to16(q.x, fmt) | (to16(q.y, fmt) << 16), to16(q.z, fmt) | (to16(q.w, fmt) << 16));
+}
+
+int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st) {
   if (c->d_pyr_quad) CVR_HIP_CHECK(hipFree(c->d_pyr_quad));
   c->d_pyr_quad = nullptr;
-  if (hipMalloc(&c->d_pyr_quad, total * sizeof(float4)) != hipSuccess) {
-    c->d_pyr_quad = nullptr;
+  float4* q32 = nullptr;
+  if (hipMalloc(&q32, total * sizeof(float4)) != hipSuccess) {
     set_error("pyramid: out of device memory (quad layout)");
     return CVR_ERR_NOMEM;
   }
   const PyrGeom& pg = c->pyr;
   for (int l = 0; l < pg.levels; ++l) {
-    const int rc = make_quad(c->d_pyr + pg.off[l], c->d_pyr_quad + pg.off[l], pg.dims[l][0], pg.dims[l][1],
-                             pg.dims[l][2], st);
-    if (rc != CVR_OK) return rc;
+    const int rc = make_quad(c->d_pyr + pg.off[l], q32 + pg.off[l], pg.dims[l][0], pg.dims[l][1], pg.dims[l][2], st);
+    if (rc != CVR_OK) {
+      (void)hipFree(q32);
+      return rc;
+    }
   }
+  if (fmt == 0) {
+    CVR_HIP_CHECK(hipStreamSynchronize(st));
+    c->d_pyr_quad = q32;
+    return CVR_OK;
+  }
+  void* q16 = nullptr;
+  if (hipMalloc(&q16, total * sizeof(uint2)) != hipSuccess) {
+    (void)hipFree(q32);
+    set_error("pyramid: out of device memory (quad layout)");
+    return CVR_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(k_pack_quad16, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, q32,
+                     static_cast<uint2*>(q16), total, fmt);
+  CVR_HIP_CHECK(hipGetLastError());
   CVR_HIP_CHECK(hipStreamSynchronize(st));
+  CVR_HIP_CHECK(hipFree(q32));
+  c->d_pyr_quad = q16;
   return CVR_OK;
 }
 

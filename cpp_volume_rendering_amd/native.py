@@ -65,7 +65,7 @@ class DosParams(C.Structure):
     _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_occlusion", C.c_int),
                 ("apply_shadow", C.c_int), ("shadow_type", C.c_int), ("spot_max_deg", C.c_float),
                 ("n_levels", C.c_int), ("base_res", C.c_int * 3), ("sigma0", C.c_float), ("occ", DosCone),
-                ("sdw", DosCone), ("ka", C.c_float), ("kd", C.c_float)]
+                ("sdw", DosCone), ("ka", C.c_float), ("kd", C.c_float), ("density_precision", C.c_int)]
 
 
 class ConeParams(C.Structure):
@@ -74,19 +74,28 @@ class ConeParams(C.Structure):
                 ("n_steps", C.c_int), ("n_occ_cones", C.c_int), ("occ_half_deg", C.c_float),
                 ("ring_deg", C.c_float), ("sdw_half_deg", C.c_float), ("t0", C.c_float),
                 ("occ_max_dist", C.c_float), ("sdw_max_dist", C.c_float), ("r_sigma", C.c_float),
-                ("ka", C.c_float), ("kd", C.c_float)]
+                ("ka", C.c_float), ("kd", C.c_float), ("density_precision", C.c_int)]
 
 
 class NnParams(C.Structure):
     _fields_ = [("cone", ConeParams), ("n_train", C.c_int), ("n_val", C.c_int), ("epochs", C.c_int),
                 ("batch", C.c_int), ("lr", C.c_float), ("rmse_gate", C.c_float), ("ood_margin", C.c_float),
-                ("force_exact", C.c_int), ("seed", C.c_uint64)]
+                ("force_exact", C.c_int), ("seed", C.c_uint64), ("tile_w", C.c_int), ("shade_batch", C.c_float)]
 
 
 class NnReport(C.Structure):
     _fields_ = [("train_loss", C.c_double), ("val_rmse_occ", C.c_double), ("val_rmse_sdw", C.c_double),
                 ("base_rmse_occ", C.c_double), ("base_rmse_sdw", C.c_double), ("steps", C.c_int),
                 ("exact_fallback", C.c_int)]
+
+
+class Job(C.Structure):
+    _fields_ = [("volume", C.c_int), ("camera", Camera), ("light", Light)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_w", C.c_int), ("shade_batch", C.c_float), ("queues", C.c_int),
+                ("workgroups_per_cu", C.c_int), ("ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -105,6 +114,9 @@ EXPORTS = [
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
     "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load", "cvr_dos_sections", "cvr_gt_rays",
+    "cvr_batch_create", "cvr_batch_destroy", "cvr_batch_add_volume", "cvr_batch_set_jobs",
+    "cvr_batch_set_tuning", "cvr_batch_get_tuning", "cvr_batch_autotune", "cvr_batch_render",
+    "cvr_batch_get_stats",
     "cvr_synth_blobs", "cvr_synth_noise",
 ]
 
@@ -159,6 +171,15 @@ def lib() -> C.CDLL:
         "cvr_nn_load": (i, [vp, vp, sz]),
         "cvr_dos_sections": (i, [vp, i, vp, i, vp, vp, vp, vp]),
         "cvr_gt_rays": (i, [vp, i, vp, i, vp]),
+        "cvr_batch_create": (i, [i, i, i, C.POINTER(vp)]),
+        "cvr_batch_destroy": (None, [vp]),
+        "cvr_batch_add_volume": (i, [vp, vp, C.POINTER(i)]),
+        "cvr_batch_set_jobs": (i, [vp, i, vp]),
+        "cvr_batch_set_tuning": (i, [vp, C.POINTER(BatchTuning)]),
+        "cvr_batch_get_tuning": (i, [vp, C.POINTER(BatchTuning)]),
+        "cvr_batch_autotune": (i, [vp, vp, i, vp, C.POINTER(BatchTuning)]),
+        "cvr_batch_render": (i, [vp, vp, vp]),
+        "cvr_batch_get_stats": (i, [vp, C.POINTER(Stats)]),
         "cvr_synth_blobs": (i, [i, i, i, i, i, d, i, C.c_uint64, vp]),
         "cvr_synth_noise": (i, [i, i, i, i, i, C.c_uint64, i, vp, vp]),
     }
@@ -366,4 +387,67 @@ class Renderer:
     def stats(self) -> Stats:
         s = Stats()
         check(lib().cvr_get_stats(self.h, C.byref(s)))
+        return s
+
+
+class Batch:
+    """Batched surrogate rendering (cvr_batch_*): CVR_CONE_NN renderers x jobs of one size."""
+
+    def __init__(self, width: int, height: int, device: int = 0):
+        h = C.c_void_p()
+        check(lib().cvr_batch_create(device, width, height, C.byref(h)))
+        self.h, self.width, self.height = h, width, height
+        self._refs = []  # the renderers must outlive the batch
+        self.n_jobs = 0
+
+    def close(self):
+        if self.h:
+            lib().cvr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_volume(self, r: "Renderer") -> int:
+        idx = C.c_int()
+        check(lib().cvr_batch_add_volume(self.h, r.h, C.byref(idx)))
+        self._refs.append(r)
+        return idx.value
+
+    def set_jobs(self, jobs):
+        """jobs: list of (volume index, Camera, Light)."""
+        arr = (Job * len(jobs))()
+        for i, (v, cam, light) in enumerate(jobs):
+            arr[i].volume, arr[i].camera, arr[i].light = v, cam, light
+        check(lib().cvr_batch_set_jobs(self.h, len(jobs), arr))
+        self.n_jobs = len(jobs)
+
+    def tuning(self) -> BatchTuning:
+        t = BatchTuning()
+        check(lib().cvr_batch_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().cvr_batch_set_tuning(self.h, C.byref(t)))
+        return t
+
+    def autotune(self, out, stride=1, stream=None) -> BatchTuning:
+        t = BatchTuning()
+        check(lib().cvr_batch_autotune(self.h, _ptr(out), stride, stream, C.byref(t)))
+        return t
+
+    def render(self, out, stream=None):
+        """out: device tensor (n_jobs, height, width, 4) float32."""
+        check(lib().cvr_batch_render(self.h, _ptr(out), stream))
+        return out
+
+    def stats(self) -> Stats:
+        s = Stats()
+        check(lib().cvr_batch_get_stats(self.h, C.byref(s)))
         return s

@@ -144,6 +144,7 @@ typedef struct cvr_dos_params {
   cvr_dos_cone occ;      /* 20 deg, packing 1 (3 cones), weight 0.35, covered 0.5 * diagonal */
   cvr_dos_cone sdw;      /* 0.5 deg, packing 0, weight 1.0, covered 0.75 * diagonal */
   float ka, kd;          /* 0.5, 0.5 */
+  int density_precision; /* pyramid storage read by the cones: 0 fp32, 1 fp16, 2 bf16 (0) */
 } cvr_dos_params;
 
 /* Cone-traced lighting over the Gaussian extinction pyramid.  The pyramid follows
@@ -169,6 +170,7 @@ typedef struct cvr_cone_params {
   float sdw_max_dist;   /* <= 0: 0.75 * diagonal */
   float r_sigma;        /* cone radius / Gaussian sigma at the level read (2.0) */
   float ka, kd;         /* 0.5, 0.5 */
+  int density_precision; /* pyramid storage read by the cones: 0 fp32, 1 fp16, 2 bf16 (0) */
 } cvr_cone_params;
 
 /* Surrogate: the cone parameters it replaces (and falls back to) plus training. */
@@ -183,6 +185,12 @@ typedef struct cvr_nn_params {
   float ood_margin;     /* inputs this far outside the training range use the exact tracer (0.02) */
   int force_exact;      /* nonzero: every sample uses the exact tracer */
   uint64_t seed;        /* training-set and initialisation seed */
+  /* Work shape of the renderer (pixels do not depend on it): wavefront tile width in
+   * pixels (4, 8, 16 or 32; the height is 64 / width) and the fraction of a wave's
+   * marching lanes that must wait on a shaded sample before the wave shades them
+   * together (0: as soon as one waits). */
+  int tile_w;           /* 8 */
+  float shade_batch;    /* 0.5 */
 } cvr_nn_params;
 
 typedef struct cvr_nn_report {
@@ -295,6 +303,50 @@ int cvr_nn_get_report(cvr_handle h, cvr_nn_report* out);
 size_t cvr_nn_checkpoint_size(void);
 int cvr_nn_save(cvr_handle h, void* out, size_t bytes);
 int cvr_nn_load(cvr_handle h, const void* in, size_t bytes);
+
+/* ---------------------------------------------------------------------------
+ * Batched rendering.  No reference counterpart: the reference renders one volume
+ * from one view per dispatch (rc1prenderer.cpp:140-151, renderingmanager.cpp:174-319).
+ * A batch references prepared-on-demand CVR_CONE_NN handles (each owns its volume,
+ * transfer function, pyramid and trained surrogate; the batch does not own them and
+ * they must outlive it) and a list of jobs (volume, camera, light) at one image size.
+ * cvr_batch_render renders every job in one persistent launch: workgroups pull
+ * (job, tile) items from device work queues (one per XCD, with stealing, or one
+ * global queue); the queue reset and the launch are captured once into a HIP graph
+ * and replayed while jobs, tuning and output stay the same.  Images land in `out`
+ * (device memory, n_jobs * height * width * 4 floats, job-major, each image as
+ * cvr_render writes it) and equal cvr_render of the same handle, camera and light
+ * bit for bit.
+ */
+typedef struct cvr_batch* cvr_batch_handle;
+
+typedef struct cvr_job {
+  int volume;        /* index returned by cvr_batch_add_volume */
+  cvr_camera camera;
+  cvr_light light;
+} cvr_job;
+
+/* Work shape of the persistent launch (pixels do not depend on it). */
+typedef struct cvr_batch_tuning {
+  int tile_w;            /* wavefront tile width: 4, 8, 16 or 32 pixels (height 64 / width) */
+  float shade_batch;     /* fraction of marching lanes that wait before a shading pass, [0, 1] */
+  int queues;            /* 1: one global queue; 8: one queue per XCD with stealing */
+  int workgroups_per_cu; /* persistent workgroups per compute unit, 1..4 */
+  double ms;             /* autotune: the measured time of the chosen shape */
+} cvr_batch_tuning;
+
+int cvr_batch_create(int device, int width, int height, cvr_batch_handle* out);
+void cvr_batch_destroy(cvr_batch_handle b);
+int cvr_batch_add_volume(cvr_batch_handle b, cvr_handle h, int* index);
+int cvr_batch_set_jobs(cvr_batch_handle b, int n, const cvr_job* jobs);
+int cvr_batch_set_tuning(cvr_batch_handle b, const cvr_batch_tuning* t);
+int cvr_batch_get_tuning(cvr_batch_handle b, cvr_batch_tuning* t);
+/* Times candidate shapes on every `stride`-th job (stride >= 1) rendered into `out`,
+ * keeps the fastest and reports it. */
+int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_stream, cvr_batch_tuning* best);
+int cvr_batch_render(cvr_batch_handle b, float* out, void* hip_stream);
+/* Counters of the last batch render (waits for the device). */
+int cvr_batch_get_stats(cvr_batch_handle b, cvr_stats* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,111 @@
+"""Batched surrogate rendering (cvr_batch_*): one persistent graph-captured launch over
+uneven volumes, views and lights gives, image by image, exactly what the single-frame
+renderer gives for the same handle, camera and light — on every graph replay, for
+every work shape, and after autotuning."""
+import numpy as np
+import pytest
+import torch
+
+from cpp_volume_rendering_amd import native as N
+from cpp_volume_rendering_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+WD, HT = 64, 48
+
+
+def surrogate(vol, rgb, a, seed):
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(vol)
+    r.set_transfer_function(rgb, a)
+    p = r.params()
+    p.n_train, p.n_val, p.epochs, p.batch, p.seed = 1 << 16, 1 << 12, 2, 4096, seed
+    p.rmse_gate = 1.0  # the network renders (the gate is tested elsewhere)
+    r.set_params(p)
+    r.prepare()
+    return r
+
+
+@pytest.fixture(scope="module")
+def scene():
+    vols = [
+        N.synth_blobs((40, 40, 40), N.U8, n_blobs=12, seed=31),
+        N.synth_noise((48, 32, 24), N.U16, octaves=3, seed=32, bands=[0.3, 0.6, 0.85]),
+        N.synth_blobs((32, 36, 28), N.U8, n_blobs=8, seed=33, noise_weight=0.25),
+    ]
+    rs = []
+    for i, v in enumerate(vols):
+        rgb, a = W.seeded_tf(40 + i, 4 + 2 * i)
+        rs.append(surrogate(v, rgb, a, 50 + i))
+    jobs = []
+    for i, v in enumerate(vols):
+        nz, ny, nx = v.shape
+        grid = (float(nx), float(ny), float(nz))
+        for az in (10.0, 130.0, 250.0):
+            for ld in ((-0.45, 0.75, 0.5), (0.6, 0.3, -0.7)):
+                jobs.append((i, W.orbit_camera(grid, WD, HT, az, 25.0), W.light_from_direction(ld, grid)))
+    ref = np.stack([rs[v].render(cam, WD, HT, light=l) for v, cam, l in jobs])
+    return rs, jobs, ref
+
+
+def make_batch(rs, jobs):
+    b = N.Batch(WD, HT)
+    for r in rs:
+        b.add_volume(r)
+    b.set_jobs(jobs)
+    return b
+
+
+def test_graph_replays_equal_single_frames(scene):
+    rs, jobs, ref = scene
+    assert ref[..., 3].max() > 0.2
+    b = make_batch(rs, jobs)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    for rep in range(3):
+        out.zero_()
+        b.render(out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert np.array_equal(got, ref), (rep, float(np.abs(got - ref).max()))
+        # eager work between replays: a single-frame render and a host copy
+        rs[rep % len(rs)].render(jobs[0][1], WD, HT, light=jobs[0][2])
+    st = b.stats()
+    assert st.shaded_samples > 0 and st.fallback_samples <= st.shaded_samples
+
+
+@pytest.mark.parametrize("tile_w,shade,queues,wg", [(16, 0.0, 1, 1), (4, 1.0, 8, 3), (32, 0.75, 8, 2)])
+def test_work_shapes_do_not_change_pixels(scene, tile_w, shade, queues, wg):
+    rs, jobs, ref = scene
+    b = make_batch(rs, jobs)
+    b.set_tuning(tile_w=tile_w, shade_batch=shade, queues=queues, workgroups_per_cu=wg)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    b.render(out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_autotune_then_render(scene):
+    rs, jobs, ref = scene
+    b = make_batch(rs, jobs)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    t = b.autotune(out, stride=2)
+    assert t.ms > 0 and t.tile_w in (4, 8, 16, 32)
+    out.zero_()
+    b.render(out)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_batch_errors(scene):
+    rs, jobs, _ = scene
+    b = N.Batch(WD, HT)
+    plain = N.Renderer(N.RC1P)
+    with pytest.raises(N.CvrError):
+        b.add_volume(plain)
+    b.add_volume(rs[0])
+    with pytest.raises(N.CvrError):
+        b.set_jobs([(1, jobs[0][1], jobs[0][2])])
+    with pytest.raises(N.CvrError):
+        b.set_tuning(tile_w=12)
+    out = torch.zeros((1, HT, WD, 4), device="cuda")
+    with pytest.raises(N.CvrError):
+        b.render(out)  # no jobs yet

@@ -116,3 +116,19 @@ def test_cone_render_full_config_sparse():
     tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
     ref, m = O.render_cone(wl.volume, tf, pyr, O.cone_params(wl.grid, light), wl.camera(), 1024, 1024, stride=32)
     assert_parity(img, ref, m, stride=32)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 2e-3), (2, 1e-2)])
+def test_cone_render_16bit_pyramid(prec, tol):
+    """The pyramid stored as fp16 (1) or bf16 (2) for the cone reads, against the FP64
+    oracle on the exact fp32 pyramid: the images stay within `tol` per channel."""
+    wl = W.cone_blobs(n=256)
+    r, p = make(wl, density_precision=prec)
+    light = wl.light()
+    img = r.render(wl.camera(), 1024, 1024, light=light)
+    pyr = O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in r.pyramid()])
+    tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
+    ref, m = O.render_cone(wl.volume, tf, pyr, O.cone_params(wl.grid, light), wl.camera(), 1024, 1024, stride=32)
+    s = O.compare(img, ref, m, tol, 32)
+    print(prec, s)
+    assert s["max_abs"] <= tol, s
