@@ -8,7 +8,9 @@ frame of the learned-surrogate renderer (CVR_CONE_NN) with volume, pyramid and
 trained network resident in HBM; the exact cone-traced renderer (CVR_CONE_TRACE) is
 timed the same way beside it and judges the surrogate's image RMSE.  Other
 workloads: rc1p (128^3 plain caster at 512^2), ebs (256^3 extinction-based shading
-at 1024^2), cone (256^3 exact cone tracer at 1024^2).  Prints one JSON line.
+at 1024^2), cone (256^3 exact cone tracer at 1024^2), dos (256^3 section-table
+directional occlusion + point-light shadow at 1024^2), batch (4 uneven volumes x 16
+views x 4 lights, 512^2, one graph-captured persistent launch).  Prints one JSON line.
 Only one GPU is used; --gpus above 1 is refused.
 """
 from __future__ import annotations
@@ -91,6 +93,16 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=20.0):
         return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
                 "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), oracle FP32 "
                           f"with fp32 SAT, OpenMP over rows; scaled"}
+    if kind == "dos":
+        pyr = params["pyramid"]
+        prm = O.dos_params(params["native"], wl.grid, light)
+        stride = params.get("stride", 32)
+        t0 = time.perf_counter()
+        O.render_dos(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+        t = time.perf_counter() - t0
+        return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
+                "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), section-table "
+                          f"cone integration, oracle FP32, OpenMP over rows; scaled"}
     # cone tracer: the exact tracer's FP32 port on the GPU-built pyramid (pyramid build not timed)
     pyr = params["pyramid"]
     prm = O.cone_params(wl.grid, light)
@@ -103,12 +115,67 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=20.0):
                       f"tracer, oracle FP32, OpenMP over rows; scaled"}
 
 
+def bench_batch(args, N, W, torch, stream):
+    """Batched render: 4 uneven volumes x 16 views x 4 lights = 256 images of 512^2 with
+    the surrogate, one graph-captured persistent launch per step, shape autotuned first."""
+    t0 = time.perf_counter()
+    wls, jobs = W.batch_scene(scale_down=max(1, args.size))
+    rs = []
+    for wl in wls:
+        r = N.Renderer(N.CONE_NN)
+        r.set_volume(wl.volume, wl.scale)
+        r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        r.prepare(stream.cuda_stream)
+        rs.append(r)
+    stream.synchronize()
+    log(f"batch: {len(wls)} volumes prepared (pyramids, training) in {time.perf_counter() - t0:.1f}s")
+    b = N.Batch(512, 512)
+    for r in rs:
+        b.add_volume(r)
+    b.set_jobs(jobs)
+    out = torch.empty((len(jobs), 512, 512, 4), dtype=torch.float32, device="cuda")
+    t1 = time.perf_counter()
+    tune = b.autotune(out, stride=8, stream=stream.cuda_stream)
+    log(f"autotune {time.perf_counter() - t1:.1f}s: tile_w {tune.tile_w} shade_batch {tune.shade_batch} "
+        f"queues {tune.queues} wg/cu {tune.workgroups_per_cu} ({tune.ms:.2f} ms on every 8th job)")
+    for _ in range(args.warmup):
+        b.render(out, stream.cuda_stream)
+    stream.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        b.render(out, stream.cuda_stream)
+    e1.record(stream)
+    stream.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    st = b.stats()
+    ips = len(jobs) / (ms * 1e-3)
+    flops = st.shaded_samples * MLP_FLOP_PER_SAMPLE
+    achieved = flops / (ms * 1e-3) / 1e12
+    line = {
+        "metric": "images/s, batched surrogate DAO+shadow (4 volumes x 16 views x 4 lights, 512^2)",
+        "value": round(ips, 3), "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic (seeded)",
+        "config": {"workload": "batch_4vol_16view_4light_512", "images_per_step": len(jobs),
+                   "volumes": [f"{wl.name} {wl.volume.shape[::-1]}" for wl in wls],
+                   "tuning": {"tile_w": tune.tile_w, "shade_batch": round(tune.shade_batch, 3),
+                              "queues": tune.queues, "workgroups_per_cu": tune.workgroups_per_cu},
+                   "samples_per_step": int(st.samples), "shaded_samples_per_step": int(st.shaded_samples),
+                   "fallback_samples_per_step": int(st.fallback_samples)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / BF16_PEAK_TFLOPS, 6), "traffic": None},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nn", choices=["nn", "cone", "rc1p", "ebs"])
+    ap.add_argument("--workload", default="nn", choices=["nn", "cone", "rc1p", "ebs", "dos", "batch"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=0, help="override the volume size (tests)")
     args = ap.parse_args()
@@ -124,6 +191,9 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
+    if args.workload == "batch":
+        bench_batch(args, N, W, torch, stream)
+        return
     t_setup = time.perf_counter()
     extra = {}
     if args.workload == "rc1p":
@@ -132,6 +202,9 @@ def main():
     elif args.workload == "ebs":
         wl = W.banded_noise(n=args.size or 256, width=1024, height=1024, seed=2)
         tech, dtype, bpv = N.EBS, "f32", 2
+    elif args.workload == "dos":
+        wl = W.cone_blobs(n=args.size or 256, width=1024, height=1024, seed=3)
+        tech, dtype, bpv = N.DOS, "f32", 1
     elif args.workload == "cone":
         wl = W.cone_blobs(n=args.size or 256, width=1024, height=1024, seed=3)
         tech, dtype, bpv = N.CONE_TRACE, "f32", 1
@@ -145,6 +218,8 @@ def main():
     r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
     if tech == N.EBS:
         r.set_params(shadow_type=1)
+    if tech == N.DOS:
+        r.set_params(apply_shadow=1)
     t_prep = time.perf_counter()
     r.prepare(stream.cuda_stream)
     stream.synchronize()
@@ -203,13 +278,14 @@ def main():
         "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config, "roofline": roof,
     }
     if not args.no_cpu:
-        kind = {N.RC1P: "rc1p", N.EBS: "ebs"}.get(tech, "cone")
+        kind = {N.RC1P: "rc1p", N.EBS: "ebs", N.DOS: "dos"}.get(tech, "cone")
         params = None
-        if kind == "cone":
+        if kind in ("cone", "dos"):
             sys.path.insert(0, str(ROOT / "tests"))
             import oracle_lib as O
-            levels = (r if tech == N.CONE_TRACE else ct).pyramid()
-            params = {"pyramid": O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in levels]), "stride": 64}
+            levels = (r if tech in (N.CONE_TRACE, N.DOS) else ct).pyramid()
+            params = {"pyramid": O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in levels]), "stride": 64,
+                      "native": r.params()}
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(kind, wl, cam, light, params)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {line['cpu_baseline']}")

@@ -3,6 +3,7 @@
 #pragma once
 #include "cvr_ctx.hpp"
 #include <hip/hip_fp16.h>
+#include <type_traits>
 
 namespace cvr {
 
@@ -20,6 +21,21 @@ struct StencilF {
 // ring), kNSdw shadow segments, the local opacity.
 constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7, kNStencil = 2 * kNOcc + kNSdw, kMaxSteps = 256;
 
+// Host dispatch over the compile-time variants of the pyramid-reading kernels: calls
+// fn(Tag<voxel type>{}, integral_constant<pyramid storage>{}).
+template <class T> struct Tag { typedef T type; };
+template <class Fn> void with_fmt(int fmt, Fn&& fn) {
+  if (fmt == 1) fn(std::integral_constant<int, 1>{});
+  else if (fmt == 2) fn(std::integral_constant<int, 2>{});
+  else fn(std::integral_constant<int, 0>{});
+}
+template <class Fn> void with_types(int dtype, int fmt, Fn&& fn) {
+  with_fmt(fmt, [&](auto f) {
+    if (dtype == CVR_U8) fn(Tag<uint8_t>{}, f);
+    else fn(Tag<uint16_t>{}, f);
+  });
+}
+
 // Per-frame cone-tracer constants.
 struct ConeK {
   int n_steps, n_occ, apply_occ, apply_sdw;
@@ -32,10 +48,11 @@ struct ConeK {
 
 // One neighbour block of the pyramid in its storage format: fp32 (float4), or four
 // 16-bit values (fp16 or bf16) in 8 bytes.
-__device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, int fmt, size_t i) {
-  if (fmt == 0) return static_cast<const float4*>(pyr)[i];
+template <int F>
+__device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, size_t i) {
+  if (F == 0) return static_cast<const float4*>(pyr)[i];
   const uint2 r = static_cast<const uint2*>(pyr)[i];
-  if (fmt == 1)
+  if (F == 1)
     return make_float4(__half2float(__ushort_as_half(ushort(r.x & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.x >> 16))),
                        __half2float(__ushort_as_half(ushort(r.y & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.y >> 16))));
   return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xFFFF0000u), __uint_as_float(r.y << 16),
@@ -44,6 +61,7 @@ __device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, int fm
 
 // GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w) through the
 // neighbour blocks (2 vector loads per trilinear read).
+template <int F>
 __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
                                          float w) {
   const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
@@ -57,7 +75,7 @@ __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const Pyr
   const int zi = int(zf);
   const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
   const size_t row = pg.off[l] + size_t(y0 * nx + x0), plane = size_t(nx * ny);
-  return quad_lerp(pyr_block(pyr, pg.fmt, row + z0 * plane), pyr_block(pyr, pg.fmt, row + z1 * plane), fx, fy, fz);
+  return quad_lerp(pyr_block<F>(pyr, row + z0 * plane), pyr_block<F>(pyr, row + z1 * plane), fx, fy, fz);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
@@ -73,6 +91,7 @@ __device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float 
 
 // textureLod with LINEAR_MIPMAP_LINEAR (lerp of levels floor(lod) and floor(lod)+1,
 // the second clamped to the chain), then the border falloff.
+template <int F>
 __device__ __forceinline__ float gauss_ext(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, float lod) {
   const float q = float(pg.levels - 1);
   lod = fminf(fmaxf(lod, 0.f), q);
@@ -80,26 +99,28 @@ __device__ __forceinline__ float gauss_ext(const void* __restrict__ pyr, const P
   const int d2 = min(d1 + 1, pg.levels - 1);
   const float fr = lod - float(d1);
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  const float a = pyr_tri(pyr, pg, d1, u, v, w);
-  const float b = pyr_tri(pyr, pg, d2, u, v, w);
+  const float a = pyr_tri<F>(pyr, pg, d1, u, v, w);
+  const float b = pyr_tri<F>(pyr, pg, d2, u, v, w);
   const float sg = exp2f(lod);
   return (a * (1.f - fr) + b * fr) * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
 // The same lookup at a whole level `l` (one trilinear read).
+template <int F>
 __device__ __forceinline__ float gauss_ext_level(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
   const float sg = float(1 << l);
-  return pyr_tri(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
+  return pyr_tri<F>(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
+template <int F>
 __device__ __forceinline__ float cone_T(const void* __restrict__ pyr, const PyrGeom& pg, const ConeStepF* __restrict__ s,
                                         int n, f3 p, f3 axis) {
   float od = 0.f;
 #pragma unroll 4
   for (int j = 0; j < n; ++j) {
     const ConeStepF c = s[j];
-    od += gauss_ext(pyr, pg, p + axis * c.tm, c.lod) * c.dt;
+    od += gauss_ext<F>(pyr, pg, p + axis * c.tm, c.lod) * c.dt;
   }
   return __expf(-od);
 }
@@ -113,15 +134,16 @@ __device__ __forceinline__ void occ_frame(f3 dir, f3& k, f3& up, f3& right) {
 
 // 1 axial cone + (n_occ-1) cones whose axes sit on a ring at ring angle from the axis,
 // cosine-weighted like gt_ray_marching.comp:161-168.
+template <int F>
 __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, const PyrGeom& pg, const ConeK& K, f3 p,
                                               f3 k, f3 up, f3 right) {
-  float num = cone_T(pyr, pg, K.so, K.n_steps, p, k), den = 1.f;
+  float num = cone_T<F>(pyr, pg, K.so, K.n_steps, p, k), den = 1.f;
   const int nr = K.n_occ - 1;
   for (int j = 0; j < nr; ++j) {
     float sphi, cphi;
     __sincosf(6.283185307179586f * float(j) / float(nr), &sphi, &cphi);
     const f3 ax = k * K.ring_cos + (up * cphi + right * sphi) * K.ring_sin;
-    num += K.ring_cos * cone_T(pyr, pg, K.so, K.n_steps, p, ax);
+    num += K.ring_cos * cone_T<F>(pyr, pg, K.so, K.n_steps, p, ax);
     den += K.ring_cos;
   }
   return num / den;
@@ -130,19 +152,20 @@ __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, cons
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
 // opacities along the light (8..14), the local opacity (15).
+template <int F>
 __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
                                          f3 p, f3 k, f3 L, float* f) {
 #pragma unroll
   for (int i = 0; i < 2 * kNOcc; ++i) {
     const StencilF so = st[i];
-    f[i] = 1.f - __expf(-gauss_ext_level(pyr, pg, p + k * so.dist, int(so.lod)) * so.seg);
+    f[i] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p + k * so.dist, int(so.lod)) * so.seg);
   }
 #pragma unroll
   for (int i = 0; i < kNSdw; ++i) {
     const StencilF ss = st[2 * kNOcc + i];
-    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext_level(pyr, pg, p + L * ss.dist, int(ss.lod)) * ss.seg);
+    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p + L * ss.dist, int(ss.lod)) * ss.seg);
   }
-  f[15] = 1.f - __expf(-gauss_ext_level(pyr, pg, p, 0));
+  f[15] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p, 0));
 }
 
 // Network-free estimate from the same inputs (the network predicts the residual):

@@ -129,7 +129,7 @@ void light_dir(const RayFrame& rf, const cvr_light* light, float L[3]);
 // Renders one workgroup tile (2 x 2 wave tiles of the NnSched shape) of frame `rf`.
 // sW/sB/s_tf: the network and transfer function staged in LDS; sXw: this wave's LDS
 // scratch.  Counters (samples, shaded, exact fallbacks) accumulate into cnt.
-template <typename T>
+template <typename T, int F>
 __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, const PyrGeom& pg,
                                         const void* __restrict__ pyr, const ConeK& K, const NnK& nk,
                                         const NnSched& sc, const short* sW, const float* sB, float* sXw,
@@ -178,10 +178,13 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
     if (np > 0 && (np == na || float(np) >= sc.park * float(na))) {
       float f[kNFeat];
       bool ood = false;
+      float bo = 0.f, bs = 0.f;
       if (parked) {
-        nn_feats(pyr, pg, K.st, p, k, L, f);
+        nn_feats<F>(pyr, pg, K.st, p, k, L, f);
 #pragma unroll
         for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
+        // the network-free estimate now, so the features die inside the network's input staging
+        nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
       } else {
 #pragma unroll
         for (int c = 0; c < kNFeat; ++c) f[c] = 0.f;
@@ -189,15 +192,13 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
       float Io, Is;
       mlp_wave(sW, sB, sXw, f, Io, Is);
       if (parked) {
-        float bo, bs;
-        nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
         Io += bo;
         Is += bs;
         ++cnt[1];
         if (ood || nk.exact_all) {
           ++cnt[2];
-          Io = K.apply_occ ? ct_occlusion(pyr, pg, K, p, k, up, right) : 0.f;
-          Is = K.apply_sdw ? cone_T(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
+          Io = K.apply_occ ? ct_occlusion<F>(pyr, pg, K, p, k, up, right) : 0.f;
+          Is = K.apply_sdw ? cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
         }
         Io = fminf(fmaxf(Io, 0.f), 1.f);
         Is = fminf(fmaxf(Is, 0.f), 1.f);

@@ -46,7 +46,7 @@ __global__ __launch_bounds__(64) void k_batch_reset(unsigned int* __restrict__ h
   if (threadIdx.x < 4) counters[threadIdx.x] = 0ull;
 }
 
-__global__ __launch_bounds__(256) void k_batch_render(const VolDesc* __restrict__ vols, const JobDesc* __restrict__ jobs,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_render(const VolDesc* __restrict__ vols, const JobDesc* __restrict__ jobs,
                                                       const int2* __restrict__ items, const int* __restrict__ qoff,
                                                       int nq, int tiles_x, NnSched sc, size_t img_px,
                                                       float4* __restrict__ out, unsigned int* __restrict__ heads,
@@ -83,12 +83,18 @@ __global__ __launch_bounds__(256) void k_batch_render(const VolDesc* __restrict_
       K.L[1] = J.L[1];
       K.L[2] = J.L[2];
       float4* o = out + size_t(job) * img_px;
-      if (V.g.dtype == CVR_U8)
-        nn_tile<uint8_t>(J.rf, V.g, V.pg, V.pyr, K, V.nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, V.ntf, V.step, V.et,
-                         o, tile % tiles_x, tile / tiles_x, cnt);
-      else
-        nn_tile<uint16_t>(J.rf, V.g, V.pg, V.pyr, K, V.nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, V.ntf, V.step,
-                          V.et, o, tile % tiles_x, tile / tiles_x, cnt);
+#define CVR_BATCH_TILE(TT, FF)                                                                                    \
+  nn_tile<TT, FF>(J.rf, V.g, V.pg, V.pyr, K, V.nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, V.ntf, V.step, V.et, o, \
+                  tile % tiles_x, tile / tiles_x, cnt)
+      switch (V.g.dtype * 3 + V.pg.fmt) {
+        case 0: CVR_BATCH_TILE(uint8_t, 0); break;
+        case 1: CVR_BATCH_TILE(uint8_t, 1); break;
+        case 2: CVR_BATCH_TILE(uint8_t, 2); break;
+        case 3: CVR_BATCH_TILE(uint16_t, 0); break;
+        case 4: CVR_BATCH_TILE(uint16_t, 1); break;
+        default: CVR_BATCH_TILE(uint16_t, 2); break;
+      }
+#undef CVR_BATCH_TILE
     }
   }
   if (counters) {

@@ -19,7 +19,7 @@
 
 namespace cvr {
 
-template <typename T>
+template <typename T, int F>
 __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                      ConeK K, const float4* __restrict__ tf, int ntf, float step,
                                                      float et, float4* __restrict__ out,
@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
       ++nsamp;
       if (src.w > 0.0f) {
         ++nshade;
-        const float Io = K.apply_occ ? ct_occlusion(pyr, pg, K, p, k, up, right) : 0.f;
-        const float Is = K.apply_sdw ? cone_T(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
+        const float Io = K.apply_occ ? ct_occlusion<F>(pyr, pg, K, p, k, up, right) : 0.f;
+        const float Is = K.apply_sdw ? cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
         const float m = inv_k * (Io * ka + Is * kd);
         src.x *= m; src.y *= m; src.z *= m;
         const float a = 1.0f - __expf(-src.w * h);
@@ -79,6 +79,7 @@ __global__ __launch_bounds__(256) void k_cone_render(RayFrame rf, VolGeom g, Pyr
 
 // Exact occlusion / shadow and the surrogate's features at given points (texture
 // space), for given camera-ray directions and light directions.
+template <int F>
 __global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const void* __restrict__ pyr, ConeK K, int n,
                                                      const float* __restrict__ pos, const float* __restrict__ dirs,
                                                      const float* __restrict__ lights, float* __restrict__ occ,
@@ -90,11 +91,11 @@ __global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const void* __r
   const f3 L = norm3(ld3(lights + 3 * i));
   f3 k, up, right;
   occ_frame(d, k, up, right);
-  if (occ) occ[i] = ct_occlusion(pyr, pg, K, p, k, up, right);
-  if (sdw) sdw[i] = cone_T(pyr, pg, K.ss, K.n_steps, p, L);
+  if (occ) occ[i] = ct_occlusion<F>(pyr, pg, K, p, k, up, right);
+  if (sdw) sdw[i] = cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L);
   if (feat) {
     float f[kNFeat];
-    nn_feats(pyr, pg, K.st, p, k, L, f);
+    nn_feats<F>(pyr, pg, K.st, p, k, L, f);
     for (int c = 0; c < kNFeat; ++c) feat[size_t(i) * kNFeat + c] = f[c];
   }
 }
@@ -224,12 +225,12 @@ int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* o
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 64 * 1024) { set_error("cone renderer: transfer function larger than 4096 entries"); return -4; }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_cone_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
-  else
-    hipLaunchKernelGGL(k_cone_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
+    using T = typename decltype(t)::type;
+    hipLaunchKernelGGL((k_cone_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
+                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, o4, cnt);
+  });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -240,8 +241,10 @@ int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos,
   if (rc) return rc;
   float L[3] = {0, 0, 1};
   ConeK K = cone_constants(c, P, L);
-  hipLaunchKernelGGL(k_cone_points, dim3((n + 255) / 256), dim3(256), 0, st, c->pyr, c->d_pyr_quad, K, n, d_pos, d_dirs,
-                     d_lights, d_occ, d_sdw, d_feat);
+  with_fmt(c->pyr.fmt, [&](auto f) {
+    hipLaunchKernelGGL((k_cone_points<decltype(f)::value>), dim3((n + 255) / 256), dim3(256), 0, st, c->pyr,
+                       c->d_pyr_quad, K, n, d_pos, d_dirs, d_lights, d_occ, d_sdw, d_feat);
+  });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }

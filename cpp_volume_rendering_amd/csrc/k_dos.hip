@@ -43,18 +43,20 @@ struct DosK {
 
 // GetGaussianExtinction (ray_bbox_marching.comp:92-112) at a section's level: textureLod
 // clamps the level to the chain, the border falloff uses the level as given.
+template <int F>
 __device__ __forceinline__ float dos_ext(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, float level) {
   const float lc = fminf(fmaxf(level, 0.f), float(pg.levels - 1));
   const int d1 = int(floorf(lc));
   const float fr = lc - float(d1);
   const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  float a = pyr_tri(pyr, pg, d1, u, v, w);
-  if (fr > 0.f) a = a * (1.f - fr) + pyr_tri(pyr, pg, min(d1 + 1, pg.levels - 1), u, v, w) * fr;
+  float a = pyr_tri<F>(pyr, pg, d1, u, v, w);
+  if (fr > 0.f) a = a * (1.f - fr) + pyr_tri<F>(pyr, pg, min(d1 + 1, pg.levels - 1), u, v, w) * fr;
   const float sg = exp2f(level);
   return a * border_falloff(pg, tp, 0.5f / (sg * sg));
 }
 
 // One cone through its section table; sub-cone axes are k*z + uy*y + vx*x.
+template <int F>
 __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const PyrGeom& pg, const DosConeK& C,
                                           const float4* __restrict__ rows, f3 p, f3 k, f3 uy, f3 vx) {
   float od[7], lt[7];
@@ -64,7 +66,7 @@ __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const Py
   int i = 0;
   for (; i < C.n1; ++i) {
     const float4 s = rows[i];
-    const float at = dos_ext(pyr, pg, p + k * track, s.y) * s.w;
+    const float at = dos_ext<F>(pyr, pg, p + k * track, s.y) * s.w;
     od[0] += (lt[0] + at) * s.z * C.w;
     lt[0] = at;
     track += s.x;
@@ -79,7 +81,7 @@ __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const Py
     const float4 s = rows[i];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const float at = dos_ext(pyr, pg, p + d3[j] * track, s.y) * s.w;
+      const float at = dos_ext<F>(pyr, pg, p + d3[j] * track, s.y) * s.w;
       od[j] += (lt[j] + at) * s.z * C.w;
       lt[j] = at;
     }
@@ -104,7 +106,7 @@ __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const Py
     const float4 s = rows[i];
 #pragma unroll
     for (int j = 0; j < 7; ++j) {
-      const float at = dos_ext(pyr, pg, p + d7[j] * track, s.y) * s.w;
+      const float at = dos_ext<F>(pyr, pg, p + d7[j] * track, s.y) * s.w;
       od[j] += (lt[j] + at) * s.z * C.w;
       lt[j] = at;
     }
@@ -115,17 +117,18 @@ __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const Py
 }
 
 // Shadow cone of ShadowEvaluationKernel (:533-562).  `wp` is the world position.
+template <int F>
 __device__ __forceinline__ float dos_shadow(const void* __restrict__ pyr, const PyrGeom& pg, const DosK& K, f3 p,
                                             f3 wp) {
-  if (K.shadow_type == 2) return dos_cone(pyr, pg, K.sdw, K.sdw_rows, p, ld3(K.lfwd), ld3(K.lup), ld3(K.lright));
+  if (K.shadow_type == 2) return dos_cone<F>(pyr, pg, K.sdw, K.sdw_rows, p, ld3(K.lfwd), ld3(K.lup), ld3(K.lright));
   const f3 k = norm3(ld3(K.lpos) - wp);
   if (K.shadow_type == 1 && dot3(k, ld3(K.lfwd)) < K.spot_cos) return 0.f;
   const f3 u1 = norm3(cross3(k, ld3(K.lright)));
   const f3 v1 = norm3(cross3(k, u1));
-  return dos_cone(pyr, pg, K.sdw, K.sdw_rows, p, k, v1, u1);
+  return dos_cone<F>(pyr, pg, K.sdw, K.sdw_rows, p, k, v1, u1);
 }
 
-template <typename T>
+template <typename T, int F>
 __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                     DosK K, const float4* __restrict__ tf, int ntf, float step, float et,
                                                     float4* __restrict__ out, unsigned long long* __restrict__ counters) {
@@ -163,8 +166,8 @@ __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrG
         ++nshade;
         const f3 wp = p - half;
         float Io = 0.f, Is = 0.f;
-        if (K.apply_occ) Io = dos_cone(pyr, pg, K.occ, K.occ_rows, p, norm3(eye - wp), up, right);
-        if (K.apply_sdw) Is = dos_shadow(pyr, pg, K, p, wp);
+        if (K.apply_occ) Io = dos_cone<F>(pyr, pg, K.occ, K.occ_rows, p, norm3(eye - wp), up, right);
+        if (K.apply_sdw) Is = dos_shadow<F>(pyr, pg, K, p, wp);
         const float m = inv_k * (Io * ka + Is * kd);
         src.x *= m; src.y *= m; src.z *= m;
         const float a = 1.0f - __expf(-src.w * h);
@@ -257,9 +260,24 @@ static cvr_dos_cone clamp_cone(cvr_dos_cone c, double covered_default) {
   return c;
 }
 
+// The schedule ends only if the Gaussians can outgrow the widest sub-cone allowed: the
+// next section sits about d_sigma * sigma further, where that sub-cone's radius grows by
+// d_sigma * sigma * tan(half / div); with d_sigma * tan(half / div) >= r_sigma, sigma
+// would double without bound.
+static bool dos_cone_converges(const cvr_dos_cone& C) {
+  static const double div[3] = {1.0, kDiv3, kDiv7};
+  const double t = std::tan((double(C.half_deg) / div[C.max_packing]) * kPi / 180.0);
+  return double(C.d_sigma) * t < double(C.r_sigma);
+}
+
 int dos_build_table(const cvr_dos_cone& C, double sigma0, DosTable& T) {
   if (!(C.initial_step > 0) || !(C.initial_step < C.covered_dist)) {
     set_error("dos params: initial_step must be > 0 and below the covered distance");
+    return CVR_ERR_INVALID;
+  }
+  if (!dos_cone_converges(C)) {
+    set_error("dos params: d_sigma * tan(half angle / packing ratio) must stay below r_sigma "
+              "(the section schedule would never close)");
     return CVR_ERR_INVALID;
   }
   Builder b{double(C.half_deg), double(C.r_sigma), C.max_packing};
@@ -310,7 +328,14 @@ int dos_build_table(const cvr_dos_cone& C, double sigma0, DosTable& T) {
   return CVR_OK;
 }
 
+static cvr_dos_cone clamp_cone(cvr_dos_cone c, double covered_default);
+
 int check_dos_params(const cvr_dos_params& P) {
+  if (!dos_cone_converges(clamp_cone(P.occ, 100.0)) || !dos_cone_converges(clamp_cone(P.sdw, 100.0))) {
+    set_error("dos params: d_sigma * tan(half angle / packing ratio) must stay below r_sigma "
+              "(the section schedule would never close)");
+    return CVR_ERR_INVALID;
+  }
   if (P.apply_occlusion == 0 && P.apply_shadow == 0) {
     set_error("dos params: occlusion or shadow must be enabled (the shading divides by ka+kd)");
     return CVR_ERR_INVALID;
@@ -395,12 +420,12 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 64 * 1024) { set_error("dos renderer: transfer function larger than 4096 entries"); return CVR_ERR_UNSUPPORTED; }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_dos_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
-  else
-    hipLaunchKernelGGL(k_dos_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
+    using T = typename decltype(t)::type;
+    hipLaunchKernelGGL((k_dos_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
+                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, o4, cnt);
+  });
   CVR_HIP_CHECK(hipGetLastError());
   return CVR_OK;
 }

@@ -244,7 +244,7 @@ __device__ __forceinline__ f3 unit_sphere(uint64_t& s) {
 // gives it nonzero extinction (the samples the renderer shades), a view direction and
 // a light direction uniform on the sphere; outputs the 16 features and the exact
 // (occlusion, shadow) of the cone tracer minus the network-free estimate.
-template <typename T>
+template <typename T, int F>
 __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const void* __restrict__ pyr, ConeK K,
                                                     const float4* __restrict__ tf, int ntf, uint64_t seed, int n,
                                                     float* __restrict__ X, float* __restrict__ Tg,
@@ -263,13 +263,13 @@ __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const
   f3 k, up, right;
   occ_frame(dir, k, up, right);
   float f[kNFeat];
-  nn_feats(pyr, pg, K.st, p, k, L, f);
+  nn_feats<F>(pyr, pg, K.st, p, k, L, f);
   for (int c = 0; c < kNFeat; ++c) X[size_t(i) * kNFeat + c] = f[c];
   float bo, bs;
   nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
   // targets are the residuals of the exact tracer to the network-free estimate
-  Tg[size_t(i) * 2] = ct_occlusion(pyr, pg, K, p, k, up, right) - bo;
-  Tg[size_t(i) * 2 + 1] = cone_T(pyr, pg, K.ss, K.n_steps, p, L) - bs;
+  Tg[size_t(i) * 2] = ct_occlusion<F>(pyr, pg, K, p, k, up, right) - bo;
+  Tg[size_t(i) * 2 + 1] = cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L) - bs;
   if (pos_out) {
     pos_out[size_t(i) * 6 + 0] = p.x; pos_out[size_t(i) * 6 + 1] = p.y; pos_out[size_t(i) * 6 + 2] = p.z;
     pos_out[size_t(i) * 6 + 3] = dir.x; pos_out[size_t(i) * 6 + 4] = dir.y; pos_out[size_t(i) * 6 + 5] = dir.z;
@@ -297,7 +297,7 @@ __global__ void k_feat_range(const float* __restrict__ X, int n, unsigned int* _
 }
 
 // ---------------------------------------------------------------- surrogate renderer
-template <typename T>
+template <typename T, int F>
 __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                    ConeK K, NnK nk, NnSched sc, const short* __restrict__ wpack,
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
   const int tw = 1 << sc.tw_log2;
   const int tiles_x = (rf.W + 2 * tw - 1) / (2 * tw);
   unsigned int cnt[3] = {0, 0, 0};
-  nn_tile<T>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step, et, out, bid % tiles_x,
+  nn_tile<T, F>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step, et, out, bid % tiles_x,
              bid / tiles_x, cnt);
   if (counters) {
     atomicAdd(&counters[0], (unsigned long long)cnt[0]);
@@ -426,12 +426,11 @@ int nn_generate(cvr_ctx* c, const cvr_cone_params& P, uint64_t seed, int n, floa
   float L[3] = {0, 0, 1};
   ConeK K = cone_constants(c, P, L);
   VolGeom g = geom_of(c);
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_nn_samples<uint8_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
-  else
-    hipLaunchKernelGGL(k_nn_samples<uint16_t>, dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr, c->d_pyr_quad, K,
-                       c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
+  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
+    using T = typename decltype(t)::type;
+    hipLaunchKernelGGL((k_nn_samples<T, decltype(f)::value>), dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr,
+                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
+  });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -605,12 +604,12 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   short* wp = static_cast<short*>(c->d_nn_pack);
   float* bp = reinterpret_cast<float*>(wp + kPackShorts);
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_nn_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, sc, wp,
-                       bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
-  else
-    hipLaunchKernelGGL(k_nn_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_quad, K, nk, sc,
-                       wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, reinterpret_cast<float4*>(out), cnt);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
+    using T = typename decltype(t)::type;
+    hipLaunchKernelGGL((k_nn_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
+                       c->d_pyr_quad, K, nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, cnt);
+  });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -114,3 +114,35 @@ def cone_blobs(n=256, width=1024, height=1024, seed=3, n_blobs=64, noise_weight=
 def surrogate_blobs_noise(n=512, width=1024, height=1024, seed=4) -> Workload:
     """Learned surrogate: n^3 uint8 of 64 Gaussian blobs plus 4-octave value noise."""
     return cone_blobs(n, width, height, seed=seed, n_blobs=64, noise_weight=0.25)
+
+
+BATCH_LIGHTS = ((-0.45, 0.75, 0.5), (0.6, 0.3, -0.7), (0.1, -0.8, 0.4), (-0.7, 0.2, -0.6))
+
+
+def batch_scene(width=512, height=512, n_views=16, lights=BATCH_LIGHTS, scale_down=1):
+    """Batched render: 4 volumes of uneven sizes and types, each with its own seeded TF
+    (4-8 control points) — 128^3 u8 blobs, 192x256x160 u16 banded noise, 384^3 u8 blobs
+    plus noise, 512^3 u16 blobs plus noise (seeds 5-8) — and per volume `n_views` orbit
+    views x the light directions.  Returns (workloads, jobs) with jobs as
+    (volume index, camera, light).  `scale_down` divides every volume size (tests)."""
+    s = max(1, int(scale_down))
+    specs = [
+        ("blobs128_u8", lambda: N.synth_blobs((128 // s,) * 3, N.U8, n_blobs=8, seed=5), 4),
+        ("noise192x256x160_u16",
+         lambda: N.synth_noise((192 // s, 256 // s, 160 // s), N.U16, octaves=4, seed=6, bands=[0.25, 0.55, 0.8]), 6),
+        ("blobs_noise384_u8", lambda: N.synth_blobs((384 // s,) * 3, N.U8, n_blobs=64, seed=7, noise_weight=0.25), 8),
+        ("blobs_noise512_u16", lambda: N.synth_blobs((512 // s,) * 3, N.U16, n_blobs=64, seed=8, noise_weight=0.25), 5),
+    ]
+    wls, jobs = [], []
+    for i, (name, make, npts) in enumerate(specs):
+        vol = make()
+        rgb, a = seeded_tf(105 + i, npts)
+        wl = Workload(name, vol, (1.0, 1.0, 1.0), rgb, a, width, height)
+        wls.append(wl)
+        for v in range(n_views):
+            az = 360.0 * v / n_views
+            el = 20.0 if v % 2 == 0 else -15.0
+            cam = orbit_camera(wl.grid, width, height, az, el)
+            for ld in lights:
+                jobs.append((i, cam, light_from_direction(ld, wl.grid)))
+    return wls, jobs

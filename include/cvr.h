@@ -315,8 +315,9 @@ int cvr_nn_load(cvr_handle h, const void* in, size_t bytes);
  * global queue); the queue reset and the launch are captured once into a HIP graph
  * and replayed while jobs, tuning and output stay the same.  Images land in `out`
  * (device memory, n_jobs * height * width * 4 floats, job-major, each image as
- * cvr_render writes it) and equal cvr_render of the same handle, camera and light
- * bit for bit.
+ * cvr_render writes it).  Each image equals cvr_render of the same handle, camera and
+ * light up to fp32 rounding (the same device code compiled into another kernel), and
+ * does not depend on the work shape.
  */
 typedef struct cvr_batch* cvr_batch_handle;
 

@@ -104,7 +104,10 @@ inline bool dos_build(const DosConeSpec& c, double sigma0, DosTableO& T) {
   while (!b.step1(pos, sg)) sg *= 2.0;
   while (pos < c.covered) {
     double si = c.d_sigma * sg;
-    while (!b.step1(pos + si + c.d_sigma * sg, sg)) sg *= 2.0;
+    while (!b.step1(pos + si + c.d_sigma * sg, sg)) {
+      sg *= 2.0;
+      if (!(sg < 1e30)) return false;  // the schedule diverges (sigma outgrows every bound)
+    }
     si += c.d_sigma * sg;
     ipos.push_back(pos);
     idist.push_back(si);

@@ -38,7 +38,8 @@ def assert_parity(img, ref, m, stride=1, max_over_frac=0.002):
 
 # ---------------------------------------------------------------------------- DOS
 @pytest.mark.parametrize("occ,sdw", [({}, {}), ({"half_deg": 30.0, "max_packing": 2}, {"half_deg": 5.0, "max_packing": 1}),
-                                     ({"half_deg": 200.0, "d_sigma": 9.0, "r_sigma": 0.1, "covered_dist": 3.5}, {})])
+                                     ({"half_deg": 200.0, "max_packing": 2, "d_sigma": 9.0, "r_sigma": 7.0,
+                                       "covered_dist": 3.5}, {})])
 def test_dos_sections_match_oracle(occ, sdw):
     wl = W.cone_blobs(n=48, width=8, height=8, seed=21)
     r, p = make(N.DOS, wl, occ=occ, sdw=sdw)
@@ -69,15 +70,11 @@ def test_dos_render_small_own_pyramid(shadow_type, apply_occ, packing):
 
 def test_dos_render_anisotropic_u16():
     vol = N.synth_noise((36, 28, 20), N.U16, octaves=3, seed=7, bands=[0.3, 0.6])
-    rgb, a = W.seeded_tf(17, 5)
-    a = np.asarray(a, np.float64).copy()
-    a[:, 1] = np.round(a[:, 1] * 65535 / 255)
-    rgb = np.asarray(rgb, np.float64).copy()
-    rgb[:, 3] = np.round(rgb[:, 3] * 65535 / 255)
+    rgb, a = W.seeded_tf(17, 5, max_density=4095)  # a 12-bit table over the 16-bit range
     r = N.Renderer(N.DOS)
     scale = (1.0, 1.5, 2.0)
     r.set_volume(vol, scale)
-    r.set_transfer_function(rgb, a, max_density=65535)
+    r.set_transfer_function(rgb, a, max_density=4095)
     p = r.params()
     p.apply_shadow, p.shadow_type, p.step_size = 1, 2, 0.625
     p.base_res[:] = [40, 32, 24]
@@ -86,7 +83,7 @@ def test_dos_render_anisotropic_u16():
     cam = W.orbit_camera(grid, 40, 40, 50.0, 30.0)
     light = W.light_from_direction((0.3, 0.8, -0.4), grid)
     img = r.render(cam, 40, 40, light=light)
-    tf = O.OTF(rgb, a, max_density=65535)
+    tf = O.OTF(rgb, a, max_density=4095)
     pyr = O.OPyr.build(vol, tf, base=(40, 32, 24), scale=scale)
     ref, m = O.render_dos(vol, tf, pyr, O.dos_params(p, grid, light), cam, 40, 40, step=0.625, scale=scale)
     assert_parity(img, ref, m)
@@ -123,6 +120,10 @@ def test_dos_parameter_errors():
         r.set_params(p)
     p = r.params()
     p.occ.initial_step = 0.0
+    with pytest.raises(N.CvrError):
+        r.set_params(p)
+    p = r.params()  # sigma would have to grow without bound: the schedule never closes
+    p.occ.half_deg, p.occ.max_packing, p.occ.d_sigma, p.occ.r_sigma = 60.0, 0, 3.0, 0.5
     with pytest.raises(N.CvrError):
         r.set_params(p)
     with pytest.raises(N.CvrError):
