@@ -60,7 +60,7 @@ def stats_frame(r, wl, cam, light, out, stream):
     return st
 
 
-def cpu_baseline(kind, wl, cam, light, params=None, budget_s=20.0):
+def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
     """The oracle's FP32 instantiation on all host cores (OpenMP over rows) on a bounded
     sample of the same frame: every `stride`-th pixel in x and y, frames/s scaled by the
     fraction of pixels rendered (stride^2)."""
@@ -86,33 +86,39 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=20.0):
         sat = O.sat(wl.volume, tf).astype(np.float32)
         prm = O.ebs_params(None, light, shadow_type=1)
         prm.sdw_max_dist = np.float32(0.75) * np.float32(np.linalg.norm(wl.grid))
-        stride = 8
-        t0 = time.perf_counter()
-        O.render_ebs(wl.volume, tf, sat, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
-        t = time.perf_counter() - t0
-        return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
-                "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), oracle FP32 "
-                          f"with fp32 SAT, OpenMP over rows; scaled"}
-    if kind == "dos":
+        what = "oracle FP32 with fp32 SAT"
+
+        def run(stride):
+            O.render_ebs(wl.volume, tf, sat, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+    elif kind == "dos":
         pyr = params["pyramid"]
         prm = O.dos_params(params["native"], wl.grid, light)
-        stride = params.get("stride", 32)
-        t0 = time.perf_counter()
-        O.render_dos(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
-        t = time.perf_counter() - t0
-        return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
-                "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), section-table "
-                          f"cone integration, oracle FP32, OpenMP over rows; scaled"}
-    # cone tracer: the exact tracer's FP32 port on the GPU-built pyramid (pyramid build not timed)
-    pyr = params["pyramid"]
-    prm = O.cone_params(wl.grid, light)
-    stride = params.get("stride", 32)
+        what = "section-table cone integration, oracle FP32"
+
+        def run(stride):
+            O.render_dos(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+    else:
+        # cone tracer: the exact tracer's FP32 port on the GPU-built pyramid (pyramid build not timed)
+        pyr = params["pyramid"]
+        prm = O.cone_params(wl.grid, light)
+        what = "exact cone tracer, oracle FP32"
+
+        def run(stride):
+            O.render_cone(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+    # a coarse probe, then a pixel stride sized for about `budget_s` of CPU work
+    stride = 64
     t0 = time.perf_counter()
-    O.render_cone(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+    run(stride)
     t = time.perf_counter() - t0
+    target = max(2, int(stride * (t / budget_s) ** 0.5))
+    if target < stride:
+        stride = target
+        t0 = time.perf_counter()
+        run(stride)
+        t = time.perf_counter() - t0
     return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th pixel of one {W_}x{H_} frame (1/{stride*stride} of it), exact cone "
-                      f"tracer, oracle FP32, OpenMP over rows; scaled"}
+            "sample": f"every {stride}th pixel in x and y of one {W_}x{H_} frame (1/{stride * stride} of it, "
+                      f"{t:.1f} s), {what}, OpenMP over rows; scaled to whole frames"}
 
 
 def bench_batch(args, N, W, torch, stream):
@@ -178,6 +184,8 @@ def main():
     ap.add_argument("--workload", default="nn", choices=["nn", "cone", "rc1p", "ebs", "dos", "batch"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=0, help="override the volume size (tests)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with three 256 MiB device-to-device copies (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -284,12 +292,25 @@ def main():
             sys.path.insert(0, str(ROOT / "tests"))
             import oracle_lib as O
             levels = (r if tech in (N.CONE_TRACE, N.DOS) else ct).pyramid()
-            params = {"pyramid": O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in levels]), "stride": 64,
-                      "native": r.params()}
+            params = {"pyramid": O.OPyr.wrap(wl.grid, [a.astype(np.float64) for a in levels]), "native": r.params()}
         t0 = time.perf_counter()
         line["cpu_baseline"] = cpu_baseline(kind, wl, cam, light, params)
         log(f"cpu baseline {time.perf_counter() - t0:.1f}s: {line['cpu_baseline']}")
     print(json.dumps(line), flush=True)
+    if args.calib_copy:
+        calib_copy(torch)
+
+
+def calib_copy(torch):
+    """Three copies of 256 MiB: a kernel of known traffic (read 256 MiB, write 256 MiB)
+    to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    n = 256 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"calibration copies: 3 x {n} bytes read + written")
 
 
 if __name__ == "__main__":

@@ -24,6 +24,8 @@ VolGeom geom_of(const cvr_ctx* c) {
   }
   g.vox = c->d_vol;
   g.quad = c->d_vol_quad;
+  g.nbx = bricks(c->nx);
+  g.nby = bricks(c->ny);
   return g;
 }
 

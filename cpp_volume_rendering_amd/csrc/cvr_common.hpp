@@ -35,13 +35,26 @@ struct RayFrame {
 };
 
 // Volume geometry shared by all marchers.
+// Brick order of the neighbour-block grids: 4 x 4 x 4 cells per brick (x fastest inside
+// a brick, bricks x fastest), so that samples close in any direction share cache lines.
+// (The brick number fits 32 bits for every grid the library accepts: at most 2^34 cells.)
+__host__ __device__ __forceinline__ size_t brick_index(int x, int y, int z, int nbx, int nby) {
+  const unsigned int b = (unsigned(z >> 2) * unsigned(nby) + unsigned(y >> 2)) * unsigned(nbx) + unsigned(x >> 2);
+  return (size_t(b) << 6) | size_t(((z & 3) << 4) | ((y & 3) << 2) | (x & 3));
+}
+__host__ __device__ __forceinline__ int bricks(int n) { return (n + 3) >> 2; }
+__host__ __device__ __forceinline__ size_t brick_cells(int nx, int ny, int nz) {
+  return size_t(bricks(nx)) * size_t(bricks(ny)) * size_t(bricks(nz)) * 64u;
+}
+
 struct VolGeom {
   int nx, ny, nz, dtype;
   float G[3];        // box size, world units
   float scale[3];    // voxel size
   float tex_mul[3];  // n / G: world -> texel units
   const void* vox;   // raw voxels, x fastest
-  const void* quad;  // per voxel its (x, x+1) x (y, y+1) block, edge-clamped (4 values)
+  const void* quad;  // per voxel its (x, x+1) x (y, y+1) block, edge-clamped (4 values), brick order
+  int nbx, nby;      // bricks along x and y
 };
 
 // --------------------------------------------------------------------------- device helpers
@@ -140,8 +153,8 @@ __device__ __forceinline__ float sample_volume_quad(const VolGeom& g, f3 p) {
   const int zi = int(zf);
   z0 = min(max(zi, 0), g.nz - 1);
   const int z1 = min(max(zi + 1, 0), g.nz - 1);
-  const size_t row = size_t(y0) * g.nx + x0, plane = size_t(g.nx) * g.ny;
-  const float4 a = Quad<T>::f(q[row + z0 * plane]), b = Quad<T>::f(q[row + z1 * plane]);
+  const float4 a = Quad<T>::f(q[brick_index(x0, y0, z0, g.nbx, g.nby)]);
+  const float4 b = Quad<T>::f(q[brick_index(x0, y0, z1, g.nbx, g.nby)]);
   return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
 }
 

@@ -59,29 +59,30 @@ __device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, size_t
                      __uint_as_float(r.y & 0xFFFF0000u));
 }
 
-// GL_LINEAR lookup of pyramid level `l` at normalised coordinate (u,v,w) through the
-// neighbour blocks (2 vector loads per trilinear read).
+// GL_LINEAR lookup of pyramid level `l` at texture-space point tp through the neighbour
+// blocks (2 vector loads per trilinear read); the texel coordinate tp * dims / G - 1/2
+// is one fma per axis.
 template <int F>
-__device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, float u, float v,
-                                         float w) {
+__device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, f3 tp) {
   const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
   int x0, y0;
   float fx, fy;
-  quad_axis(u * float(nx) - 0.5f, nx, x0, fx);
-  quad_axis(v * float(ny) - 0.5f, ny, y0, fy);
-  const float zt = w * float(nz) - 0.5f;
+  quad_axis(fmaf(tp.x, pg.ts[l][0], -0.5f), nx, x0, fx);
+  quad_axis(fmaf(tp.y, pg.ts[l][1], -0.5f), ny, y0, fy);
+  const float zt = fmaf(tp.z, pg.ts[l][2], -0.5f);
   const float zf = floorf(zt);
   const float fz = zt - zf;
   const int zi = int(zf);
   const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
-  const size_t row = pg.off[l] + size_t(y0 * nx + x0), plane = size_t(nx * ny);
-  return quad_lerp(pyr_block<F>(pyr, row + z0 * plane), pyr_block<F>(pyr, row + z1 * plane), fx, fy, fz);
+  const size_t base = pg.boff[l];
+  return quad_lerp(pyr_block<F>(pyr, base + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])),
+                   pyr_block<F>(pyr, base + brick_index(x0, y0, z1, pg.nb[l][0], pg.nb[l][1])), fx, fy, fz);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
 // exp(-|clamp(p, 0, G) - p|^2 / (2 (2^lod)^2)); the distance is 0 inside the box, so the
 // factor is exactly 1 there and no branch is needed (the loads of many lookups can then
-// be issued back to back).
+// be issued back to back).  inv2sg2 = 1 / (2 (2^lod)^2).
 __device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float inv2sg2) {
   const float cx = fminf(fmaxf(tp.x, 0.f), pg.G[0]) - tp.x;
   const float cy = fminf(fmaxf(tp.y, 0.f), pg.G[1]) - tp.y;
@@ -98,19 +99,15 @@ __device__ __forceinline__ float gauss_ext(const void* __restrict__ pyr, const P
   const int d1 = int(floorf(lod));
   const int d2 = min(d1 + 1, pg.levels - 1);
   const float fr = lod - float(d1);
-  const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  const float a = pyr_tri<F>(pyr, pg, d1, u, v, w);
-  const float b = pyr_tri<F>(pyr, pg, d2, u, v, w);
-  const float sg = exp2f(lod);
-  return (a * (1.f - fr) + b * fr) * border_falloff(pg, tp, 0.5f / (sg * sg));
+  const float a = pyr_tri<F>(pyr, pg, d1, tp);
+  const float b = pyr_tri<F>(pyr, pg, d2, tp);
+  return (a * (1.f - fr) + b * fr) * border_falloff(pg, tp, 0.5f * exp2f(-2.f * lod));
 }
 
 // The same lookup at a whole level `l` (one trilinear read).
 template <int F>
 __device__ __forceinline__ float gauss_ext_level(const void* __restrict__ pyr, const PyrGeom& pg, f3 tp, int l) {
-  const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  const float sg = float(1 << l);
-  return pyr_tri<F>(pyr, pg, l, u, v, w) * border_falloff(pg, tp, 0.5f / (sg * sg));
+  return pyr_tri<F>(pyr, pg, l, tp) * border_falloff(pg, tp, __uint_as_float((126u - 2u * unsigned(l)) << 23));
 }
 
 template <int F>

@@ -11,7 +11,10 @@ struct PyrGeom {
   int levels;
   int fmt;  // storage of the neighbour blocks: 0 fp32 (float4), 1 fp16, 2 bf16 (4 x 16 bit)
   int dims[kMaxLevels][3];
-  size_t off[kMaxLevels];
+  size_t off[kMaxLevels];   // level offsets in the linear fp32 copy
+  size_t boff[kMaxLevels];  // level offsets in the brick-ordered neighbour blocks
+  int nb[kMaxLevels][2];    // bricks along x and y per level
+  float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
 };
 // Section table of one directional-occlusion cone: rows (interval, level, d_integral,

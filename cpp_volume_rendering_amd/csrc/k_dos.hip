@@ -48,11 +48,9 @@ __device__ __forceinline__ float dos_ext(const void* __restrict__ pyr, const Pyr
   const float lc = fminf(fmaxf(level, 0.f), float(pg.levels - 1));
   const int d1 = int(floorf(lc));
   const float fr = lc - float(d1);
-  const float u = tp.x / pg.G[0], v = tp.y / pg.G[1], w = tp.z / pg.G[2];
-  float a = pyr_tri<F>(pyr, pg, d1, u, v, w);
-  if (fr > 0.f) a = a * (1.f - fr) + pyr_tri<F>(pyr, pg, min(d1 + 1, pg.levels - 1), u, v, w) * fr;
-  const float sg = exp2f(level);
-  return a * border_falloff(pg, tp, 0.5f / (sg * sg));
+  float a = pyr_tri<F>(pyr, pg, d1, tp);
+  if (fr > 0.f) a = a * (1.f - fr) + pyr_tri<F>(pyr, pg, min(d1 + 1, pg.levels - 1), tp) * fr;
+  return a * border_falloff(pg, tp, 0.5f * exp2f(-2.f * level));
 }
 
 // One cone through its section table; sub-cone axes are k*z + uy*y + vx*x.

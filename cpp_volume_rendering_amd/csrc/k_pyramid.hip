@@ -140,12 +140,17 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   PyrGeom pg{};
   pg.levels = L;
   pg.fmt = fmt;
-  size_t total = 0;
+  size_t total = 0, btotal = 0;
   for (int i = 0; i < L; ++i) {
     pg.dims[i][0] = std::max(1, base_x >> i);
     pg.dims[i][1] = std::max(1, base_y >> i);
     pg.dims[i][2] = std::max(1, base_z >> i);
     pg.off[i] = total;
+    pg.boff[i] = btotal;
+    for (int k = 0; k < 3; ++k) pg.ts[i][k] = float(double(pg.dims[i][k]) / double(geom_of(c).G[k]));
+    pg.nb[i][0] = bricks(pg.dims[i][0]);
+    pg.nb[i][1] = bricks(pg.dims[i][1]);
+    btotal += brick_cells(pg.dims[i][0], pg.dims[i][1], pg.dims[i][2]);
     total += size_t(pg.dims[i][0]) * pg.dims[i][1] * pg.dims[i][2];
   }
   VolGeom g = geom_of(c);
@@ -184,7 +189,7 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   CVR_HIP_CHECK(hipFree(d_op));
   c->pyr = pg;
-  if (build_pyramid_quad(c, total, fmt, st) != CVR_OK) {
+  if (build_pyramid_quad(c, btotal, fmt, st) != CVR_OK) {
     c->pyr_built_for = ~0ull;
     return -5;
   }

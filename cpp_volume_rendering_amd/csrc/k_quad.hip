@@ -3,7 +3,9 @@
 // neighbours clamped to the edge, so a GL_LINEAR trilinear read is two vector loads
 // (planes z0 and z1) instead of eight scalar gathers.  Used for the raw volume
 // (uchar4 / ushort4 per voxel) and for every level of the extinction pyramid (float4,
-// or four fp16 / bf16 values in 8 bytes).
+// or four fp16 / bf16 values in 8 bytes).  The blocks are stored in brick order
+// (4 x 4 x 4 cells per brick, brick_index), so that the samples of a wavefront, which
+// spread over a small 3D region, touch few cache lines.
 #include "cvr_ctx.hpp"
 #include <hip/hip_fp16.h>
 
@@ -23,7 +25,7 @@ __global__ __launch_bounds__(256) void k_make_quad(const T* __restrict__ src, Q*
   q.y = src[r0 + x1];
   q.z = src[r1 + x];
   q.w = src[r1 + x1];
-  dst[r0 + x] = q;
+  dst[brick_index(x, y, z, bricks(nx), bricks(ny))] = q;
 }
 
 template <typename T, typename Q>
@@ -36,7 +38,7 @@ static int make_quad(const T* src, Q* dst, int nx, int ny, int nz, hipStream_t s
 }
 
 int build_volume_quad(cvr_ctx* c, hipStream_t st) {
-  const size_t cells = size_t(c->nx) * c->ny * c->nz;
+  const size_t cells = brick_cells(c->nx, c->ny, c->nz);
   const size_t bytes = cells * (c->dtype == CVR_U8 ? sizeof(uchar4) : sizeof(ushort4));
   if (c->d_vol_quad) CVR_HIP_CHECK(hipFree(c->d_vol_quad));
   c->d_vol_quad = nullptr;
@@ -83,7 +85,7 @@ int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st) {
   }
   const PyrGeom& pg = c->pyr;
   for (int l = 0; l < pg.levels; ++l) {
-    const int rc = make_quad(c->d_pyr + pg.off[l], q32 + pg.off[l], pg.dims[l][0], pg.dims[l][1], pg.dims[l][2], st);
+    const int rc = make_quad(c->d_pyr + pg.off[l], q32 + pg.boff[l], pg.dims[l][0], pg.dims[l][1], pg.dims[l][2], st);
     if (rc != CVR_OK) {
       (void)hipFree(q32);
       return rc;
