@@ -8,6 +8,9 @@ namespace cvr {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4v_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8v_t __attribute__((ext_vector_type(8)));
 
 constexpr int kH = 64, kOut = 16;
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kNFeat;
@@ -15,10 +18,18 @@ constexpr int kOffW2 = kOffB1 + kH, kOffB2 = kOffW2 + kH * kH;
 constexpr int kOffW3 = kOffB2 + kH, kOffB3 = kOffW3 + kH * kH;
 constexpr int kOffW4 = kOffB3 + kH, kOffB4 = kOffW4 + kOut * kH;
 constexpr int kNParams = kOffB4 + kOut;  // 10448
-// Packed inference weights: 40 A-operand tiles of 64 lanes x 4 bf16.
-constexpr int kTiles1 = 4, kTiles2 = 16, kTiles3 = 16, kTiles4 = 4;
+// Packed inference weights: layer 1 as 4 A tiles of v_mfma_f32_16x16x16_bf16 (64 lanes x
+// 4 bf16), layers 2, 3 and 4 as A tiles of v_mfma_f32_16x16x32_bf16 (64 lanes x 8 bf16):
+// 4 x 2, 4 x 2 and 1 x 2 (row tile x 32-wide K chunk).
+constexpr int kTiles1 = 4, kTiles2 = 8, kTiles3 = 8, kTiles4 = 2;
 constexpr int kNTiles = kTiles1 + kTiles2 + kTiles3 + kTiles4;
-constexpr int kPackShorts = kNTiles * 64 * 4;
+constexpr int kPackShorts = kTiles1 * 64 * 4 + (kTiles2 + kTiles3 + kTiles4) * 64 * 8;
+
+// K order of a 32-wide chunk of v_mfma_f32_16x16x32_bf16 (lane quarter q holds K = 8q..8q+7)
+// mapped onto the units of the layer before, so that a layer's D tiles are the next
+// layer's B operand without any lane exchange: K = 8q + e of chunk c is unit
+// 16 (2c + e/4) + 4q + e%4 (rows 4q..4q+3 of D tiles 2c and 2c+1 of that lane).
+__host__ __device__ constexpr int chunk_unit(int c, int q, int e) { return 16 * (2 * c + (e >> 2)) + 4 * q + (e & 3); }
 constexpr int kNBias = kH + kH + kH + kOut;
 constexpr int kWaveScratch = kNFeat * 64 + 2 * 64;
 
@@ -33,6 +44,21 @@ __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(uint32_t
 __device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ f32x4_t mfma32(bf16x8v_t a, bf16x8v_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ReLU of two D tiles and round to nearest even into one 8 x bf16 B operand
+// (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ bf16x8v_t relu_pack(f32x4_t a, f32x4_t b) {
+  f32x8_t v;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    v[e] = fmaxf(a[e], 0.f);
+    v[4 + e] = fmaxf(b[e], 0.f);
+  }
+  return __builtin_convertvector(v, bf16x8v_t);
+}
 
 // Wavefront-cooperative inference: every lane brings one sample's 16 features; the
 // 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
@@ -46,50 +72,40 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const bf16x4_t* W = reinterpret_cast<const bf16x4_t*>(sW);
-  const bf16x4_t* W4 = W + (kTiles1 + kTiles2 + kTiles3) * 64;
+  const bf16x4_t* W1 = reinterpret_cast<const bf16x4_t*>(sW);
+  const bf16x8v_t* W2 = reinterpret_cast<const bf16x8v_t*>(sW + kTiles1 * 64 * 4);
+  const bf16x8v_t* W3 = W2 + kTiles2 * 64;
+  const bf16x8v_t* W4 = W3 + kTiles3 * 64;
+  const f32x4_t* B4 = reinterpret_cast<const f32x4_t*>(sB);  // 4 biases of rows 4q..4q+3 per (layer, row tile)
   // Sample tiles are independent through the network: each 16-sample column tile goes
-  // through all four layers before the next, so only its 4 activation tiles are live.
+  // through all four layers before the next, so only its activation tiles are live.
 #pragma unroll 1
   for (int nt = 0; nt < 4; ++nt) {
-    bf16x4_t xb, hb[4];
+    f32x4_t xv;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) xb[e] = f2bf(sX[(4 * q + e) * 64 + nt * 16 + r]);
+    for (int e = 0; e < 4; ++e) xv[e] = sX[(4 * q + e) * 64 + nt * 16 + r];
+    const bf16x4_t xb = __builtin_bit_cast(bf16x4_t, __builtin_convertvector(xv, bf16x4v_t));
+    f32x4_t h[4];
     // layer 1: K = 16
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      f32x4_t acc;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] = sB[mt * 16 + 4 * q + e];
-      acc = mfma16(W[mt * 64 + lane], xb, acc);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
-    }
-    // layers 2 and 3: K = 64
+    for (int mt = 0; mt < 4; ++mt) h[mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
+    // layers 2 and 3: K = 64 as two 32-wide chunks
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-      const bf16x4_t* Wl = W + (kTiles1 + l * kTiles2) * 64;
-      const float* bl = sB + kH * (l + 1);
-      bf16x4_t nb[4];
+      const bf16x8v_t* Wl = l == 0 ? W2 : W3;
+      const bf16x8v_t b0 = relu_pack(h[0], h[1]), b1 = relu_pack(h[2], h[3]);
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
-        f32x4_t acc;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e] = bl[mt * 16 + 4 * q + e];
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = mfma16(Wl[(mt * 4 + kt) * 64 + lane], hb[kt], acc);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) nb[mt][e] = f2bf(fmaxf(acc[e], 0.f));
+        f32x4_t acc = B4[(l + 1) * 16 + mt * 4 + q];
+        acc = mfma32(Wl[(mt * 2) * 64 + lane], b0, acc);
+        h[mt] = mfma32(Wl[(mt * 2 + 1) * 64 + lane], b1, acc);
       }
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) hb[mt] = nb[mt];
     }
     // layer 4: 16 outputs (rows 0, 1 used)
-    f32x4_t acc;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[e] = sB[3 * kH + 4 * q + e];
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) acc = mfma16(W4[kt * 64 + lane], hb[kt], acc);
+    const bf16x8v_t b0 = relu_pack(h[0], h[1]), b1 = relu_pack(h[2], h[3]);
+    f32x4_t acc = B4[48 + q];
+    acc = mfma32(W4[lane], b0, acc);
+    acc = mfma32(W4[64 + lane], b1, acc);
     // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r
     if (q == 0) {
       sX[kNFeat * 64 + (nt * 16 + r) * 2] = acc[0];
@@ -105,7 +121,6 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
   __builtin_amdgcn_wave_barrier();
 }
 
-// ---------------------------------------------------------------- surrogate renderer
 struct NnK {
   float lo[kNFeat], hi[kNFeat];  // training range (+- margin): outside -> exact tracer
   int exact_all;                 // validation gate failed: every sample uses the exact tracer

@@ -51,8 +51,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                       int nq, int tiles_x, NnSched sc, size_t img_px,
                                                       float4* __restrict__ out, unsigned int* __restrict__ heads,
                                                       unsigned long long* __restrict__ counters) {
-  __shared__ short sW[kPackShorts];
-  __shared__ float sB[kNBias];
+  __shared__ __align__(16) short sW[kPackShorts];
+  __shared__ __align__(16) float sB[kNBias];
   __shared__ float sX[4][kWaveScratch];
   __shared__ int s_item;
   extern __shared__ float4 s_tf[];

@@ -183,17 +183,22 @@ __global__ __launch_bounds__(64) void k_mlp_eval(const float* __restrict__ prm, 
   }
 }
 
-// Packs the fp32 master weights into bf16 A-operand tiles (tile-major, lane-major,
-// 4 consecutive k) and the biases into fp32.
+// Packs the fp32 master weights into the bf16 A-operand tiles mlp_wave reads (layer 1:
+// 4 consecutive k per lane; layers 2-4: the 8 units chunk_unit gives per lane) and the
+// biases into fp32.
 __global__ void k_mlp_pack(const float* __restrict__ prm, short* __restrict__ wpack, float* __restrict__ bpack) {
   const int t = blockIdx.x;  // tile
   const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
-  int off, in, mt, kt;
-  if (t < kTiles1) { off = kOffW1; in = kNFeat; mt = t; kt = 0; }
-  else if (t < kTiles1 + kTiles2) { off = kOffW2; in = kH; mt = (t - kTiles1) / 4; kt = (t - kTiles1) % 4; }
-  else if (t < kTiles1 + kTiles2 + kTiles3) { off = kOffW3; in = kH; mt = (t - kTiles1 - kTiles2) / 4; kt = (t - kTiles1 - kTiles2) % 4; }
-  else { off = kOffW4; in = kH; mt = 0; kt = t - kTiles1 - kTiles2 - kTiles3; }
-  for (int e = 0; e < 4; ++e) wpack[(t * 64 + lane) * 4 + e] = f2bf(prm[off + (mt * 16 + r) * in + kt * 16 + 4 * q + e]);
+  if (t < kTiles1) {
+    for (int e = 0; e < 4; ++e) wpack[(t * 64 + lane) * 4 + e] = f2bf(prm[kOffW1 + (t * 16 + r) * kNFeat + 4 * q + e]);
+  } else {
+    int u = t - kTiles1, off, mt, c;
+    if (u < kTiles2) { off = kOffW2; mt = u / 2; c = u % 2; }
+    else if (u < kTiles2 + kTiles3) { u -= kTiles2; off = kOffW3; mt = u / 2; c = u % 2; }
+    else { u -= kTiles2 + kTiles3; off = kOffW4; mt = 0; c = u; }
+    short* dst = wpack + kTiles1 * 64 * 4 + (size_t(t - kTiles1) * 64 + lane) * 8;
+    for (int e = 0; e < 8; ++e) dst[e] = f2bf(prm[off + (mt * 16 + r) * kH + chunk_unit(c, q, e)]);
+  }
   if (t == 0) {
     for (int i = lane; i < kNBias; i += 64) {
       const int o = i < 3 * kH ? (i < kH ? kOffB1 + i : (i < 2 * kH ? kOffB2 + i - kH : kOffB3 + i - 2 * kH)) : kOffB4 + i - 3 * kH;
@@ -205,8 +210,8 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, short* __restrict__ wp
 // Batch inference through the register path (the one the renderer uses).
 __global__ __launch_bounds__(256) void k_mlp_infer(const short* __restrict__ wpack, const float* __restrict__ bpack,
                                                    const float* __restrict__ X, int n, float* __restrict__ Y) {
-  __shared__ short sW[kPackShorts];
-  __shared__ float sB[kNBias];
+  __shared__ __align__(16) short sW[kPackShorts];
+  __shared__ __align__(16) float sB[kNBias];
   __shared__ float sX[4][kWaveScratch];
   for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
     reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
@@ -303,8 +308,8 @@ __global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGe
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
                                                    int ntf, float step, float et, float4* __restrict__ out,
                                                    unsigned long long* __restrict__ counters) {
-  __shared__ short sW[kPackShorts];
-  __shared__ float sB[kNBias];
+  __shared__ __align__(16) short sW[kPackShorts];
+  __shared__ __align__(16) float sB[kNBias];
   __shared__ float sX[4][kWaveScratch];
   extern __shared__ float4 s_tf[];
   nn_stage(wpack, bpack, tf, ntf, sW, sB, s_tf);

@@ -65,10 +65,15 @@ def test_inference_matches_fp64(nn_handle, path):
 
 
 def test_register_path_equals_training_path(nn_handle):
+    """The renderer's register path (16x16x32 MFMA chunks, hardware bf16 rounding) and the
+    training kernels' path (16x16x16 MFMA from LDS) round the same bf16 activations; fp32
+    sums in another order can move an activation across a bf16 rounding boundary, so
+    they agree to bf16 resolution."""
     nn_handle.nn_set_weights(random_net(3))
     X = np.random.default_rng(4).uniform(0, 1, (777, 16)).astype(np.float32)
     a, b = nn_handle.nn_eval(X, 0), nn_handle.nn_eval(X, 1)
-    assert np.abs(a - b).max() <= 1e-5
+    print(np.abs(a - b).max())
+    assert np.abs(a - b).max() <= 2e-3 * (1 + np.abs(a).max())
 
 
 def test_gradient_matches_fp64(nn_handle):
