@@ -60,6 +60,44 @@ def stats_frame(r, wl, cam, light, out, stream):
     return st
 
 
+def gather_bytes(tech, st, r, bpv):
+    """Algorithmic bytes one frame gathers: each trilinear read counts its 8 texels (volume
+    samples at the voxel size, pyramid reads as fp32 or 16-bit values); the cone tracer
+    reads 2 levels per step, the section-table renderer one level per section and
+    sub-cone, the surrogate 16 whole-level reads per shaded sample."""
+    from cpp_volume_rendering_amd import native as N
+    p = r.params()
+    vol = st.samples * 8 * bpv
+    if tech == N.DOS:
+        b = 4 if p.density_precision == 0 else 2
+        reads = 0
+        for which, on in ((0, p.apply_occlusion), (1, p.apply_shadow)):
+            if on:
+                c = r.dos_sections(which)[1]
+                reads += int(c[0]) + 3 * int(c[1]) + 7 * int(c[2])
+        return vol + st.shaded_samples * reads * 8 * b
+    if tech == N.CONE_TRACE:
+        b = 4 if p.density_precision == 0 else 2
+        return vol + st.shaded_samples * (p.n_occ_cones + 1) * p.n_steps * 2 * 8 * b
+    if tech == N.CONE_NN:
+        b = 4 if p.cone.density_precision == 0 else 2
+        exact = st.fallback_samples * (p.cone.n_occ_cones + 1) * p.cone.n_steps * 2 * 8 * b
+        return vol + st.shaded_samples * 16 * 8 * b + exact
+    return vol
+
+
+def measured_traffic(workload, width):
+    """HBM bytes per launch of the surrogate renderer from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), when it was
+    measured on this workload."""
+    f = ROOT / "profiles" / f"k_nn_render_{workload}_{width}.md"
+    if not f.exists():
+        return None
+    import re
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", f.read_text())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
     """The oracle's FP32 instantiation on all host cores (OpenMP over rows) on a bounded
     sample of the same frame: every `stride`-th pixel in x and y, frames/s scaled by the
@@ -269,17 +307,16 @@ def main():
                        "exact_cone_fps": round(1000.0 / ms_ct, 4), "exact_cone_ms": round(ms_ct, 4),
                        "speedup_vs_exact": round(ms_ct / ms, 3), "image_rmse_vs_exact": round(rmse, 6),
                        "exact_gate_fallback": int(rep.exact_fallback)})
-        # roofline of the dominant kernel (the surrogate render): MLP flops per launch
-        flops = st.shaded_samples * MLP_FLOP_PER_SAMPLE
-        achieved = flops / (ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / BF16_PEAK_TFLOPS, 6), "traffic": None}
-    else:
-        # every trilinear sample gathers 8 voxels, plus the frame written once
-        gathered = st.samples * 8 * bpv + wl.width * wl.height * 16
-        achieved = gathered / (ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
+        # the network's share of the frame: 20480 FLOP per shaded sample
+        config["mlp_tflops"] = round(st.shaded_samples * MLP_FLOP_PER_SAMPLE / (ms * 1e-3) / 1e12, 3)
+    # roofline of the dominant kernel (one launch per frame): the algorithmic bytes of its
+    # gathers, every trilinear read counted as its 8 texels, plus the frame written once
+    gathered = gather_bytes(tech, st, r, bpv) + wl.width * wl.height * 16
+    achieved = gathered / (ms * 1e-3) / 1e9
+    config["gathered_bytes_per_frame"] = int(gathered)
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": measured_traffic(wl.name, wl.width) if tech == N.CONE_NN else None}
     line = {
         "metric": METRIC, "value": round(fps, 4), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",

@@ -177,6 +177,7 @@ static void defaults(int technique, void* p) {
     n->seed = 4;
     n->tile_w = 8;
     n->shade_batch = 0.5f;
+    n->span = 1;
   }
 }
 
@@ -425,8 +426,8 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
   if (h->technique == CVR_CONE_NN) {
     const cvr_nn_params* q = static_cast<const cvr_nn_params*>(params);
     if ((q->tile_w != 4 && q->tile_w != 8 && q->tile_w != 16 && q->tile_w != 32) ||
-        !(q->shade_batch >= 0.f && q->shade_batch <= 1.f)) {
-      set_error("cvr_set_params: tile_w must be 4, 8, 16 or 32 and shade_batch in [0, 1]");
+        !(q->shade_batch >= 0.f && q->shade_batch <= 1.f) || (q->span != 1 && q->span != 2 && q->span != 4 && q->span != 8)) {
+      set_error("cvr_set_params: tile_w must be 4, 8, 16 or 32, shade_batch in [0, 1], span 1, 2, 4 or 8");
       return CVR_ERR_INVALID;
     }
   }

@@ -135,51 +135,82 @@ struct NnK {
 struct NnSched {
   int tw_log2;
   float park;
+  int span;  // wave tiles per wave (pixels handed out as lanes free up)
 };
 
 NnK nn_consts(cvr_ctx* c);
 ConeK cone_constants(const cvr_ctx* c, const cvr_cone_params& P, const float L[3]);
 void light_dir(const RayFrame& rf, const cvr_light* light, float L[3]);
 
-// Renders one workgroup tile (2 x 2 wave tiles of the NnSched shape) of frame `rf`.
-// sW/sB/s_tf: the network and transfer function staged in LDS; sXw: this wave's LDS
-// scratch.  Counters (samples, shaded, exact fallbacks) accumulate into cnt.
+// Renders one workgroup tile of frame `rf`: 2 x 2 waves, each owning `span` wave tiles
+// of the NnSched shape stacked along y.  A lane whose ray ends writes its pixel and,
+// once enough lanes of the wave are idle, takes the wave's next unrendered pixel, so the
+// wave keeps its lanes busy until its pixels run out instead of idling behind its
+// longest ray.  sW/sB/s_tf: the network and transfer function staged in LDS; sXw: this
+// wave's LDS scratch.  Counters (samples, shaded, exact fallbacks) accumulate into cnt.
 template <typename T, int F>
 __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, const PyrGeom& pg,
                                         const void* __restrict__ pyr, const ConeK& K, const NnK& nk,
                                         const NnSched& sc, const short* sW, const float* sB, float* sXw,
                                         const float4* s_tf, int ntf, float step, float et, float4* __restrict__ out,
                                         int tx, int ty, unsigned int* cnt) {
-  const int tw = 1 << sc.tw_log2, th = 64 >> sc.tw_log2;
+  const int tl = sc.tw_log2, tw = 1 << tl, th = 64 >> tl, span = sc.span;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int px = tx * 2 * tw + (w & 1) * tw + (lane & (tw - 1));
-  const int py = ty * 2 * th + (w >> 1) * th + (lane >> sc.tw_log2);
-  const bool in_img = px < rf.W && py < rf.H;
-  const f3 dir = camera_ray(rf, in_img ? px : 0, in_img ? py : 0);
+  const int x0 = tx * 2 * tw + (w & 1) * tw;
+  const int y0 = (ty * 2 + (w >> 1)) * th * span;
+  const int total = 64 * span;
   const f3 eye = ld3(rf.eye);
-  float tn = 0.f, tf_ = 0.f;
-  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  const f3 half = mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
   const float ka = K.apply_occ ? K.ka : 0.f, kd = K.apply_sdw ? K.kd : 0.f;
   const float inv_k = 1.0f / (ka + kd);
-  bool active = in_img && ray_box(eye, dir, g.G, tn, tf_);
-  f3 k, up, right;
-  occ_frame(dir, k, up, right);
   const f3 L = ld3(K.L);
-  const float D = fabsf(tf_ - tn);
-  const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-  int j = 0;
-  // A lane whose sample needs shading parks on it (keeps p, h, src) while the other
-  // lanes march on; the wave shades all parked samples together once enough lanes
-  // wait, or when every marching lane waits.  The loop runs while any lane marches.
+  const uint64_t below = (1ull << lane) - 1ull;
+  // per-lane ray state (pix < 0: idle)
+  int pix = -1, px = 0, py = 0, j = 0;
+  float D = 0.f, h = 0.f;
+  f3 dir = mk3(0.f, 0.f, 1.f), tp = eye, p = eye, k, up, right;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f), src = dst;
   bool parked = false;
-  float h = 0.f;
-  f3 p = tp;
-  float4 src = make_float4(0.f, 0.f, 0.f, 0.f);
-  while (__any(active)) {
-    if (active && !parked) {
+  int next = 0;  // wave-uniform: next pixel of the span to hand out
+  for (;;) {
+    const uint64_t idle = __ballot(pix < 0);
+    const int n_idle = __popcll(idle);
+    if (next < total && (n_idle >= 16 || n_idle == 64)) {
+      // hand the next pixels to the idle lanes, in lane order
+      const int mine = next + __popcll(idle & below);
+      if (pix < 0 && mine < total) {
+        const int t = mine >> 6, l = mine & 63;
+        px = x0 + (l & (tw - 1));
+        py = y0 + t * th + (l >> tl);
+        float tn = 0.f, tf_ = 0.f;
+        if (px < rf.W && py < rf.H) {
+          dir = camera_ray(rf, px, py);
+          if (ray_box(eye, dir, g.G, tn, tf_)) {
+            pix = mine;
+            occ_frame(dir, k, up, right);
+            D = fabsf(tf_ - tn);
+            tp = eye + dir * tn + half;
+            j = 0;
+            dst = make_float4(0.f, 0.f, 0.f, 0.f);
+          } else {
+            out[size_t(py) * rf.W + px] = make_float4(0.f, 0.f, 0.f, 0.f);  // the ray misses the box
+          }
+        }
+      }
+      next += n_idle;
+    }
+    const bool busy = pix >= 0;
+    if (!__any(busy)) {
+      if (next >= total) break;
+      continue;
+    }
+    // one sample of every marching lane; a lane whose sample needs shading parks on it
+    // (keeps p, h, src) while the others march on
+    bool finished = false;
+    if (busy && !parked) {
       const float s = float(j) * step;
       if (!(s < D) || j >= kMaxSamples) {
-        active = false;
+        finished = true;
       } else {
         h = fminf(step, D - s);
         p = tp + dir * (s + 0.5f * h);
@@ -189,7 +220,9 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
         else ++j;
       }
     }
-    const int np = __popcll(__ballot(parked)), na = __popcll(__ballot(active));
+    // the wave shades all parked samples together once enough lanes wait, or when every
+    // marching lane waits
+    const int np = __popcll(__ballot(parked)), na = __popcll(__ballot(busy && !finished));
     if (np > 0 && (np == na || float(np) >= sc.park * float(na))) {
       float f[kNFeat];
       bool ood = false;
@@ -226,11 +259,14 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
         dst.w += wgt;
         ++j;
         parked = false;
-        if (dst.w > et) active = false;
+        if (dst.w > et) finished = true;
       }
     }
+    if (finished) {
+      out[size_t(py) * rf.W + px] = dst;
+      pix = -1;
+    }
   }
-  if (in_img) out[size_t(py) * rf.W + px] = dst;
 }
 
 // Stages the packed network and the transfer function of one volume in LDS (all

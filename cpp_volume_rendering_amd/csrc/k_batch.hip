@@ -112,7 +112,7 @@ struct cvr_batch {
   int device = 0, W = 0, H = 0;
   std::vector<cvr_handle> vols;
   std::vector<cvr_job> jobs;
-  cvr_batch_tuning tune{8, 0.5f, kQueues, 2, 0.0};
+  cvr_batch_tuning tune{8, 0.5f, kQueues, 2, 1, 0.0};
   // device side
   VolDesc* d_vols = nullptr;
   JobDesc* d_jobs = nullptr;
@@ -221,7 +221,7 @@ int build_tables(cvr_batch* b) {
     if (cells[va] != cells[vc]) return cells[va] > cells[vc];
     return va < vc;
   });
-  const int tl = tw_log2_of(b->tune.tile_w), tw = 1 << tl, th = 64 >> tl;
+  const int tl = tw_log2_of(b->tune.tile_w), tw = 1 << tl, th = (64 >> tl) * b->tune.span;
   const int tiles_x = (b->W + 2 * tw - 1) / (2 * tw), tiles_y = (b->H + 2 * th - 1) / (2 * th);
   const int nq = b->tune.queues;
   std::vector<std::vector<int2>> q(nq);
@@ -256,7 +256,7 @@ int build_graph(cvr_batch* b, float* out) {
   const int grid = std::max(1, ncu * b->tune.workgroups_per_cu);
   const int tl = tw_log2_of(b->tune.tile_w), tw = 1 << tl;
   const int tiles_x = (b->W + 2 * tw - 1) / (2 * tw);
-  NnSched sc{tl, b->tune.shade_batch};
+  NnSched sc{tl, b->tune.shade_batch, b->tune.span};
   const size_t shm = size_t(b->max_ntf) * sizeof(float4);
   CVR_HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->cap, b->d_heads, b->d_counters);
@@ -280,8 +280,10 @@ int build_graph(cvr_batch* b, float* out) {
 
 int check_tuning(const cvr_batch_tuning& t) {
   if ((t.tile_w != 4 && t.tile_w != 8 && t.tile_w != 16 && t.tile_w != 32) || !(t.shade_batch >= 0.f && t.shade_batch <= 1.f) ||
-      (t.queues != 1 && t.queues != kQueues) || t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4) {
-    set_error("batch tuning: tile_w in {4,8,16,32}, shade_batch in [0,1], queues 1 or 8, workgroups_per_cu in [1,4]");
+      (t.queues != 1 && t.queues != kQueues) || t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 ||
+      (t.span != 1 && t.span != 2 && t.span != 4 && t.span != 8)) {
+    set_error("batch tuning: tile_w in {4,8,16,32}, shade_batch in [0,1], queues 1 or 8, workgroups_per_cu in [1,4], "
+              "span 1, 2, 4 or 8");
     return CVR_ERR_INVALID;
   }
   return CVR_OK;
@@ -371,7 +373,7 @@ int cvr_batch_set_tuning(cvr_batch_handle b, const cvr_batch_tuning* t) {
   if (!b || !t) { set_error("cvr_batch_set_tuning: null argument"); return CVR_ERR_INVALID; }
   int rc = check_tuning(*t);
   if (rc) return rc;
-  const bool items_change = t->tile_w != b->tune.tile_w || t->queues != b->tune.queues;
+  const bool items_change = t->tile_w != b->tune.tile_w || t->queues != b->tune.queues || t->span != b->tune.span;
   b->tune = *t;
   if (items_change) b->tables_dirty = true;
   b->dirty = true;
@@ -433,15 +435,19 @@ int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_str
   const float sbs[] = {0.f, 0.25f, 0.5f, 0.75f, 1.f};
   const int qs[] = {1, kQueues};
   const int wgs[] = {1, 2, 3};
-  for (int pass = 0; pass < 4 && rc == CVR_OK; ++pass) {
-    const int nopt = pass == 0 ? 4 : pass == 1 ? 5 : pass == 2 ? 2 : 3;
+  const int spans[] = {1, 2, 4, 8};
+  for (int pass = 0; pass < 5 && rc == CVR_OK; ++pass) {
+    const int nopt = pass == 0 ? 4 : pass == 1 ? 5 : pass == 2 ? 2 : pass == 3 ? 3 : 4;
     for (int o = 0; o < nopt && rc == CVR_OK; ++o) {
       cvr_batch_tuning t = cur;
       if (pass == 0) t.tile_w = tws[o];
       if (pass == 1) t.shade_batch = sbs[o];
       if (pass == 2) t.queues = qs[o];
       if (pass == 3) t.workgroups_per_cu = wgs[o];
-      if (std::memcmp(&t, &cur, sizeof(t) - sizeof(double)) == 0) continue;
+      if (pass == 4) t.span = spans[o];
+      if (t.tile_w == cur.tile_w && t.shade_batch == cur.shade_batch && t.queues == cur.queues &&
+          t.workgroups_per_cu == cur.workgroups_per_cu && t.span == cur.span)
+        continue;
       double ms = 0;
       rc = time_of(t, &ms);
       if (rc == CVR_OK && ms < cur_ms) {

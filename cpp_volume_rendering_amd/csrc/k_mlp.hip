@@ -303,7 +303,7 @@ __global__ void k_feat_range(const float* __restrict__ X, int n, unsigned int* _
 
 // ---------------------------------------------------------------- surrogate renderer
 template <typename T, int F>
-__global__ __launch_bounds__(256) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                    ConeK K, NnK nk, NnSched sc, const short* __restrict__ wpack,
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
                                                    int ntf, float step, float et, float4* __restrict__ out,
@@ -600,9 +600,9 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   }
   VolGeom g = geom_of(c);
   const float step = N.cone.step_size > 0 ? N.cone.step_size : default_step(c);
-  NnSched sc{0, N.shade_batch};
+  NnSched sc{0, N.shade_batch, N.span};
   while ((1 << sc.tw_log2) < N.tile_w) ++sc.tw_log2;
-  const int tw = 1 << sc.tw_log2, th = 64 / tw;
+  const int tw = 1 << sc.tw_log2, th = (64 / tw) * N.span;
   const int tiles = ((rf.W + 2 * tw - 1) / (2 * tw)) * ((rf.H + 2 * th - 1) / (2 * th));
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 32 * 1024) { set_error("surrogate renderer: transfer function larger than 2048 entries"); return -4; }

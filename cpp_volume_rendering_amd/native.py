@@ -80,7 +80,8 @@ class ConeParams(C.Structure):
 class NnParams(C.Structure):
     _fields_ = [("cone", ConeParams), ("n_train", C.c_int), ("n_val", C.c_int), ("epochs", C.c_int),
                 ("batch", C.c_int), ("lr", C.c_float), ("rmse_gate", C.c_float), ("ood_margin", C.c_float),
-                ("force_exact", C.c_int), ("seed", C.c_uint64), ("tile_w", C.c_int), ("shade_batch", C.c_float)]
+                ("force_exact", C.c_int), ("seed", C.c_uint64), ("tile_w", C.c_int), ("shade_batch", C.c_float),
+                ("span", C.c_int)]
 
 
 class NnReport(C.Structure):
@@ -95,7 +96,7 @@ class Job(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_w", C.c_int), ("shade_batch", C.c_float), ("queues", C.c_int),
-                ("workgroups_per_cu", C.c_int), ("ms", C.c_double)]
+                ("workgroups_per_cu", C.c_int), ("span", C.c_int), ("ms", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -191,6 +191,7 @@ typedef struct cvr_nn_params {
    * together (0: as soon as one waits). */
   int tile_w;           /* 8 */
   float shade_batch;    /* 0.5 */
+  int span;             /* wave tiles per wavefront, handed to lanes as their rays end: 1, 2, 4 or 8 (1) */
 } cvr_nn_params;
 
 typedef struct cvr_nn_report {
@@ -333,6 +334,7 @@ typedef struct cvr_batch_tuning {
   float shade_batch;     /* fraction of marching lanes that wait before a shading pass, [0, 1] */
   int queues;            /* 1: one global queue; 8: one queue per XCD with stealing */
   int workgroups_per_cu; /* persistent workgroups per compute unit, 1..4 */
+  int span;              /* wave tiles per wavefront, handed to lanes as their rays end: 1, 2, 4 or 8 */
   double ms;             /* autotune: the measured time of the chosen shape */
 } cvr_batch_tuning;
 

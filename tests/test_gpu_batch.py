@@ -52,11 +52,12 @@ def scene():
 def batch_reference(b, n):
     """The batch's images at the default work shape (rendered once per batch)."""
     t = b.tuning()
-    b.set_tuning(tile_w=8, shade_batch=0.5, queues=8, workgroups_per_cu=2)
+    b.set_tuning(tile_w=8, shade_batch=0.5, queues=8, workgroups_per_cu=2, span=1)
     out = torch.zeros((n, HT, WD, 4), device="cuda")
     b.render(out)
     torch.cuda.synchronize()
-    b.set_tuning(tile_w=t.tile_w, shade_batch=t.shade_batch, queues=t.queues, workgroups_per_cu=t.workgroups_per_cu)
+    b.set_tuning(tile_w=t.tile_w, shade_batch=t.shade_batch, queues=t.queues, workgroups_per_cu=t.workgroups_per_cu,
+                 span=t.span)
     return out.cpu().numpy()
 
 
@@ -91,11 +92,11 @@ def test_graph_replays_equal_single_frames(scene):
     assert st.shaded_samples > 0 and st.fallback_samples <= st.shaded_samples
 
 
-@pytest.mark.parametrize("tile_w,shade,queues,wg", [(16, 0.0, 1, 1), (4, 1.0, 8, 3), (32, 0.75, 8, 2)])
-def test_work_shapes_do_not_change_pixels(scene, tile_w, shade, queues, wg):
+@pytest.mark.parametrize("tile_w,shade,queues,wg,span", [(16, 0.0, 1, 1, 1), (4, 1.0, 8, 3, 8), (32, 0.75, 8, 2, 2)])
+def test_work_shapes_do_not_change_pixels(scene, tile_w, shade, queues, wg, span):
     rs, jobs, ref = scene
     b = make_batch(rs, jobs)
-    b.set_tuning(tile_w=tile_w, shade_batch=shade, queues=queues, workgroups_per_cu=wg)
+    b.set_tuning(tile_w=tile_w, shade_batch=shade, queues=queues, workgroups_per_cu=wg, span=span)
     out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
     b.render(out)
     torch.cuda.synchronize()
