@@ -26,6 +26,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "frames/s 512³@1024² DAO+shadow: neural vs cone-traced at RMSE≤0.01; HBM GB/s"
+WORKLOAD_METRIC = {
+    "rc1p": "frames/s 128³ u8@512² plain single-pass compositing (rc1pass); HBM GB/s",
+    "ebs": "frames/s 256³ u16@1024² extinction-based shading (SAT AO + directional cone shadow); HBM GB/s",
+    "cone": "frames/s 256³ u8@1024² cone-traced DAO+shadow (8+1 cones x 64 steps, 6-level pyramid); HBM GB/s",
+    "dos": "frames/s 256³ u8@1024² section-table directional occlusion + point-light shadow; HBM GB/s",
+}
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_SAMPLE = 2 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 16)  # 20480
@@ -68,6 +74,9 @@ def gather_bytes(tech, st, r, bpv):
     from cpp_volume_rendering_amd import native as N
     p = r.params()
     vol = st.samples * 8 * bpv
+    if tech == N.EBS:
+        # each box sum reads 8 trilinear SAT taps of 8 float2 (hi, lo) cells
+        return vol + st.box_sums * 8 * 8 * 8
     if tech == N.DOS:
         b = 4 if p.density_precision == 0 else 2
         reads = 0
@@ -86,11 +95,11 @@ def gather_bytes(tech, st, r, bpv):
     return vol
 
 
-def measured_traffic(workload, width):
-    """HBM bytes per launch of the surrogate renderer from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), when it was
+def measured_traffic(kernel, workload, width):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), when one was
     measured on this workload."""
-    f = ROOT / "profiles" / f"k_nn_render_{workload}_{width}.md"
+    f = ROOT / "profiles" / f"{kernel}_{workload}_{width}.md"
     if not f.exists():
         return None
     import re
@@ -155,7 +164,7 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
         run(stride)
         t = time.perf_counter() - t0
     return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}th pixel in x and y of one {W_}x{H_} frame (1/{stride * stride} of it, "
+            "sample": f"one pixel in {stride} along x and along y of one {W_}x{H_} frame (1/{stride * stride} of it, "
                       f"{t:.1f} s), {what}, OpenMP over rows; scaled to whole frames"}
 
 
@@ -314,11 +323,14 @@ def main():
     gathered = gather_bytes(tech, st, r, bpv) + wl.width * wl.height * 16
     achieved = gathered / (ms * 1e-3) / 1e9
     config["gathered_bytes_per_frame"] = int(gathered)
+    kernel = {N.RC1P: "k_rc1p", N.EBS: "k_ebs", N.DOS: "k_dos_render", N.CONE_TRACE: "k_cone_render",
+              N.CONE_NN: "k_nn_render"}[tech]
+    config["kernel"] = kernel
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(wl.name, wl.width) if tech == N.CONE_NN else None}
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, wl.name, wl.width)}
     line = {
-        "metric": METRIC, "value": round(fps, 4), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "metric": WORKLOAD_METRIC.get(args.workload, METRIC), "value": round(fps, 4), "unit": "frames/s",
+        "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded)", "config": config, "roofline": roof,
     }

@@ -524,6 +524,7 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
     h->stats.samples = cnt[0];
     h->stats.shaded_samples = cnt[1];
     h->stats.fallback_samples = cnt[2];
+    h->stats.box_sums = cnt[3];
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_ms = ms;
   }

@@ -195,7 +195,7 @@ __device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i ==
 
 // EvaluateShadowSAT3D (:148-188) summed along the cone; directional light from the table.
 __device__ float ebs_shadow_table(const float2* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
-                                  f3 tx) {
+                                  f3 tx, unsigned int& nbox) {
   const float pa = comp(tx, k.axis), lo = k.minVol[k.axis], hi = k.maxVol[k.axis];
   double st = 0.0;
   for (int r = 0; r < nrows; ++r) {
@@ -204,12 +204,13 @@ __device__ float ebs_shadow_table(const float2* __restrict__ sat, const EbsK& k,
     if (!(nxt > lo && nxt < hi)) break;
     f3 p1 = tx + ld3(R.o1), p2 = tx + ld3(R.o2);
     st += box_sum(sat, k, p1, p2) * double(R.inv_vq) * double(k.sdw_weight);
+    ++nbox;
   }
   return __expf(-float(st));
 }
 
 // Point light: the cone axis depends on the sample (:437-438).
-__device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k, f3 tx) {
+__device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k, f3 tx, unsigned int& nbox) {
   f3 real = tx - mk3(0.5f * k.g.G[0], 0.5f * k.g.G[1], 0.5f * k.g.G[2]);
   f3 cvv = norm3(ld3(k.light_pos) - real);
   float cv[3] = {cvv.x, cvv.y, cvv.z};
@@ -228,6 +229,7 @@ __device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k,
     const float vq = (fabsf(p1.x - p2.x) / k.scale[0]) * (fabsf(p1.y - p2.y) / k.scale[1]) *
                      (fabsf(p1.z - p2.z) / k.scale[2]);
     st += (box_sum(sat, k, p1, p2) / double(vq)) * double(k.sdw_weight);
+    ++nbox;
     apos = apos + cg.interval;
   }
   return __expf(-float(st));
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
   f3 eye = ld3(rf.eye);
   float tn, tf_;
   float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
-  unsigned int nsamp = 0, nshade = 0;
+  unsigned int nsamp = 0, nshade = 0, nbox = 0;
   const float ka = k.apply_occ ? k.ka : 0.f, kd = k.apply_sdw ? k.kd : 0.f;
   const float inv_k = 1.0f / (ka + kd);
   if (ray_box(eye, dir, k.g.G, tn, tf_)) {
@@ -305,8 +307,12 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
       if (src.w > 0.0f) {
         ++nshade;
         float Io = 0.f, Is = 0.f;
-        if (k.apply_occ) Io = ebs_ao(sat, k, p);
-        if (k.apply_sdw) Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, p) : ebs_shadow_point(sat, k, p);
+        if (k.apply_occ) {
+          Io = ebs_ao(sat, k, p);
+          nbox += unsigned(k.shells);
+        }
+        if (k.apply_sdw)
+          Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, p, nbox) : ebs_shadow_point(sat, k, p, nbox);
         // (1/(ka+kd)) * (rgb*Io*ka + rgb*Is*kd), :547
         src.x = inv_k * (src.x * Io * ka + src.x * Is * kd);
         src.y = inv_k * (src.y * Io * ka + src.y * Is * kd);
@@ -325,6 +331,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
   if (counters) {
     atomicAdd(&counters[0], (unsigned long long)nsamp);
     atomicAdd(&counters[1], (unsigned long long)nshade);
+    atomicAdd(&counters[3], (unsigned long long)nbox);
   }
 }
 

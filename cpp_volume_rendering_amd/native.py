@@ -101,7 +101,7 @@ class BatchTuning(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("shaded_samples", C.c_uint64), ("fallback_samples", C.c_uint64),
-                ("last_ms", C.c_double)]
+                ("last_ms", C.c_double), ("box_sums", C.c_uint64)]
 
 
 PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, CONE_TRACE: ConeParams,

@@ -251,6 +251,7 @@ typedef struct cvr_stats {
   uint64_t shaded_samples;   /* samples with nonzero extinction (lighting evaluated) */
   uint64_t fallback_samples; /* surrogate: samples the exact tracer shaded */
   double last_ms;            /* device time of the last render's main kernel */
+  uint64_t box_sums;         /* CVR_EBS: SAT box sums evaluated (8 trilinear SAT reads each) */
 } cvr_stats;
 int cvr_get_stats(cvr_handle h, cvr_stats* out);
 int cvr_set_collect_stats(cvr_handle h, int enable);
