@@ -21,6 +21,21 @@ struct StencilF {
 // ring), kNSdw shadow segments, the local opacity.
 constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7, kNStencil = 2 * kNOcc + kNSdw, kMaxSteps = 256;
 
+// One surrogate input's trilinear read with its pyramid level resolved on the host:
+// the stencil point (distance along its axis, segment length) and the geometry of the
+// level it reads (texel coordinate per world unit, size, bricks, block offset, border
+// falloff 1 / (2 (2^l)^2)).  64 bytes; the table is read through scalar loads.
+struct FeatTap {
+  float dist, seg;
+  float ts[3];
+  float inv2sg2;
+  int dims[3];
+  int nb[2];
+  unsigned int boff_lo, boff_hi;
+  int level, pad[2];
+};
+static_assert(sizeof(FeatTap) == 64, "FeatTap is 16 dwords");
+
 // Host dispatch over the compile-time variants of the pyramid-reading kernels: calls
 // fn(Tag<voxel type>{}, integral_constant<pyramid storage>{}).
 template <class T> struct Tag { typedef T type; };
@@ -43,7 +58,7 @@ struct ConeK {
   float L[3];               // shadow cone axis (towards the light)
   const ConeStepF* so;      // occlusion schedule [n_steps]
   const ConeStepF* ss;      // shadow schedule [n_steps]
-  const StencilF* st;       // surrogate stencil: axial/ring pairs [2*kNOcc], then shadow [kNSdw]
+  const FeatTap* ft;        // surrogate inputs: axial/ring pairs [2*kNOcc], shadow [kNSdw], local [1]
 };
 
 // One neighbour block of the pyramid in its storage format: fp32 (float4), or four
@@ -148,21 +163,68 @@ __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, cons
 
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
-// opacities along the light (8..14), the local opacity (15).
+// opacities along the light (8..14), the local opacity (15).  All 32 block loads are
+// issued before the first is used (texel weights and falloff x segment length kept in
+// registers), so one memory round trip serves the whole stencil; each read is
+// gauss_ext_level's, with the segment length folded into the falloff factor.
+template <int F> struct PyrBlk { typedef uint2 type; };
+template <> struct PyrBlk<0> { typedef float4 type; };
+
 template <int F>
-__device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const StencilF* __restrict__ st,
+__device__ __forceinline__ float4 pyr_decode(typename PyrBlk<F>::type r) {
+  if constexpr (F == 0) {
+    return r;
+  } else if constexpr (F == 1) {
+    return make_float4(__half2float(__ushort_as_half(ushort(r.x & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.x >> 16))),
+                       __half2float(__ushort_as_half(ushort(r.y & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.y >> 16))));
+  } else {
+    return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xFFFF0000u), __uint_as_float(r.y << 16),
+                       __uint_as_float(r.y & 0xFFFF0000u));
+  }
+}
+
+typedef const __attribute__((address_space(4))) FeatTap* ConstTaps;
+
+#ifndef CVR_FEAT_GROUP
+#define CVR_FEAT_GROUP 4
+#endif
+constexpr int kFeatGroup = CVR_FEAT_GROUP;
+
+template <int F>
+__device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const FeatTap* __restrict__ taps,
                                          f3 p, f3 k, f3 L, float* f) {
+  typedef typename PyrBlk<F>::type B;
+  // the table is wave-uniform and read-only: constant address space -> scalar loads
+  const ConstTaps ft = (ConstTaps)(uintptr_t)taps;
 #pragma unroll
-  for (int i = 0; i < 2 * kNOcc; ++i) {
-    const StencilF so = st[i];
-    f[i] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p + k * so.dist, int(so.lod)) * so.seg);
-  }
+  for (int g0 = 0; g0 < kNFeat; g0 += kFeatGroup) {
+    B b0[kFeatGroup], b1[kFeatGroup];
+    float fx[kFeatGroup], fy[kFeatGroup], fz[kFeatGroup], w[kFeatGroup];
 #pragma unroll
-  for (int i = 0; i < kNSdw; ++i) {
-    const StencilF ss = st[2 * kNOcc + i];
-    f[2 * kNOcc + i] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p + L * ss.dist, int(ss.lod)) * ss.seg);
+    for (int j = 0; j < kFeatGroup; ++j) {
+      const int i = g0 + j;
+      const f3 a = i < 2 * kNOcc ? k : L;
+      const f3 tp = i == kNFeat - 1 ? p : p + a * ft[i].dist;
+      int x0, y0;
+      quad_axis(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dims[0], x0, fx[j]);
+      quad_axis(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dims[1], y0, fy[j]);
+      const float zt = fmaf(tp.z, ft[i].ts[2], -0.5f);
+      const float zf = floorf(zt);
+      fz[j] = zt - zf;
+      const int zi = int(zf), nz = ft[i].dims[2];
+      const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
+      const B* lv = static_cast<const B*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
+      b0[j] = lv[brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1])];
+      b1[j] = lv[brick_index(x0, y0, z1, ft[i].nb[0], ft[i].nb[1])];
+      w[j] = border_falloff(pg, tp, ft[i].inv2sg2);
+      if (i != kNFeat - 1) w[j] *= ft[i].seg;
+    }
+    // keep the machine scheduler from pulling the lerps up between the loads
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kFeatGroup; ++j)
+      f[g0 + j] = 1.f - __expf(-(quad_lerp(pyr_decode<F>(b0[j]), pyr_decode<F>(b1[j]), fx[j], fy[j], fz[j]) * w[j]));
   }
-  f[15] = 1.f - __expf(-gauss_ext_level<F>(pyr, pg, p, 0));
 }
 
 // Network-free estimate from the same inputs (the network predicts the residual):

@@ -64,7 +64,7 @@ struct cvr_ctx {
   void* d_pyr_quad = nullptr;  // neighbour blocks of every level (PyrGeom::fmt)
   cvr::PyrGeom pyr{};
   uint64_t pyr_built_for = ~0ull;
-  void* d_cone_tab = nullptr;  // cone step schedules + surrogate stencil
+  void* d_cone_tab = nullptr;  // surrogate input taps + cone step schedules
 
   // directional occlusion shading: section tables (occlusion rows, then shadow rows)
   cvr::DosTable dos_occ, dos_sdw;

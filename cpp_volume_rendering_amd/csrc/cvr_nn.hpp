@@ -228,7 +228,7 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
       bool ood = false;
       float bo = 0.f, bs = 0.f;
       if (parked) {
-        nn_feats<F>(pyr, pg, K.st, p, k, L, f);
+        nn_feats<F>(pyr, pg, K.ft, p, k, L, f);
 #pragma unroll
         for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
         // the network-free estimate now, so the features die inside the network's input staging

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_cone_points(PyrGeom pg, const void* __r
   if (sdw) sdw[i] = cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L);
   if (feat) {
     float f[kNFeat];
-    nn_feats<F>(pyr, pg, K.st, p, k, L, f);
+    nn_feats<F>(pyr, pg, K.ft, p, k, L, f);
     for (int c = 0; c < kNFeat; ++c) feat[size_t(i) * kNFeat + c] = f[c];
   }
 }
@@ -140,6 +140,26 @@ static void stencil(const cvr_cone_params& P, double occ_max, double sdw_max, in
   }
 }
 
+// A stencil point read at its whole level: the level's geometry copied next to it.
+static FeatTap feat_tap(const PyrGeom& pg, const StencilF& s) {
+  FeatTap t{};
+  const int l = int(s.lod);
+  t.dist = s.dist;
+  t.seg = s.seg;
+  for (int a = 0; a < 3; ++a) {
+    t.ts[a] = pg.ts[l][a];
+    t.dims[a] = pg.dims[l][a];
+  }
+  uint32_t bits = (126u - 2u * unsigned(l)) << 23;  // 1 / (2 (2^l)^2), exactly
+  std::memcpy(&t.inv2sg2, &bits, 4);
+  t.nb[0] = pg.nb[l][0];
+  t.nb[1] = pg.nb[l][1];
+  t.boff_lo = unsigned(pg.boff[l] & 0xFFFFFFFFu);
+  t.boff_hi = unsigned(pg.boff[l] >> 32);
+  t.level = l;
+  return t;
+}
+
 int check_cone_params(const cvr_cone_params& P) {
   if (P.n_steps < 1 || P.n_steps > kMaxSteps || P.n_occ_cones < 1 || P.n_occ_cones > 64 || !(P.t0 > 0) ||
       !(P.r_sigma > 0) || !(P.sigma0 > 0) || !(P.occ_half_deg > 0 && P.occ_half_deg < 90) ||
@@ -177,12 +197,16 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   schedule(P.n_steps, P.t0, sdw_max, P.sdw_half_deg, P.r_sigma, P.sigma0, c->pyr.levels, ss);
   std::vector<StencilF> stn;
   stencil(P, occ_max, sdw_max, c->pyr.levels, stn);
-  const size_t bytes = (so.size() + ss.size()) * sizeof(ConeStepF) + stn.size() * sizeof(StencilF);
-  if (!c->d_cone_tab) CVR_HIP_CHECK(hipMalloc(&c->d_cone_tab, 2 * kMaxSteps * sizeof(ConeStepF) + kNStencil * sizeof(StencilF)));
+  stn.push_back(StencilF{0.f, 1.f, 0.f, 0.f});  // the local opacity: level 0 at the sample
+  std::vector<FeatTap> taps(kNFeat);
+  for (int i = 0; i < kNFeat; ++i) taps[i] = feat_tap(c->pyr, stn[i]);
+  // [taps (kNFeat x 64 B)] [occlusion schedule] [shadow schedule]
+  const size_t bytes = kNFeat * sizeof(FeatTap) + (so.size() + ss.size()) * sizeof(ConeStepF);
+  if (!c->d_cone_tab) CVR_HIP_CHECK(hipMalloc(&c->d_cone_tab, kNFeat * sizeof(FeatTap) + 2 * kMaxSteps * sizeof(ConeStepF)));
   std::vector<char> buf(bytes);
-  std::memcpy(buf.data(), so.data(), so.size() * sizeof(ConeStepF));
-  std::memcpy(buf.data() + so.size() * sizeof(ConeStepF), ss.data(), ss.size() * sizeof(ConeStepF));
-  std::memcpy(buf.data() + (so.size() + ss.size()) * sizeof(ConeStepF), stn.data(), stn.size() * sizeof(StencilF));
+  std::memcpy(buf.data(), taps.data(), kNFeat * sizeof(FeatTap));
+  std::memcpy(buf.data() + kNFeat * sizeof(FeatTap), so.data(), so.size() * sizeof(ConeStepF));
+  std::memcpy(buf.data() + kNFeat * sizeof(FeatTap) + so.size() * sizeof(ConeStepF), ss.data(), ss.size() * sizeof(ConeStepF));
   CVR_HIP_CHECK(hipMemcpy(c->d_cone_tab, buf.data(), bytes, hipMemcpyHostToDevice));
   c->cone_n_steps = P.n_steps;
   return 0;
@@ -201,10 +225,9 @@ ConeK cone_constants(const cvr_ctx* c, const cvr_cone_params& P, const float L[3
   K.kd = P.kd;
   double l = std::sqrt(double(L[0]) * L[0] + double(L[1]) * L[1] + double(L[2]) * L[2]);
   for (int i = 0; i < 3; ++i) K.L[i] = float(L[i] / l);
-  const ConeStepF* base = static_cast<const ConeStepF*>(c->d_cone_tab);
-  K.so = base;
-  K.ss = base + P.n_steps;
-  K.st = reinterpret_cast<const StencilF*>(base + 2 * P.n_steps);
+  K.ft = static_cast<const FeatTap*>(c->d_cone_tab);
+  K.so = reinterpret_cast<const ConeStepF*>(K.ft + kNFeat);
+  K.ss = K.so + P.n_steps;
   return K;
 }
 

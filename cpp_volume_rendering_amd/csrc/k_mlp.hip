@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const
   f3 k, up, right;
   occ_frame(dir, k, up, right);
   float f[kNFeat];
-  nn_feats<F>(pyr, pg, K.st, p, k, L, f);
+  nn_feats<F>(pyr, pg, K.ft, p, k, L, f);
   for (int c = 0; c < kNFeat; ++c) X[size_t(i) * kNFeat + c] = f[c];
   float bo, bs;
   nn_base(f, float(K.n_occ - 1) * K.ring_cos, bo, bs);
