@@ -1,9 +1,11 @@
 // Headless counterpart of the reference application's main (cppvolrend/main.cpp:49-68):
-// registers the techniques with the RenderingManager, loads a volume (.raw) or makes a
-// seeded synthetic one, loads a .tf1d transfer function, renders frames, writes the
-// eval.csv line and a PFM/PPM of the last frame.
+// registers the techniques with the RenderingManager, loads a volume (.raw, .syn,
+// .nrrd/.nhrd, .dat, or one entry of a dataset list) or makes a seeded synthetic one,
+// loads a .tf1d transfer function, renders frames, writes the eval.csv line and a
+// PFM/PPM of the last frame.
 //
-//   cvr_cli --technique rc1p|ebs|cone|nn|gt|dos [--volume f.raw | --synth n[,seed[,blobs]]]
+//   cvr_cli --technique rc1p|ebs|cone|nn|gt|dos
+//           [--volume file | --datasets dir/list --dataset id|name | --synth n[,seed[,blobs]]]
 //           [--tf f.tf1d] [--size WxH] [--frames N] [--csv eval.csv] [--pfm out.pfm]
 //           [--ppm out.ppm] [--light x,y,z] [--check-inputs]
 //
@@ -19,14 +21,15 @@
 
 static int usage() {
   std::fprintf(stderr,
-               "usage: cvr_cli --technique rc1p|ebs|cone|nn|gt|dos [--volume f.raw | --synth n[,seed[,blobs]]] "
+               "usage: cvr_cli --technique rc1p|ebs|cone|nn|gt|dos [--volume file | --datasets dir/list "
+               "--dataset id|name | --synth n[,seed[,blobs]]] "
                "[--tf f.tf1d] [--size WxH] [--frames N] [--csv f] [--pfm f] [--ppm f] [--light x,y,z] "
                "[--check-inputs]\n");
   return 2;
 }
 
 int main(int argc, char** argv) {
-  std::string tech = "rc1p", volume, tfpath, csv, pfm, ppm, synth = "64,1,8";
+  std::string tech = "rc1p", volume, tfpath, csv, pfm, ppm, synth = "64,1,8", datasets, dataset;
   int W = 512, H = 512, frames = 1;
   bool check_only = false, have_light = false;
   std::array<float, 3> light{0, 0, 1};
@@ -35,6 +38,8 @@ int main(int argc, char** argv) {
     auto next = [&]() -> std::string { return i + 1 < argc ? argv[++i] : ""; };
     if (a == "--technique") tech = next();
     else if (a == "--volume") volume = next();
+    else if (a == "--datasets") datasets = next();
+    else if (a == "--dataset") dataset = next();
     else if (a == "--synth") synth = next();
     else if (a == "--tf") tfpath = next();
     else if (a == "--size") { if (std::sscanf(next().c_str(), "%dx%d", &W, &H) != 2) return usage(); }
@@ -48,7 +53,26 @@ int main(int argc, char** argv) {
   }
   std::string err;
   vis::StructuredGridVolume* vol = nullptr;
-  if (!volume.empty()) {
+  if (!datasets.empty()) {
+    vis::DataProvider dp;
+    const size_t t = datasets.find_last_of('/');
+    const std::string dir = t == std::string::npos ? "." : datasets.substr(0, t);
+    const std::string list = t == std::string::npos ? datasets : datasets.substr(t + 1);
+    if (!dp.SetStructuredGridFileList(dir, list, &err)) { std::fprintf(stderr, "datasets: %s\n", err.c_str()); return 1; }
+    int id = dp.FindStructuredGridName(dataset);
+    if (id < 0 && !dataset.empty() && dataset.find_first_not_of("0123456789") == std::string::npos) id = std::atoi(dataset.c_str());
+    if (id < 0 || id >= dp.GetNumberOfStructuredGrids()) {
+      std::fprintf(stderr, "datasets: no dataset '%s' among %d\n", dataset.c_str(), dp.GetNumberOfStructuredGrids());
+      return 1;
+    }
+    if (check_only) {
+      std::printf("datasets");
+      for (const std::string& n : dp.GetStructuredGridNameList()) std::printf(" [%s]", n.c_str());
+      std::printf("\n");
+    }
+    vol = dp.LoadStructuredGrid(unsigned(id), &err);
+    if (!vol) { std::fprintf(stderr, "volume: %s\n", err.c_str()); return 1; }
+  } else if (!volume.empty()) {
     vol = vis::VolumeReader::ReadStructuredVolume(volume, &err);
     if (!vol) { std::fprintf(stderr, "volume: %s\n", err.c_str()); return 1; }
   } else {
@@ -73,9 +97,18 @@ int main(int argc, char** argv) {
     tf->Build();
   }
   if (check_only) {
-    std::printf("volume %s %dx%dx%d bytes=%d diagonal=%.6f sample(0,0,0)=%.9f\n", vol->GetName().c_str(),
-                vol->GetWidth(), vol->GetHeight(), vol->GetDepth(), vol->GetBytesPerValue(), vol->GetDiagonal(),
-                vol->GetNormalizedSample(0, 0, 0));
+    // sum over voxels i (x fastest) of (i + 1) * value, modulo 2^64, identifies the data
+    uint64_t wsum = 0;
+    const size_t nv = size_t(vol->GetWidth()) * vol->GetHeight() * vol->GetDepth();
+    const uint8_t* vb = static_cast<const uint8_t*>(vol->GetArrayData());
+    for (size_t i = 0; i < nv; ++i) {
+      const uint64_t v = vol->GetBytesPerValue() == 1 ? vb[i] : uint64_t(vb[2 * i]) | (uint64_t(vb[2 * i + 1]) << 8);
+      wsum += uint64_t(i + 1) * v;
+    }
+    std::printf("volume %s %dx%dx%d bytes=%d diagonal=%.6f sample(0,0,0)=%.9f scale=%g,%g,%g wsum=%llu\n",
+                vol->GetName().c_str(), vol->GetWidth(), vol->GetHeight(), vol->GetDepth(), vol->GetBytesPerValue(),
+                vol->GetDiagonal(), vol->GetNormalizedSample(0, 0, 0), vol->GetScaleX(), vol->GetScaleY(),
+                vol->GetScaleZ(), (unsigned long long)wsum);
     const std::vector<double>& t = tf->Table();
     std::printf("tf max_density=%d extinction_input=%d", tf->GetMaxDensity(), tf->IsExtinctionCoefficientInput() ? 1 : 0);
     for (int i : {0, tf->GetMaxDensity() / 2, tf->GetMaxDensity()})

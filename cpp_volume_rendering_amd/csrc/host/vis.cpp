@@ -2,6 +2,7 @@
 #include "vis.h"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -170,16 +171,72 @@ TransferFunction1D* TransferFunctionReader::readtf1d(const std::string& file, st
   return tf;
 }
 
+namespace {
+std::string lower(std::string t) {
+  for (char& c : t) c = char(std::tolower(static_cast<unsigned char>(c)));
+  return t;
+}
+std::string trim(const std::string& t) {
+  const size_t a = t.find_first_not_of(" \t\r\n");
+  if (a == std::string::npos) return "";
+  return t.substr(a, t.find_last_not_of(" \t\r\n") - a + 1);
+}
+// folder of a path (no trailing separator; "." for a bare name) and its file name
+std::pair<std::string, std::string> split_path(const std::string& p) {
+  const size_t t = p.find_last_of("/\\");
+  if (t == std::string::npos) return {".", p};
+  return {p.substr(0, t), p.substr(t + 1)};
+}
+// "key: value" lines up to the first empty line (or the end); '#' lines are comments
+std::vector<std::pair<std::string, std::string>> header_keys(std::istream& in, size_t* data_offset) {
+  std::vector<std::pair<std::string, std::string>> kv;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (trim(line).empty()) break;
+    if (line[0] == '#') continue;
+    const size_t c = line.find(':');
+    if (c == std::string::npos) continue;
+    kv.emplace_back(trim(line.substr(0, c)), trim(line.substr(c + 1)));
+  }
+  if (data_offset) {
+    const std::streampos p = in.tellg();
+    *data_offset = p < 0 ? 0 : size_t(p);
+  }
+  return kv;
+}
+}  // namespace
+
 StructuredGridVolume* VolumeReader::ReadStructuredVolume(const std::string& file, std::string* err) {
-  const size_t dot = file.find_last_of('.');
-  if (dot != std::string::npos && file.substr(dot + 1) == "raw") return readraw(file, err);
-  fail(err, "unsupported volume file (only .raw): " + file);
+  const std::string ext = lower(DataProvider::GetExtension(file));
+  if (ext == "raw") return readraw(file, err);
+  if (ext == "syn") return readsyn(file, err);
+  if (ext == "nrrd") return readnrrd(file, err);
+  if (ext == "nhrd" || ext == "nhdr") return readnhrd(file, err);
+  if (ext == "dat") return readdat(file, err);
+  if (ext == "pvm" || ext == "pvmold") {
+    fail(err, "the .pvm format (third-party compressed volumes) is not supported: " + file);
+    return nullptr;
+  }
+  fail(err, "unsupported volume file (.raw, .syn, .nrrd, .nhrd, .dat): " + file);
   return nullptr;
+}
+
+bool VolumeReader::SetArrayDataFromRawFile(const std::string& path, StructuredGridVolume* sg, int bpv,
+                                           std::string* err, size_t offset) {
+  if (bpv != 1 && bpv != 2) { fail(err, "bytes per value must be 1 or 2"); return false; }
+  std::ifstream in(path, std::ios::binary);
+  if (!in.is_open()) { fail(err, "cannot open " + path); return false; }
+  if (offset) in.seekg(std::streamoff(offset));
+  std::vector<uint8_t> bytes(size_t(sg->GetWidth()) * sg->GetHeight() * sg->GetDepth() * size_t(bpv));
+  in.read(reinterpret_cast<char*>(bytes.data()), std::streamsize(bytes.size()));
+  if (size_t(in.gcount()) != bytes.size()) { fail(err, "short read: " + path); return false; }
+  return sg->SetArrayData(std::move(bytes), bpv);
 }
 
 StructuredGridVolume* VolumeReader::readraw(const std::string& path, std::string* err) {
   // strip directories, then the .raw extension, then parse <bytes>.<W>x<H>x<D>
-  std::string name = path.substr(path.find_last_of("/\\") == std::string::npos ? 0 : path.find_last_of("/\\") + 1);
+  std::string name = split_path(path).second;
   const size_t e = name.find_last_of('.');
   if (e == std::string::npos) { fail(err, "no extension"); return nullptr; }
   name = name.substr(0, e);
@@ -195,16 +252,251 @@ StructuredGridVolume* VolumeReader::readraw(const std::string& path, std::string
     fail(err, "bad sizes or bytes per value in " + path);
     return nullptr;
   }
-  std::ifstream in(path, std::ios::binary);
-  if (!in.is_open()) { fail(err, "cannot open " + path); return nullptr; }
-  std::vector<uint8_t> bytes(size_t(w) * h * d * size_t(bpv));
-  in.read(reinterpret_cast<char*>(bytes.data()), std::streamsize(bytes.size()));
-  if (size_t(in.gcount()) != bytes.size()) { fail(err, "short read: " + path); return nullptr; }
-  StructuredGridVolume* v = new StructuredGridVolume(rest.substr(0, s2), w, h, d);
+  std::unique_ptr<StructuredGridVolume> v(new StructuredGridVolume(rest.substr(0, s2), w, h, d));
   v->SetScale(1.0, 1.0, 1.0);
   v->SetName(path);
-  v->SetArrayData(std::move(bytes), bpv);
-  return v;
+  if (!SetArrayDataFromRawFile(path, v.get(), bpv, err)) return nullptr;
+  return v.release();
+}
+
+StructuredGridVolume* VolumeReader::readsyn(const std::string& path, std::string* err) {
+  std::ifstream in(path);
+  if (!in.is_open()) { fail(err, "cannot open " + path); return nullptr; }
+  long long w = 0, h = 0, d = 0;
+  if (!(in >> w >> h >> d) || w < 1 || h < 1 || d < 1 || w * h * d > (1ll << 34)) {
+    fail(err, "bad .syn size line in " + path);
+    return nullptr;
+  }
+  // voxels the file does not set stay 0
+  std::vector<uint8_t> bytes(size_t(w * h * d), 0);
+  auto in_box = [&](long long x, long long y, long long z) { return x >= 0 && y >= 0 && z >= 0 && x < w && y < h && z < d; };
+  long long kind = 0;
+  while (in >> kind) {
+    if (kind == 1) {
+      long long x0, y0, z0, x1, y1, z1, v;
+      if (!(in >> x0 >> y0 >> z0 >> x1 >> y1 >> z1 >> v)) { fail(err, "truncated box record in " + path); return nullptr; }
+      if (x0 < x1 && y0 < y1 && z0 < z1 && !(in_box(x0, y0, z0) && in_box(x1 - 1, y1 - 1, z1 - 1))) {
+        fail(err, "box outside the volume in " + path);
+        return nullptr;
+      }
+      for (long long z = z0; z < z1; ++z)
+        for (long long y = y0; y < y1; ++y)
+          for (long long x = x0; x < x1; ++x) bytes[size_t(x + w * y + w * h * z)] = uint8_t(v);
+    } else {
+      long long x, y, z, v;
+      if (!(in >> x >> y >> z >> v)) { fail(err, "truncated voxel record in " + path); return nullptr; }
+      if (!in_box(x, y, z)) { fail(err, "voxel outside the volume in " + path); return nullptr; }
+      bytes[size_t(x + w * y + w * h * z)] = uint8_t(v);
+    }
+  }
+  if (!in.eof()) { fail(err, "unreadable record in " + path); return nullptr; }
+  StructuredGridVolume* sg = new StructuredGridVolume(path, int(w), int(h), int(d));
+  sg->SetScale(1.0, 1.0, 1.0);
+  sg->SetArrayData(std::move(bytes), 1);
+  return sg;
+}
+
+StructuredGridVolume* VolumeReader::readnrrd(const std::string& path, std::string* err) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in.is_open()) { fail(err, "cannot open " + path); return nullptr; }
+  std::string magic;
+  std::getline(in, magic);
+  if (magic.compare(0, 4, "NRRD") != 0) { fail(err, "not a NRRD header (first line NRRDxxxx): " + path); return nullptr; }
+  size_t data_offset = 0;
+  const auto kv = header_keys(in, &data_offset);
+  std::string type, sizes, data_file, encoding = "raw", endian = "little";
+  int dimension = 3;
+  for (const auto& p : kv) {
+    const std::string k = lower(p.first);
+    if (k == "type") type = lower(p.second);
+    else if (k == "sizes") sizes = p.second;
+    else if (k == "dimension") dimension = std::atoi(p.second.c_str());
+    else if (k == "data file" || k == "datafile") data_file = p.second;
+    else if (k == "encoding") encoding = lower(p.second);
+    else if (k == "endian") endian = lower(p.second);
+  }
+  int res[3] = {0, 0, 0};
+  if (dimension != 3 || std::sscanf(sizes.c_str(), "%d %d %d", &res[0], &res[1], &res[2]) != 3 || res[0] < 1 ||
+      res[1] < 1 || res[2] < 1) {
+    fail(err, "NRRD: need dimension 3 and three sizes in " + path);
+    return nullptr;
+  }
+  int bpv = 0;
+  if (type == "uchar" || type == "unsigned char" || type == "uint8" || type == "uint8_t") bpv = 1;
+  if (type == "ushort" || type == "unsigned short" || type == "uint16" || type == "uint16_t" ||
+      type == "unsigned short int")
+    bpv = 2;
+  if (!bpv) { fail(err, "NRRD: type must be an unsigned 8 or 16 bit integer, got '" + type + "'"); return nullptr; }
+  if (encoding != "raw") { fail(err, "NRRD: only raw encoding is supported, got '" + encoding + "'"); return nullptr; }
+  if (bpv == 2 && endian != "little") { fail(err, "NRRD: only little-endian 16 bit data is supported"); return nullptr; }
+  const auto dn = split_path(path);
+  std::unique_ptr<StructuredGridVolume> sg(new StructuredGridVolume(dn.second, res[0], res[1], res[2]));
+  sg->SetScale(1.0, 1.0, 1.0);
+  // a detached data file is relative to the header's folder; without one the data
+  // follows the header's empty line in the same file
+  const bool ok = data_file.empty() ? SetArrayDataFromRawFile(path, sg.get(), bpv, err, data_offset)
+                                    : SetArrayDataFromRawFile(dn.first + "/" + data_file, sg.get(), bpv, err);
+  if (!ok) return nullptr;
+  return sg.release();
+}
+
+StructuredGridVolume* VolumeReader::readnhrd(const std::string& path, std::string* err) { return readnrrd(path, err); }
+
+StructuredGridVolume* VolumeReader::readdat(const std::string& path, std::string* err) {
+  std::ifstream in(path);
+  if (!in.is_open()) { fail(err, "cannot open " + path); return nullptr; }
+  const auto kv = header_keys(in, nullptr);
+  std::string object, format;
+  int res[3] = {0, 0, 0};
+  double th[3] = {1.0, 1.0, 1.0};
+  bool have_res = false;
+  for (const auto& p : kv) {
+    if (p.first == "ObjectFileName") object = p.second;
+    else if (p.first == "Resolution") have_res = std::sscanf(p.second.c_str(), "%d %d %d", &res[0], &res[1], &res[2]) == 3;
+    else if (p.first == "SliceThickness") {
+      if (std::sscanf(p.second.c_str(), "%lf %lf %lf", &th[0], &th[1], &th[2]) != 3) {
+        fail(err, ".dat: SliceThickness needs three values in " + path);
+        return nullptr;
+      }
+    } else if (p.first == "Format") format = p.second;
+  }
+  if (!have_res || res[0] < 1 || res[1] < 1 || res[2] < 1 || object.empty()) {
+    fail(err, ".dat: need ObjectFileName and Resolution in " + path);
+    return nullptr;
+  }
+  if (!(th[0] > 0 && th[1] > 0 && th[2] > 0)) { fail(err, ".dat: SliceThickness must be positive"); return nullptr; }
+  int bpv = 0;
+  if (format.find("UCHAR") != std::string::npos) bpv = 1;
+  else if (format.find("USHORT") != std::string::npos) bpv = 2;
+  if (!bpv) { fail(err, ".dat: Format must be UCHAR or USHORT in " + path); return nullptr; }
+  const auto dn = split_path(path);
+  std::unique_ptr<StructuredGridVolume> sg(new StructuredGridVolume(dn.second, res[0], res[1], res[2]));
+  sg->SetScale(th[0], th[1], th[2]);
+  if (!SetArrayDataFromRawFile(dn.first + "/" + object, sg.get(), bpv, err)) return nullptr;
+  return sg.release();
+}
+
+// ------------------------------------------------------------------ dataset lists
+void DataProvider::ClearStructuredGridFileList() {
+  names_.clear();
+  paths_.clear();
+  rows_.clear();
+  tags_.clear();
+}
+
+std::string DataProvider::GetExtension(const std::string& filename) {
+  const size_t slash = filename.find_last_of("/\\"), dot = filename.find_last_of('.');
+  if (dot == std::string::npos || (slash != std::string::npos && dot < slash)) return "";
+  return filename.substr(dot + 1);
+}
+
+int DataProvider::Split(std::vector<std::string>& tags, const std::string& line, const std::string& separator) {
+  tags.clear();
+  size_t a = 0;
+  for (;;) {
+    const size_t b = line.find(separator, a);
+    if (b == std::string::npos) break;
+    tags.push_back(line.substr(a, b - a));
+    a = b + separator.size();
+  }
+  tags.push_back(line.substr(a));
+  return int(tags.size());
+}
+
+int DataProvider::GetBytesPerValueByType(const std::string& t) {
+  if (t == "uchar" || t == "uint8") return 1;
+  if (t == "ushort" || t == "uint16") return 2;
+  return 0;
+}
+
+bool DataProvider::SetStructuredGridFileList(const std::string& file_path, const std::string& file_name,
+                                             std::string* err) {
+  ClearStructuredGridFileList();
+  dir_ = file_path.empty() ? std::string(".") : file_path;
+  while (dir_.size() > 1 && (dir_.back() == '/' || dir_.back() == '\\')) dir_.pop_back();
+  list_ = file_name;
+  std::ifstream in(dir_ + "/" + list_);
+  if (!in.is_open()) { fail(err, "cannot open the dataset list " + dir_ + "/" + list_); return false; }
+  csv_ = file_name.find('.') != std::string::npos;
+  std::string line;
+  if (!csv_) {
+    // "<relative path>   <ui name>" per line; an empty name falls back to the path
+    while (std::getline(in, line)) {
+      if (!line.empty() && line.back() == '\r') line.pop_back();
+      const size_t p0 = line.find('<'), p1 = line.find('>');
+      if (p0 == std::string::npos || p1 == std::string::npos || p1 < p0) continue;
+      const size_t n0 = line.find_last_of('<'), n1 = line.find_last_of('>');
+      const std::string path = line.substr(p0 + 1, p1 - p0 - 1);
+      const std::string name = n0 > p0 ? line.substr(n0 + 1, n1 - n0 - 1) : std::string();
+      paths_.push_back(path);
+      names_.push_back(name.empty() ? path : name);
+    }
+    return true;
+  }
+  if (lower(GetExtension(file_name)) != "csv") { fail(err, "dataset lists are CSV files or extension-less lists"); return false; }
+  if (!std::getline(in, line)) { fail(err, "empty dataset list"); return false; }
+  if (!line.empty() && line.back() == '\r') line.pop_back();
+  sep_ = ";";
+  if (line.compare(0, 3, "sep") == 0) {
+    sep_ = line.substr(line.size() - 1, 1);
+    if (!std::getline(in, line)) { fail(err, "dataset list without a header"); return false; }
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+  }
+  Split(tags_, line, sep_);
+  for (const char* need : {"Name", "Source", "File", "Size", "Spacing", "Type"})
+    if (std::find(tags_.begin(), tags_.end(), need) == tags_.end()) {
+      fail(err, std::string("dataset list lacks the column ") + need);
+      return false;
+    }
+  const int iname = int(std::find(tags_.begin(), tags_.end(), "Name") - tags_.begin());
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (trim(line).empty()) continue;
+    std::vector<std::string> v;
+    if (Split(v, line, sep_) != int(tags_.size())) { fail(err, "dataset list row with a wrong column count: " + line); return false; }
+    names_.push_back(v[iname]);
+    rows_.push_back(std::move(v));
+  }
+  return true;
+}
+
+int DataProvider::FindStructuredGridName(const std::string& name) const {
+  for (size_t i = 0; i < names_.size(); ++i)
+    if (names_[i] == name) return int(i);
+  return -1;
+}
+
+StructuredGridVolume* DataProvider::LoadStructuredGridFromRawFile(const std::string& name, const std::string& raw_file_path,
+                                                                 const std::array<int, 3>& r,
+                                                                 const std::array<double, 3>& scale, int bpv,
+                                                                 std::string* err) {
+  if (r[0] < 1 || r[1] < 1 || r[2] < 1 || !(scale[0] > 0 && scale[1] > 0 && scale[2] > 0)) {
+    fail(err, "bad size or spacing for " + name);
+    return nullptr;
+  }
+  std::unique_ptr<StructuredGridVolume> sg(new StructuredGridVolume(name, r[0], r[1], r[2]));
+  sg->SetScale(scale[0], scale[1], scale[2]);
+  if (!VolumeReader::SetArrayDataFromRawFile(raw_file_path, sg.get(), bpv, err)) return nullptr;
+  return sg.release();
+}
+
+StructuredGridVolume* DataProvider::LoadStructuredGrid(unsigned int id, std::string* err) {
+  if (id >= names_.size()) { fail(err, "no dataset " + std::to_string(id)); return nullptr; }
+  if (!csv_) return VolumeReader::ReadStructuredVolume(dir_ + "/" + paths_[id], err);
+  const std::vector<std::string>& v = rows_[id];
+  auto col = [&](const char* t) { return v[std::find(tags_.begin(), tags_.end(), t) - tags_.begin()]; };
+  const std::string file = dir_ + "/" + col("Source") + "/" + col("File");
+  if (lower(GetExtension(file)) != "raw") return VolumeReader::ReadStructuredVolume(file, err);
+  std::array<int, 3> r{0, 0, 0};
+  std::array<double, 3> sp{0, 0, 0};
+  if (std::sscanf(col("Size").c_str(), "%dx%dx%d", &r[0], &r[1], &r[2]) != 3 ||
+      std::sscanf(col("Spacing").c_str(), "%lfx%lfx%lf", &sp[0], &sp[1], &sp[2]) != 3) {
+    fail(err, "dataset " + names_[id] + ": Size must be WxHxD and Spacing sxxsyxsz");
+    return nullptr;
+  }
+  const int bpv = GetBytesPerValueByType(col("Type"));
+  if (!bpv) { fail(err, "dataset " + names_[id] + ": Type must be uchar, uint8, ushort or uint16"); return nullptr; }
+  return LoadStructuredGridFromRawFile(names_[id], file, r, sp, bpv, err);
 }
 
 // ------------------------------------------------------------------ camera / parameters

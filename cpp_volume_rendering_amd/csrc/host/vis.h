@@ -1,11 +1,14 @@
 // Headless host data model mirroring the reference's libs/volvis_utils and libs/vis_utils:
-// structured volume, 1D transfer function and its .tf1d reader, .raw reader, camera,
+// structured volume, 1D transfer function and its .tf1d reader, the structured volume
+// readers (.raw, .syn, .nrrd/.nhrd, .dat), the dataset list provider, camera,
 // rendering parameters and the data manager.  No OpenGL: GPU resources live behind the
 // C ABI (include/cvr.h).
 #pragma once
 #include <array>
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../../include/cvr.h"
@@ -82,12 +85,57 @@ class TransferFunctionReader {
   static TransferFunction1D* readtf1d(const std::string& file, std::string* err = nullptr);
 };
 
-// VolumeReader::readraw (reader.cpp:162-225): sizes and bytes per value are encoded in
-// the name, <name>.<bytes>.<W>x<H>x<D>.raw; voxel scale 1.
+// VolumeReader (reader.cpp:29-60): dispatch on the extension.  Every reader returns
+// nullptr and an error message instead of printing and continuing on bad input.
+//  - readraw (reader.cpp:162-225): <name>.<bytes>.<W>x<H>x<D>.raw, voxel scale 1.
+//  - readsyn (reader.cpp:283-371): text "W H D", then records "1 x0 y0 z0 x1 y1 z1 v"
+//    (fill the half-open box with v) or "<other> x y z v" (one voxel); 8 bit, scale 1.
+//  - readnrrd / readnhrd (reader.cpp:373-496): "key: value" header; sizes, type,
+//    data file (relative to the header's folder); scale 1.
+//  - readdat (reader.cpp:498-628): ObjectFileName, Resolution, SliceThickness (the
+//    voxel scale), Format UCHAR / USHORT.
+// .pvm / .pvmold (a third-party compressed format) are rejected.
 class VolumeReader {
  public:
   static StructuredGridVolume* ReadStructuredVolume(const std::string& file, std::string* err = nullptr);
   static StructuredGridVolume* readraw(const std::string& file, std::string* err = nullptr);
+  static StructuredGridVolume* readsyn(const std::string& file, std::string* err = nullptr);
+  static StructuredGridVolume* readnrrd(const std::string& file, std::string* err = nullptr);
+  static StructuredGridVolume* readnhrd(const std::string& file, std::string* err = nullptr);
+  static StructuredGridVolume* readdat(const std::string& file, std::string* err = nullptr);
+  // SetArrayDataFromRawFile (reader.cpp:62-98): W*H*D values of 1 or 2 bytes, x fastest
+  static bool SetArrayDataFromRawFile(const std::string& file, StructuredGridVolume* sg, int bytes_per_value,
+                                      std::string* err = nullptr, size_t offset = 0);
+};
+
+// DataProvider (volvis_utils/dataprovider.cpp:27-215): a list of structured datasets,
+// either the "<path> <ui name>" list (data/#list_structured_datasets) or a CSV with
+// an optional "sep=;" line and the columns Name, Source, File, Size, Spacing, Type
+// (data/volume_list.csv).  Paths are relative to the list's folder.
+class DataProvider {
+ public:
+  void ClearStructuredGridFileList();
+  // file_path is the folder (with or without a trailing '/'), file_name the list
+  bool SetStructuredGridFileList(const std::string& file_path, const std::string& file_name, std::string* err = nullptr);
+  int GetNumberOfStructuredGrids() const { return int(names_.size()); }
+  int FindStructuredGridName(const std::string& name) const;
+  const std::vector<std::string>& GetStructuredGridNameList() const { return names_; }
+  StructuredGridVolume* LoadStructuredGrid(unsigned int id, std::string* err = nullptr);
+  static StructuredGridVolume* LoadStructuredGridFromRawFile(const std::string& name, const std::string& raw_file_path,
+                                                            const std::array<int, 3>& resolution,
+                                                            const std::array<double, 3>& scale, int bytes_per_value,
+                                                            std::string* err = nullptr);
+  static std::string GetExtension(const std::string& filename);
+  static int Split(std::vector<std::string>& tags, const std::string& line, const std::string& separator);
+  // uchar / uint8 -> 1, ushort / uint16 -> 2, anything else -> 0
+  static int GetBytesPerValueByType(const std::string& value_type);
+
+ private:
+  std::string dir_, list_;
+  bool csv_ = false;
+  std::string sep_ = ";";
+  std::vector<std::string> tags_, names_, paths_;  // paths_: list format only
+  std::vector<std::vector<std::string>> rows_;     // CSV rows
 };
 
 // vis_utils/camera.cpp: lookAt camera, fovy 45 degrees, aspect w/h.
