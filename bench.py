@@ -152,14 +152,16 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
 
         def run(stride):
             O.render_cone(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
-    # a coarse probe, then a pixel stride sized for about `budget_s` of CPU work
+    # a coarse probe, then pixel strides sized for about `budget_s` of CPU work (the
+    # work scales with the number of pixels, 1 / stride^2)
     stride = 64
     t0 = time.perf_counter()
     run(stride)
     t = time.perf_counter() - t0
-    target = max(2, int(stride * (t / budget_s) ** 0.5))
-    if target < stride:
-        stride = target
+    for _ in range(3):
+        if t >= 0.5 * budget_s or stride <= 1:
+            break
+        stride = max(1, min(stride - 1, int(stride * (t / budget_s) ** 0.5)))
         t0 = time.perf_counter()
         run(stride)
         t = time.perf_counter() - t0

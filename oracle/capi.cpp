@@ -130,10 +130,10 @@ void orc_render_ebs_f64(int nx, int ny, int nz, int dtype, const void* data, dou
 }
 void orc_render_ebs_f32(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
                         void* tf, const float* sat, const EbsC* prm, const float* cam, int W, int H, float step,
-                        float et, float* out, int nthreads) {
+                        float et, float* out, int nthreads, int stride) {
   threads(nthreads);
   render_ebs<float, float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), sat, ebs_from(prm),
-                           mkcam(cam), W, H, step, et, out, nullptr);
+                           mkcam(cam), W, H, step, et, out, nullptr, stride);
 }
 
 // AO and shadow factors of the EBS lighting at one texture-space position (FP64).

@@ -51,7 +51,7 @@ def lib():
         "orc_sat_f64": (None, [i, i, i, i, vp, vp, vp, i]),
         "orc_sat_input_f64": (None, [i, i, i, i, vp, vp, vp]),
         "orc_render_ebs_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
-        "orc_render_ebs_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, f, f, vp, i]),
+        "orc_render_ebs_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, i, i, f, f, vp, i, i]),
         "orc_ebs_point": (None, [i, i, i, d, d, d, vp, vp, d, d, d, vp, vp]),
         "orc_pyr_new": (vp, [i, i, i, i, vp, d, d, d, vp, i, i, i, d, i, i]),
         "orc_pyr_wrap": (vp, [d, d, d, i, vp, vp]),
@@ -189,7 +189,7 @@ def render_ebs(vol, tf: OTF, sat_arr, prm: EbsC, cam, W, H, step=0.5, et=0.99, s
         s32 = np.ascontiguousarray(sat_arr, dtype=np.float32)
         out = np.zeros((H, W, 4), np.float32)
         lib().orc_render_ebs_f32(*va, tf.h, s32.ctypes.data, C.byref(prm), c.ctypes.data, W, H, step, et,
-                                 out.ctypes.data, threads)
+                                 out.ctypes.data, threads, stride)
         return out
     s64 = np.ascontiguousarray(sat_arr, dtype=np.float64)
     out = np.zeros((H, W, 4), np.float64)
