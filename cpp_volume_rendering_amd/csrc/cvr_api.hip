@@ -124,6 +124,8 @@ static void defaults(int technique, void* p) {
     e->shadow_type = 0;
     e->ka = 0.5f;
     e->kd = 0.5f;
+    e->light_cache = 0;  // ebsrenderer.cpp:46-47
+    e->light_cache_res[0] = e->light_cache_res[1] = e->light_cache_res[2] = 32;
   } else if (technique == CVR_CONE_GT) {
     // crtgtrenderer.cpp:32-51 (both lighting terms on here: with both off the shading
     // divides by zero)
@@ -160,6 +162,8 @@ static void defaults(int technique, void* p) {
     d->ka = 0.5f;
     d->kd = 0.5f;
     d->density_precision = 0;
+    d->light_cache = 0;  // dosrcrenderer.cpp:64
+    d->light_cache_res[0] = d->light_cache_res[1] = d->light_cache_res[2] = 32;
   } else if (technique == CVR_CONE_TRACE) {
     cone_defaults(static_cast<cvr_cone_params*>(p));
   } else if (technique == CVR_CONE_NN) {
@@ -195,6 +199,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
   if (c->d_dos_rows) (void)hipFree(c->d_dos_rows);
   if (c->d_gt_rays) (void)hipFree(c->d_gt_rays);
+  if (c->d_lc) (void)hipFree(c->d_lc);
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
                   static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
     if (p) (void)hipFree(p);
@@ -417,6 +422,17 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
     const int rc = check_dos_params(*static_cast<const cvr_dos_params*>(params));
     if (rc) return rc;
   }
+  if (h->technique == CVR_EBS || h->technique == CVR_DOS) {
+    const int* r = h->technique == CVR_EBS ? static_cast<const cvr_ebs_params*>(params)->light_cache_res
+                                           : static_cast<const cvr_dos_params*>(params)->light_cache_res;
+    const int on = h->technique == CVR_EBS ? static_cast<const cvr_ebs_params*>(params)->light_cache
+                                           : static_cast<const cvr_dos_params*>(params)->light_cache;
+    for (int i = 0; i < 3 && on; ++i)
+      if (r[i] < 1 || r[i] > 512) {
+        set_error("cvr_set_params: light_cache_res must be in [1, 512] per axis");
+        return CVR_ERR_INVALID;
+      }
+  }
   if (h->technique == CVR_CONE_TRACE || h->technique == CVR_CONE_NN) {
     const cvr_cone_params& cp = h->technique == CVR_CONE_TRACE ? *static_cast<const cvr_cone_params*>(params)
                                                                 : static_cast<const cvr_nn_params*>(params)->cone;
@@ -544,6 +560,19 @@ int cvr_get_sat(cvr_handle h, double* out_host, size_t count) {
   std::vector<float2> tmp(cells);
   CVR_HIP_CHECK(hipMemcpy(tmp.data(), h->d_sat, cells * sizeof(float2), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < cells; ++i) out_host[i] = double(tmp[i].x) + double(tmp[i].y);
+  return CVR_OK;
+}
+
+int cvr_get_light_cache(cvr_handle h, float* out_host, size_t count, int* res) {
+  if (!h || !res) { set_error("cvr_get_light_cache: null argument"); return CVR_ERR_INVALID; }
+  if (!h->lc_valid) { set_error("cvr_get_light_cache: no light cache rendered yet"); return CVR_ERR_STATE; }
+  for (int i = 0; i < 3; ++i) res[i] = h->lc_res[i];
+  const size_t n = size_t(h->lc_res[0]) * h->lc_res[1] * h->lc_res[2] * 2;
+  if (!out_host) return CVR_OK;
+  if (count < n) { set_error("cvr_get_light_cache: output too small"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  CVR_HIP_CHECK(hipDeviceSynchronize());
+  CVR_HIP_CHECK(hipMemcpy(out_host, h->d_lc, n * sizeof(float), hipMemcpyDeviceToHost));
   return CVR_OK;
 }
 

@@ -84,6 +84,12 @@ struct cvr_ctx {
   bool nn_trained = false;
   cvr_nn_report nn_report{};
 
+  // object-space light cache (EBS, DOS): (occlusion, shadow) per cell, x fastest
+  float2* d_lc = nullptr;
+  size_t lc_cap = 0;
+  int lc_res[3] = {0, 0, 0};
+  bool lc_valid = false;
+
   // stats
   bool collect_stats = false;
   unsigned long long* d_counters = nullptr;
@@ -104,6 +110,10 @@ float volume_diagonal(const cvr_ctx* c);
 // kernels (launchers return a CVR status)
 int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st);
 int build_sat(cvr_ctx* c, hipStream_t st);
+// light cache: validates the resolution and sizes the buffer; the march over it
+int lc_prepare(cvr_ctx* c, const int res[3]);
+int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sdw, float ka, float kd, float step,
+                     float et, float* out, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int build_volume_quad(cvr_ctx* c, hipStream_t st);
 int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st);

@@ -185,6 +185,31 @@ __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrG
   }
 }
 
+// Object-space light cache (rc1pdosct/lightcachecomputation.comp:523-546): cell i
+// holds (occlusion, shadow) at texture-space position (i + 1/2) * cell, 1 for a term
+// that is off.  The occlusion cone turns towards the eye in the frame
+// right = normalize(cross(-k, camera up)), up = normalize(cross(k, right)) (:279-291);
+// the shadow cone is the renderer's own.  One lane per cell.
+template <int F>
+__global__ __launch_bounds__(256) void k_lc_dos(PyrGeom pg, const void* __restrict__ pyr, DosK K, f3 cam_up, int rx,
+                                                int ry, int rz, f3 cell, f3 half, float2* __restrict__ lc) {
+  const size_t n = size_t(rx) * ry * rz;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = int(i % size_t(rx)), y = int((i / size_t(rx)) % size_t(ry)), z = int(i / (size_t(rx) * ry));
+  const f3 tp = mk3((float(x) + 0.5f) * cell.x, (float(y) + 0.5f) * cell.y, (float(z) + 0.5f) * cell.z);
+  const f3 wp = tp - half;
+  float Io = 1.f, Is = 1.f;
+  if (K.apply_occ) {
+    const f3 k = norm3(ld3(K.eye) - wp);
+    const f3 vr = norm3(cross3(mk3(-k.x, -k.y, -k.z), cam_up));
+    const f3 vu = norm3(cross3(k, vr));
+    Io = dos_cone<F>(pyr, pg, K.occ, K.occ_rows, tp, k, vu, vr);
+  }
+  if (K.apply_sdw) Is = dos_shadow<F>(pyr, pg, K, tp, wp);
+  lc[i] = make_float2(Io, Is);
+}
+
 // ------------------------------------------------------------------ host: section tables
 namespace {
 const double kPi = 3.14159265358979323846;
@@ -417,6 +442,25 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 64 * 1024) { set_error("dos renderer: transfer function larger than 4096 entries"); return CVR_ERR_UNSUPPORTED; }
+  if (P.light_cache) {
+    // PreComputeLightCache (dosrcrenderer.cpp:555-651), then the object-space march
+    if ((rc = lc_prepare(c, P.light_cache_res))) return rc;
+    const int rx = c->lc_res[0], ry = c->lc_res[1], rz = c->lc_res[2];
+    const int dims[3] = {c->nx, c->ny, c->nz}, res[3] = {rx, ry, rz};
+    const float sc[3] = {c->sx, c->sy, c->sz};
+    float cs[3];
+    for (int i = 0; i < 3; ++i) cs[i] = sc[i] * (float(dims[i]) / float(res[i]));  // VolumeScales * (dims / res)
+    const size_t n = size_t(rx) * ry * rz;
+    const f3 cam_up{rf.u[0], rf.u[1], rf.u[2]};
+    const f3 half{0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]};
+    const f3 cell{cs[0], cs[1], cs[2]};
+    with_fmt(c->pyr.fmt, [&](auto f) {
+      hipLaunchKernelGGL((k_lc_dos<decltype(f)::value>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, c->pyr,
+                         c->d_pyr_quad, K, cam_up, rx, ry, rz, cell, half, c->d_lc);
+    });
+    CVR_HIP_CHECK(hipGetLastError());
+    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, out, st);
+  }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   float4* o4 = reinterpret_cast<float4*>(out);
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {

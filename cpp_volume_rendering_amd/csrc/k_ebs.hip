@@ -235,21 +235,12 @@ __device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k,
   return __expf(-float(st));
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
-                                              const float2* __restrict__ sat, float step, float et, float4* __restrict__ out,
-                                              unsigned long long* __restrict__ counters) {
-  extern __shared__ float4 smem[];
-  __shared__ int s_nrows;
-  float4* s_tf = smem;
-  ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem + ntf);
-  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
-  if (threadIdx.x == 0) s_nrows = k.nrows;
-  __syncthreads();
+// Directional light: the cone chain is the same for every sample.  The workgroup builds
+// it in FP64 (the decisions in ceil() and the distance test then match a double
+// evaluation of :190-430 exactly) into LDS; row r is the r-th box.  *s_nrows enters as
+// k.nrows (set and synchronised by the caller) and leaves as the chain length.
+__device__ void ebs_build_rows(const EbsK& k, ShadowRow* s_rows, int* s_nrows) {
   if (k.apply_sdw && k.shadow_type == 1) {
-    // Directional light: the cone chain is the same for every sample.  Each block
-    // builds it in FP64 (the decisions in ceil() and the distance test then match a
-    // double evaluation of :190-430 exactly) into LDS; row r is the r-th box.
     double cv[3] = {double(k.light_fwd[0]), double(k.light_fwd[1]), double(k.light_fwd[2])};
     nrm(cv);
     const double scd[3] = {double(k.scale[0]), double(k.scale[1]), double(k.scale[2])};
@@ -259,7 +250,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
       double apos = cg.apos0;
       for (int i = 0; i < r; ++i) apos = apos + cg.interval;
       if (!((apos / cv[cg.axis]) < double(k.sdw_max_dist))) {
-        atomicMin(&s_nrows, r);
+        atomicMin(s_nrows, r);
         continue;
       }
       double o1[3], o2[3];
@@ -277,6 +268,20 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
+                                              const float2* __restrict__ sat, float step, float et, float4* __restrict__ out,
+                                              unsigned long long* __restrict__ counters) {
+  extern __shared__ float4 smem[];
+  __shared__ int s_nrows;
+  float4* s_tf = smem;
+  ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem + ntf);
+  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+  if (threadIdx.x == 0) s_nrows = k.nrows;
+  __syncthreads();
+  ebs_build_rows(k, s_rows, &s_nrows);
   const int nrows = s_nrows;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles_x = (rf.W + 15) >> 4;
@@ -335,6 +340,31 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
   }
 }
 
+// Object-space light cache (rc1pextbsd/lightcachecomputation.comp:440-468): cell i holds
+// (AO, shadow) at texture-space position (i + 1/2) * cell, 1 for a term that is off,
+// from the renderer's own AO and shadow evaluations.  One lane per cell.
+__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const float2* __restrict__ sat, int rx, int ry, int rz, f3 cell,
+                                                float2* __restrict__ lc) {
+  extern __shared__ float4 smem[];
+  __shared__ int s_nrows;
+  ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem);
+  if (threadIdx.x == 0) s_nrows = k.nrows;
+  __syncthreads();
+  ebs_build_rows(k, s_rows, &s_nrows);
+  const int nrows = s_nrows;
+  const size_t n = size_t(rx) * ry * rz;
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int x = int(i % size_t(rx)), y = int((i / size_t(rx)) % size_t(ry)), z = int(i / (size_t(rx) * ry));
+  const f3 tp = mk3((float(x) + 0.5f) * cell.x, (float(y) + 0.5f) * cell.y, (float(z) + 0.5f) * cell.z);
+  unsigned int nbox = 0;
+  float Io = 1.f, Is = 1.f;
+  if (k.apply_occ) Io = ebs_ao(sat, k, tp);
+  if (k.apply_sdw)
+    Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, tp, nbox) : ebs_shadow_point(sat, k, tp, nbox);
+  lc[i] = make_float2(Io, Is);
+}
+
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
   const cvr_ebs_params& P = c->ebs;
   if (P.apply_occlusion == 0 && P.apply_shadow == 0) {
@@ -363,7 +393,6 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     k.maxVol[i] = G - s * 0.5f;
     // texel = p / (G + 2s) * (n+2) - 1/2  (GL_LINEAR over the SAT texture)
     k.sat_tex_mul[i] = double(sdim[i]) / (double(G) + 2.0 * double(s));
-    (void)dims;
   }
   k.apply_occ = P.apply_occlusion; k.apply_sdw = P.apply_shadow;
   k.shells = P.ao_shells; k.shadow_type = P.shadow_type;
@@ -394,6 +423,20 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     k.nrows = int(n);
   }
   const float step = P.step_size > 0 ? P.step_size : default_step(c);
+  if (P.light_cache) {
+    // PreComputeLightCache (ebsrenderer.cpp:441-520), then the object-space march
+    if ((rc = lc_prepare(c, P.light_cache_res))) return rc;
+    const int rx = c->lc_res[0], ry = c->lc_res[1], rz = c->lc_res[2];
+    const int res[3] = {rx, ry, rz};
+    float cs[3];
+    for (int i = 0; i < 3; ++i) cs[i] = k.scale[i] * (float(dims[i]) / float(res[i]));  // VolumeScales * (dims / res)
+    const f3 cell{cs[0], cs[1], cs[2]};
+    const size_t n = size_t(rx) * ry * rz;
+    hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 255) / 256)), dim3(256), size_t(k.nrows) * sizeof(ShadowRow), st, k,
+                       c->d_sat, rx, ry, rz, cell, c->d_lc);
+    CVR_HIP_CHECK(hipGetLastError());
+    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, out, st);
+  }
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4) + size_t(k.nrows) * sizeof(ShadowRow);
   if (shm > 150 * 1024) {

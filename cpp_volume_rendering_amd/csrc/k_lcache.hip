@@ -1,0 +1,142 @@
+// Object-space light cache of the EBS and DOS renderers (gfx950): the march that reads
+// it and its buffer.  The cache kernels themselves live with their techniques
+// (k_ebs.hip: k_lc_ebs, k_dos.hip: k_lc_dos).
+//
+// PreIlluminationStructuredVolume (cppvolrend/utils/preillumination.cpp:7-80) keeps a
+// 2-channel (occlusion, shadow) volume, 32^3 by default, rebuilt every Update() when
+// active (ebsrenderer.cpp:127-134, dosrcrenderer.cpp:134-143).  The frame is then marched
+// by _common_shaders/obj_ray_marching.comp:223-333: every opaque sample reads the cache
+// with GL_LINEAR / CLAMP_TO_EDGE at tex_pos / VolumeScaledSizes and shades
+// rgb <- (rgb*Iocc*ka + rgb*Isdw*kd) / (ka + kd) (non-Phong branch, :180-220).  Here the
+// cache is fp32 (the reference stores RG16F), laid out x fastest as float2; at the
+// default size it is 256 KiB and stays in L2, so its 8 taps per sample are cache hits.
+#include "cvr_ctx.hpp"
+
+namespace cvr {
+
+constexpr int kMaxLightCacheRes = 512;
+
+// GL_LINEAR of the cache at texel coordinates p * tex_mul - 1/2 (clamped to the edge).
+__device__ __forceinline__ float2 lc_sample(const float2* __restrict__ lc, int rx, int ry, int rz, const float tm[3],
+                                            f3 p) {
+  int x0, x1, y0, y1, z0, z1;
+  float fx, fy, fz;
+  lin_axis(p.x * tm[0] - 0.5f, rx, x0, x1, fx);
+  lin_axis(p.y * tm[1] - 0.5f, ry, y0, y1, fy);
+  lin_axis(p.z * tm[2] - 0.5f, rz, z0, z1, fz);
+  const size_t sy = size_t(rx), sz = size_t(rx) * size_t(ry);
+  const size_t b00 = size_t(y0) * sy + size_t(z0) * sz, b10 = size_t(y1) * sy + size_t(z0) * sz;
+  const size_t b01 = size_t(y0) * sy + size_t(z1) * sz, b11 = size_t(y1) * sy + size_t(z1) * sz;
+  const float2 c000 = lc[b00 + x0], c100 = lc[b00 + x1], c010 = lc[b10 + x0], c110 = lc[b10 + x1];
+  const float2 c001 = lc[b01 + x0], c101 = lc[b01 + x1], c011 = lc[b11 + x0], c111 = lc[b11 + x1];
+  auto lerp2 = [](float2 a, float2 b, float t) { return make_float2(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y)); };
+  const float2 c0 = lerp2(lerp2(c000, c100, fx), lerp2(c010, c110, fx), fy);
+  const float2 c1 = lerp2(lerp2(c001, c101, fx), lerp2(c011, c111, fx), fy);
+  return lerp2(c0, c1, fz);
+}
+
+struct LcK {
+  int rx, ry, rz;
+  float tex_mul[3];  // cache texels per world unit: res / G
+  float ka, kd;      // 0 for a term that is off
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lc_render(RayFrame rf, VolGeom g, LcK L, const float2* __restrict__ lc,
+                                                   const float4* __restrict__ tf, int ntf, float step, float et,
+                                                   float4* __restrict__ out, unsigned long long* __restrict__ counters) {
+  extern __shared__ float4 s_tf[];
+  for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
+  __syncthreads();
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tiles_x = (rf.W + 15) >> 4;
+  const int tx = bid % tiles_x, ty = bid / tiles_x;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
+  const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);
+  if (px >= rf.W || py >= rf.H) return;
+  const f3 dir = camera_ray(rf, px, py);
+  const f3 eye = ld3(rf.eye);
+  float tn, tf_;
+  float4 dst = make_float4(0.f, 0.f, 0.f, 0.f);
+  unsigned int nsamp = 0, nshade = 0;
+  const float inv_k = 1.0f / (L.ka + L.kd);
+  if (ray_box(eye, dir, g.G, tn, tf_)) {
+    const float D = fabsf(tf_ - tn);
+    const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
+    for (int j = 0; j < kMaxSamples; ++j) {
+      const float s = float(j) * step;
+      if (!(s < D)) break;
+      const float h = fminf(step, D - s);
+      const f3 p = tp + dir * (s + 0.5f * h);
+      float4 src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
+      ++nsamp;
+      if (src.w > 0.0f) {
+        ++nshade;
+        const float2 I = lc_sample(lc, L.rx, L.ry, L.rz, L.tex_mul, p);
+        const float m = inv_k * (I.x * L.ka + I.y * L.kd);
+        src.x *= m;
+        src.y *= m;
+        src.z *= m;
+        const float a = 1.0f - __expf(-src.w * h);
+        const float wgt = (1.0f - dst.w) * a;
+        dst.x += wgt * src.x;
+        dst.y += wgt * src.y;
+        dst.z += wgt * src.z;
+        dst.w += wgt;
+        if (dst.w > et) break;
+      }
+    }
+  }
+  out[size_t(py) * rf.W + px] = dst;
+  if (counters) {
+    atomicAdd(&counters[0], (unsigned long long)nsamp);
+    atomicAdd(&counters[1], (unsigned long long)nshade);
+  }
+}
+
+int lc_prepare(cvr_ctx* c, const int res[3]) {
+  for (int i = 0; i < 3; ++i)
+    if (res[i] < 1 || res[i] > kMaxLightCacheRes) {
+      set_error("light cache: light_cache_res must be in [1, 512] per axis");
+      return CVR_ERR_INVALID;
+    }
+  const size_t n = size_t(res[0]) * size_t(res[1]) * size_t(res[2]);
+  if (n > c->lc_cap) {
+    if (c->d_lc) CVR_HIP_CHECK(hipFree(c->d_lc));
+    c->d_lc = nullptr;
+    c->lc_cap = 0;
+    CVR_HIP_CHECK(hipMalloc(&c->d_lc, n * sizeof(float2)));
+    c->lc_cap = n;
+  }
+  for (int i = 0; i < 3; ++i) c->lc_res[i] = res[i];
+  c->lc_valid = false;
+  return CVR_OK;
+}
+
+int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sdw, float ka, float kd, float step,
+                     float et, float* out, hipStream_t st) {
+  LcK L{};
+  const VolGeom g = geom_of(c);
+  L.rx = c->lc_res[0];
+  L.ry = c->lc_res[1];
+  L.rz = c->lc_res[2];
+  const int res[3] = {L.rx, L.ry, L.rz};
+  for (int i = 0; i < 3; ++i) L.tex_mul[i] = float(double(res[i]) / double(g.G[i]));
+  L.ka = apply_occ ? ka : 0.f;
+  L.kd = apply_sdw ? kd : 0.f;
+  const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
+  const size_t shm = size_t(c->tf_size) * sizeof(float4);
+  unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
+  if (c->dtype == CVR_U8)
+    hipLaunchKernelGGL(k_lc_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
+                       c->tf_size, step, et, reinterpret_cast<float4*>(out), cnt);
+  else
+    hipLaunchKernelGGL(k_lc_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
+                       c->tf_size, step, et, reinterpret_cast<float4*>(out), cnt);
+  CVR_HIP_CHECK(hipGetLastError());
+  c->lc_valid = true;
+  return CVR_OK;
+}
+
+}  // namespace cvr

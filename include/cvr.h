@@ -93,6 +93,13 @@ typedef struct cvr_ebs_params {
   float sdw_max_dist;    /* DirSdwConeMaxDistance; <= 0: 0.75 * volume diagonal */
   int shadow_type;       /* TypeOfShadow: 0 point light (position), 1 directional (forward) */
   float ka, kd;          /* Kambient, Kdiffuse: 0.5, 0.5 */
+  /* PreIlluminationStructuredVolume (preillumination.cpp:7-80, ebsrenderer.cpp:46-47,
+   * 127-134): nonzero renders through an object-space light cache of
+   * light_cache_res cells (occlusion, shadow per cell, rebuilt every render,
+   * lightcachecomputation.comp:440-468) read trilinearly by the march
+   * (_common_shaders/obj_ray_marching.comp:223-333).  Off, 32^3. */
+  int light_cache;
+  int light_cache_res[3];
 } cvr_ebs_params;
 
 /* RC1PConeLightGroundTruthSteps parameters (crtgtrenderer.cpp:23-56, 106-113).  Each
@@ -145,6 +152,11 @@ typedef struct cvr_dos_params {
   cvr_dos_cone sdw;      /* 0.5 deg, packing 0, weight 1.0, covered 0.75 * diagonal */
   float ka, kd;          /* 0.5, 0.5 */
   int density_precision; /* pyramid storage read by the cones: 0 fp32, 1 fp16, 2 bf16 (0) */
+  /* Object-space light cache as for CVR_EBS (dosrcrenderer.cpp:64, 134-143, 555-651,
+   * rc1pdosct/lightcachecomputation.comp:523-546): the occlusion cone of a cell turns
+   * towards the eye in a frame built from the camera's up vector.  Off, 32^3. */
+  int light_cache;
+  int light_cache_res[3];
 } cvr_dos_params;
 
 /* Cone-traced lighting over the Gaussian extinction pyramid.  The pyramid follows
@@ -244,6 +256,11 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
  * ebsrenderer.cpp:624-723 + summedareatable.h:218-278), copied to the host as
  * double (hi + lo of the two-float storage).  Builds it if needed. */
 int cvr_get_sat(cvr_handle h, double* out_host, size_t count);
+
+/* The light cache of the last CVR_EBS / CVR_DOS render with light_cache on: res[3]
+ * receives its size, out_host 2 floats (occlusion, shadow) per cell, x fastest
+ * (count = floats available).  Replaces reading back the RG16F cache texture. */
+int cvr_get_light_cache(cvr_handle h, float* out_host, size_t count, int* res);
 
 /* Counters of the last render (samples marched, shaded samples). */
 typedef struct cvr_stats {

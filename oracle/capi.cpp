@@ -4,6 +4,7 @@
 #include "oracle.hpp"
 #include "oracle_cone.hpp"
 #include "oracle_dos.hpp"
+#include "oracle_lc.hpp"
 #include <omp.h>
 
 using namespace orc;
@@ -401,6 +402,51 @@ void orc_render_gt_f32(int nx, int ny, int nz, int dtype, const void* data, doub
   threads(nthreads);
   render_gt<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), gt_from(c), mkcam(cam), W, H,
                    step, et, out, nullptr, stride);
+}
+
+// Light caches (2 doubles per cell, (occlusion, shadow), x fastest) and the march that
+// reads them (cache given as 2 doubles per cell).
+void orc_lc_ebs(int nx, int ny, int nz, double sx, double sy, double sz, const double* sat, const EbsC* prm, int rx,
+                int ry, int rz, double* out, int nthreads) {
+  threads(nthreads);
+  LightCacheO lc;
+  lc_ebs(mkvol(nx, ny, nz, 0, nullptr, sx, sy, sz), sat, ebs_from(prm), rx, ry, rz, lc);
+  for (size_t i = 0; i < lc.occ.size(); ++i) { out[2 * i] = lc.occ[i]; out[2 * i + 1] = lc.sdw[i]; }
+}
+int orc_lc_dos(int nx, int ny, int nz, double sx, double sy, double sz, void* pyr, const DosC* c, double sigma0,
+               const float* cam, int W, int H, int rx, int ry, int rz, double* out, int nthreads) {
+  threads(nthreads);
+  LightCacheO lc;
+  bool ok = false;
+  lc_dos(mkvol(nx, ny, nz, 0, nullptr, sx, sy, sz), *static_cast<Pyramid*>(pyr), dos_from(c), sigma0, mkcam(cam), W, H,
+         rx, ry, rz, lc, &ok);
+  if (!ok) return -1;
+  for (size_t i = 0; i < lc.occ.size(); ++i) { out[2 * i] = lc.occ[i]; out[2 * i + 1] = lc.sdw[i]; }
+  return 0;
+}
+static LightCacheO lc_from(const double* cache, int rx, int ry, int rz) {
+  LightCacheO lc;
+  lc.rx = rx; lc.ry = ry; lc.rz = rz;
+  const size_t n = size_t(rx) * ry * rz;
+  lc.occ.resize(n);
+  lc.sdw.resize(n);
+  for (size_t i = 0; i < n; ++i) { lc.occ[i] = cache[2 * i]; lc.sdw[i] = cache[2 * i + 1]; }
+  return lc;
+}
+void orc_render_lc_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                       const double* cache, int rx, int ry, int rz, int apply_occ, int apply_sdw, double ka, double kd,
+                       const float* cam, int W, int H, double step, double et, double* out, double* margin,
+                       int nthreads, int stride) {
+  threads(nthreads);
+  render_lc<double>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), lc_from(cache, rx, ry, rz),
+                    apply_occ, apply_sdw, ka, kd, mkcam(cam), W, H, step, et, out, margin, stride);
+}
+void orc_render_lc_f32(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                       const double* cache, int rx, int ry, int rz, int apply_occ, int apply_sdw, double ka, double kd,
+                       const float* cam, int W, int H, float step, float et, float* out, int nthreads, int stride) {
+  threads(nthreads);
+  render_lc<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), lc_from(cache, rx, ry, rz),
+                   apply_occ, apply_sdw, ka, kd, mkcam(cam), W, H, step, et, out, nullptr, stride);
 }
 
 int orc_mlp_nparams() { return kNParams; }

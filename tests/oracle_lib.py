@@ -74,6 +74,11 @@ def lib():
         "orc_gt_ray": (d, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, d, d]),
         "orc_render_gt_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
         "orc_render_gt_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, i, i, f, f, vp, i, i]),
+        "orc_lc_ebs": (None, [i, i, i, d, d, d, vp, vp, i, i, i, vp, i]),
+        "orc_lc_dos": (i, [i, i, i, d, d, d, vp, vp, d, vp, i, i, i, i, i, vp, i]),
+        "orc_render_lc_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, i, i, d, d, vp, vp, i,
+                                     i]),
+        "orc_render_lc_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, i, i, f, f, vp, i, i]),
         "orc_mlp_nparams": (i, []),
         "orc_mlp_forward": (None, [vp, i, vp, vp]),
         "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
@@ -496,4 +501,47 @@ def render_gt(vol, tf: OTF, prm: GtC, cam, W, H, step=0.5, et=0.99, scale=(1, 1,
     margin = np.zeros((H, W, 2), np.float64)
     lib().orc_render_gt_f64(*va, tf.h, C.byref(prm), c.ctypes.data, W, H, step, et, out.ctypes.data,
                             margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+# ---------------------------------------------------------------- light caches
+def lc_ebs(shape, scale, sat_arr, prm: EbsC, res, threads=0):
+    """EBS light cache at resolution res = (rx, ry, rz): (rz, ry, rx, 2) of (occlusion, shadow)."""
+    nx, ny, nz = shape
+    rx, ry, rz = res
+    s64 = np.ascontiguousarray(sat_arr, dtype=np.float64)
+    out = np.zeros((rz, ry, rx, 2))
+    lib().orc_lc_ebs(nx, ny, nz, *map(float, scale), s64.ctypes.data, C.byref(prm), rx, ry, rz, out.ctypes.data,
+                     threads)
+    return out
+
+
+def lc_dos(shape, scale, pyr: "OPyr", prm: "DosC", cam, W, H, res, sigma0=1.0, threads=0):
+    """DOS light cache for the camera `cam` (its eye and up vector)."""
+    nx, ny, nz = shape
+    rx, ry, rz = res
+    out = np.zeros((rz, ry, rx, 2))
+    c = cam_array(cam)
+    if lib().orc_lc_dos(nx, ny, nz, *map(float, scale), pyr.h, C.byref(prm), sigma0, c.ctypes.data, W, H, rx, ry, rz,
+                        out.ctypes.data, threads) != 0:
+        raise ValueError("cone sections do not converge")
+    return out
+
+
+def render_lc(vol, tf: OTF, cache, occ, sdw, ka, kd, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1), fp32=False,
+              threads=0, stride=1):
+    """The object-space march over a light cache (rz, ry, rx, 2)."""
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    cache = np.ascontiguousarray(cache, dtype=np.float64)
+    rz, ry, rx, _ = cache.shape
+    if fp32:
+        out = np.zeros((H, W, 4), np.float32)
+        lib().orc_render_lc_f32(*va, tf.h, cache.ctypes.data, rx, ry, rz, int(occ), int(sdw), float(ka), float(kd),
+                                c.ctypes.data, W, H, step, et, out.ctypes.data, threads, stride)
+        return out
+    out = np.zeros((H, W, 4), np.float64)
+    margin = np.zeros((H, W, 2), np.float64)
+    lib().orc_render_lc_f64(*va, tf.h, cache.ctypes.data, rx, ry, rz, int(occ), int(sdw), float(ka), float(kd),
+                            c.ctypes.data, W, H, step, et, out.ctypes.data, margin.ctypes.data, threads, stride)
     return out, margin
