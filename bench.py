@@ -31,6 +31,8 @@ WORKLOAD_METRIC = {
     "ebs": "frames/s 256³ u16@1024² extinction-based shading (SAT AO + directional cone shadow); HBM GB/s",
     "cone": "frames/s 256³ u8@1024² cone-traced DAO+shadow (8+1 cones x 64 steps, 6-level pyramid); HBM GB/s",
     "dos": "frames/s 256³ u8@1024² section-table directional occlusion + point-light shadow; HBM GB/s",
+    "ebs-lc": "frames/s 256³ u16@1024² extinction-based shading through a 32³ object-space light cache; HBM GB/s",
+    "dos-lc": "frames/s 256³ u8@1024² directional occlusion + shadow through a 32³ object-space light cache; HBM GB/s",
 }
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
@@ -129,7 +131,26 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
                 break
         return {"value": frames / t, "unit": "frames/s", "cores": threads, "kind": "port",
                 "sample": f"{frames} full {W_}x{H_} frames, oracle FP32, OpenMP over rows"}
-    if kind == "ebs":
+    if kind in ("ebs-lc", "dos-lc"):
+        # the cache (FP64 oracle) is part of every frame; the march over it in FP32
+        if kind == "ebs-lc":
+            sat64 = O.sat(wl.volume, tf)  # preprocessing, not part of a frame
+            prm = O.ebs_params(None, light, shadow_type=1)
+            prm.sdw_max_dist = np.float32(0.75) * np.float32(np.linalg.norm(wl.grid))
+            t0 = time.perf_counter()
+            cache = O.lc_ebs(wl.volume.shape[::-1], wl.scale, sat64, prm, (32, 32, 32), threads=threads)
+            t_cache = time.perf_counter() - t0
+        else:
+            t0 = time.perf_counter()
+            p = params["native"]
+            cache = O.lc_dos(wl.volume.shape[::-1], wl.scale, params["pyramid"], O.dos_params(p, wl.grid, light), cam,
+                             W_, H_, (32, 32, 32), sigma0=p.sigma0, threads=threads)
+            t_cache = time.perf_counter() - t0
+        what = f"FP64 oracle light cache ({t_cache:.2f} s per frame) + FP32 march over it"
+
+        def run(stride):
+            O.render_lc(wl.volume, tf, cache, 1, 1, 0.5, 0.5, cam, W_, H_, fp32=True, threads=threads, stride=stride)
+    elif kind == "ebs":
         sat = O.sat(wl.volume, tf).astype(np.float32)
         prm = O.ebs_params(None, light, shadow_type=1)
         prm.sdw_max_dist = np.float32(0.75) * np.float32(np.linalg.norm(wl.grid))
@@ -165,7 +186,10 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
         t0 = time.perf_counter()
         run(stride)
         t = time.perf_counter() - t0
-    return {"value": 1.0 / (t * stride * stride), "unit": "frames/s", "cores": threads, "kind": "port",
+    frame_s = t * stride * stride
+    if kind in ("ebs-lc", "dos-lc"):
+        frame_s += t_cache
+    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"one pixel in {stride} along x and along y of one {W_}x{H_} frame (1/{stride * stride} of it, "
                       f"{t:.1f} s), {what}, OpenMP over rows; scaled to whole frames"}
 
@@ -230,7 +254,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nn", choices=["nn", "cone", "rc1p", "ebs", "dos", "batch"])
+    ap.add_argument("--workload", default="nn",
+                    choices=["nn", "cone", "rc1p", "ebs", "dos", "ebs-lc", "dos-lc", "batch"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=0, help="override the volume size (tests)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -256,10 +281,10 @@ def main():
     if args.workload == "rc1p":
         wl = W.plain_blobs(n=args.size or 128, width=512, height=512, seed=1)
         tech, dtype, bpv = N.RC1P, "f32", 1
-    elif args.workload == "ebs":
+    elif args.workload in ("ebs", "ebs-lc"):
         wl = W.banded_noise(n=args.size or 256, width=1024, height=1024, seed=2)
         tech, dtype, bpv = N.EBS, "f32", 2
-    elif args.workload == "dos":
+    elif args.workload in ("dos", "dos-lc"):
         wl = W.cone_blobs(n=args.size or 256, width=1024, height=1024, seed=3)
         tech, dtype, bpv = N.DOS, "f32", 1
     elif args.workload == "cone":
@@ -273,10 +298,11 @@ def main():
     r = N.Renderer(tech)
     r.set_volume(wl.volume, wl.scale)
     r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    light_cache = args.workload.endswith("-lc")
     if tech == N.EBS:
-        r.set_params(shadow_type=1)
+        r.set_params(shadow_type=1, light_cache=int(light_cache))
     if tech == N.DOS:
-        r.set_params(apply_shadow=1)
+        r.set_params(apply_shadow=1, light_cache=int(light_cache))
     t_prep = time.perf_counter()
     r.prepare(stream.cuda_stream)
     stream.synchronize()
@@ -322,11 +348,18 @@ def main():
         config["mlp_tflops"] = round(st.shaded_samples * MLP_FLOP_PER_SAMPLE / (ms * 1e-3) / 1e12, 3)
     # roofline of the dominant kernel (one launch per frame): the algorithmic bytes of its
     # gathers, every trilinear read counted as its 8 texels, plus the frame written once
-    gathered = gather_bytes(tech, st, r, bpv) + wl.width * wl.height * 16
+    if light_cache:
+        # the march over the cache: volume taps + 8 float2 cache taps per shaded sample
+        gathered = st.samples * 8 * bpv + st.shaded_samples * 8 * 8 + wl.width * wl.height * 16
+    else:
+        gathered = gather_bytes(tech, st, r, bpv) + wl.width * wl.height * 16
     achieved = gathered / (ms * 1e-3) / 1e9
     config["gathered_bytes_per_frame"] = int(gathered)
     kernel = {N.RC1P: "k_rc1p", N.EBS: "k_ebs", N.DOS: "k_dos_render", N.CONE_TRACE: "k_cone_render",
               N.CONE_NN: "k_nn_render"}[tech]
+    if light_cache:
+        kernel = "k_lc_render"
+        config["light_cache"] = "32x32x32 (occlusion, shadow) rebuilt every frame"
     config["kernel"] = kernel
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, wl.name, wl.width)}
@@ -338,8 +371,10 @@ def main():
     }
     if not args.no_cpu:
         kind = {N.RC1P: "rc1p", N.EBS: "ebs", N.DOS: "dos"}.get(tech, "cone")
+        if light_cache:
+            kind += "-lc"
         params = None
-        if kind in ("cone", "dos"):
+        if kind in ("cone", "dos", "dos-lc"):
             sys.path.insert(0, str(ROOT / "tests"))
             import oracle_lib as O
             levels = (r if tech in (N.CONE_TRACE, N.DOS) else ct).pyramid()
