@@ -108,6 +108,13 @@ static void defaults(int technique, void* p) {
     cvr_rc1p_params* r = static_cast<cvr_rc1p_params*>(p);
     r->step_size = 0.f;
     r->et_threshold = 0.99f;
+    r->apply_gradient_shading = 0;  // rc1prenderer.cpp:22
+    r->gradient_type = CVR_GRADIENT_SOBEL;
+    r->ka = 0.5f;  // renderingparameters.cpp:23-26
+    r->kd = 0.5f;
+    r->ks = 0.8f;
+    r->shininess = 30.0f;
+    r->specular[0] = r->specular[1] = r->specular[2] = 1.0f;  // lightsourcelist.cpp:24
   } else if (technique == CVR_EBS) {
     cvr_ebs_params* e = static_cast<cvr_ebs_params*>(p);
     e->step_size = 0.f;
@@ -126,6 +133,11 @@ static void defaults(int technique, void* p) {
     e->kd = 0.5f;
     e->light_cache = 0;  // ebsrenderer.cpp:46-47
     e->light_cache_res[0] = e->light_cache_res[1] = e->light_cache_res[2] = 32;
+    e->apply_phong = 0;  // ebsrenderer.cpp:22
+    e->gradient_type = CVR_GRADIENT_SOBEL;
+    e->ks = 0.8f;  // renderingparameters.cpp:25-26
+    e->shininess = 30.0f;
+    e->specular[0] = e->specular[1] = e->specular[2] = 1.0f;  // lightsourcelist.cpp:24
   } else if (technique == CVR_CONE_GT) {
     // crtgtrenderer.cpp:32-51 (both lighting terms on here: with both off the shading
     // divides by zero)
@@ -164,6 +176,11 @@ static void defaults(int technique, void* p) {
     d->density_precision = 0;
     d->light_cache = 0;  // dosrcrenderer.cpp:64
     d->light_cache_res[0] = d->light_cache_res[1] = d->light_cache_res[2] = 32;
+    d->apply_phong = 0;  // dosrcrenderer.cpp:33
+    d->gradient_type = CVR_GRADIENT_SOBEL;
+    d->ks = 0.8f;
+    d->shininess = 30.0f;
+    d->specular[0] = d->specular[1] = d->specular[2] = 1.0f;
   } else if (technique == CVR_CONE_TRACE) {
     cone_defaults(static_cast<cvr_cone_params*>(p));
   } else if (technique == CVR_CONE_NN) {
@@ -200,6 +217,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_dos_rows) (void)hipFree(c->d_dos_rows);
   if (c->d_gt_rays) (void)hipFree(c->d_gt_rays);
   if (c->d_lc) (void)hipFree(c->d_lc);
+  if (c->d_grad) (void)hipFree(c->d_grad);
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
                   static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
     if (p) (void)hipFree(p);
@@ -295,6 +313,8 @@ int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, fl
     h->d_vol = nullptr;
     if (h->d_sat) CVR_HIP_CHECK(hipFree(h->d_sat));
     h->d_sat = nullptr;
+    if (h->d_grad) CVR_HIP_CHECK(hipFree(h->d_grad));
+    h->d_grad = nullptr;
     if (hipMalloc(&h->d_vol, bytes) != hipSuccess) {
       h->d_vol = nullptr;
       set_error("cvr_set_volume: out of device memory");
@@ -422,6 +442,16 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
     const int rc = check_dos_params(*static_cast<const cvr_dos_params*>(params));
     if (rc) return rc;
   }
+  {
+    int gt = -1;
+    if (h->technique == CVR_RC1P) gt = static_cast<const cvr_rc1p_params*>(params)->gradient_type;
+    if (h->technique == CVR_EBS) gt = static_cast<const cvr_ebs_params*>(params)->gradient_type;
+    if (h->technique == CVR_DOS) gt = static_cast<const cvr_dos_params*>(params)->gradient_type;
+    if (gt != -1 && gt != CVR_GRADIENT_SOBEL && gt != CVR_GRADIENT_FINITE_DIFF) {
+      set_error("cvr_set_params: gradient_type must be CVR_GRADIENT_SOBEL or CVR_GRADIENT_FINITE_DIFF");
+      return CVR_ERR_INVALID;
+    }
+  }
   if (h->technique == CVR_EBS || h->technique == CVR_DOS) {
     const int* r = h->technique == CVR_EBS ? static_cast<const cvr_ebs_params*>(params)->light_cache_res
                                            : static_cast<const cvr_dos_params*>(params)->light_cache_res;
@@ -519,7 +549,7 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
   RayFrame rf = frame_of(cam, width, height);
   CVR_HIP_CHECK(hipEventRecord(h->ev0, st));
   switch (h->technique) {
-    case CVR_RC1P: rc = launch_rc1p(h, rf, dst, st); break;
+    case CVR_RC1P: rc = launch_rc1p(h, rf, light, dst, st); break;
     case CVR_EBS: rc = launch_ebs(h, rf, light, dst, st); break;
     case CVR_CONE_GT: rc = launch_gt(h, rf, light, dst, st); break;
     case CVR_DOS: rc = launch_dos(h, rf, light, dst, st); break;
@@ -560,6 +590,33 @@ int cvr_get_sat(cvr_handle h, double* out_host, size_t count) {
   std::vector<float2> tmp(cells);
   CVR_HIP_CHECK(hipMemcpy(tmp.data(), h->d_sat, cells * sizeof(float2), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < cells; ++i) out_host[i] = double(tmp[i].x) + double(tmp[i].y);
+  return CVR_OK;
+}
+
+int cvr_get_gradient(cvr_handle h, int type, float* out_host, size_t count) {
+  if (!h || !out_host) { set_error("cvr_get_gradient: null argument"); return CVR_ERR_INVALID; }
+  if (type != CVR_GRADIENT_SOBEL && type != CVR_GRADIENT_FINITE_DIFF) {
+    set_error("cvr_get_gradient: unknown gradient type");
+    return CVR_ERR_INVALID;
+  }
+  int rc = check_inputs(h);
+  if (rc) return rc;
+  const size_t n = size_t(h->nx) * h->ny * h->nz;
+  if (count < 3 * n) { set_error("cvr_get_gradient: output too small"); return CVR_ERR_INVALID; }
+  DeviceGuard dg(h->device);
+  if ((rc = build_gradient(h, type, nullptr))) return rc;
+  const int nbx = bricks(h->nx), nby = bricks(h->ny);
+  std::vector<float4> tmp(brick_cells(h->nx, h->ny, h->nz));
+  CVR_HIP_CHECK(hipMemcpy(tmp.data(), h->d_grad, tmp.size() * sizeof(float4), hipMemcpyDeviceToHost));
+  size_t o = 0;
+  for (int z = 0; z < h->nz; ++z)
+    for (int y = 0; y < h->ny; ++y)
+      for (int x = 0; x < h->nx; ++x, o += 3) {
+        const float4 g = tmp[brick_index(x, y, z, nbx, nby)];
+        out_host[o] = g.x;
+        out_host[o + 1] = g.y;
+        out_host[o + 2] = g.z;
+      }
   return CVR_OK;
 }
 

@@ -1,6 +1,7 @@
 // Technique instance state behind a cvr_handle.
 #pragma once
 #include "cvr_common.hpp"
+#include "cvr_phong.hpp"
 #include "../../include/cvr.h"
 #include <vector>
 
@@ -84,6 +85,10 @@ struct cvr_ctx {
   bool nn_trained = false;
   cvr_nn_report nn_report{};
 
+  // volume gradient (float4 per voxel, brick order like the neighbour blocks)
+  float4* d_grad = nullptr;
+  uint64_t grad_built_for = ~0ull;
+
   // object-space light cache (EBS, DOS): (occlusion, shadow) per cell, x fastest
   float2* d_lc = nullptr;
   size_t lc_cap = 0;
@@ -108,12 +113,17 @@ float default_step(const cvr_ctx* c);
 float volume_diagonal(const cvr_ctx* c);
 
 // kernels (launchers return a CVR status)
-int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st);
+int launch_rc1p(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
+// Blinn-Phong constants of a render (builds the gradient when shading is on)
+int phong_setup(cvr_ctx* c, int on, int gradient_type, float ks, float shininess, const float spec[3],
+                const RayFrame& rf, const cvr_light* light, PhongK& ph, hipStream_t st);
 int build_sat(cvr_ctx* c, hipStream_t st);
+// volume gradient of `type` for Blinn-Phong shading
+int build_gradient(cvr_ctx* c, int type, hipStream_t st);
 // light cache: validates the resolution and sizes the buffer; the march over it
 int lc_prepare(cvr_ctx* c, const int res[3]);
 int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sdw, float ka, float kd, float step,
-                     float et, float* out, hipStream_t st);
+                     float et, const PhongK& ph, float* out, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int build_volume_quad(cvr_ctx* c, hipStream_t st);
 int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st);

@@ -129,7 +129,8 @@ __device__ __forceinline__ float dos_shadow(const void* __restrict__ pyr, const 
 template <typename T, int F>
 __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                     DosK K, const float4* __restrict__ tf, int ntf, float step, float et,
-                                                    float4* __restrict__ out, unsigned long long* __restrict__ counters) {
+                                                    PhongK ph, const float4* __restrict__ grad, float4* __restrict__ out,
+                                                    unsigned long long* __restrict__ counters) {
   extern __shared__ float4 s_tf[];
   for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
   __syncthreads();
@@ -166,8 +167,19 @@ __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrG
         float Io = 0.f, Is = 0.f;
         if (K.apply_occ) Io = dos_cone<F>(pyr, pg, K.occ, K.occ_rows, p, norm3(eye - wp), up, right);
         if (K.apply_sdw) Is = dos_shadow<F>(pyr, pg, K, p, wp);
-        const float m = inv_k * (Io * ka + Is * kd);
-        src.x *= m; src.y *= m; src.z *= m;
+        if (ph.on) {
+          // Phong branch (:627-643): a sample whose gradient vanishes keeps its colour
+          float dd, sp;
+          if (phong_terms(grad, g, ph, p, eye, dd, sp)) {
+            const float m = inv_k * (Io * ka + Is * kd * dd), s = Is * (K.apply_sdw ? ph.ks : 0.f) * sp;
+            src.x = src.x * m + ph.spec[0] * s;
+            src.y = src.y * m + ph.spec[1] * s;
+            src.z = src.z * m + ph.spec[2] * s;
+          }
+        } else {
+          const float m = inv_k * (Io * ka + Is * kd);
+          src.x *= m; src.y *= m; src.z *= m;
+        }
         const float a = 1.0f - __expf(-src.w * h);
         const float wgt = (1.0f - dst.w) * a;
         dst.x += wgt * src.x;
@@ -442,6 +454,8 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 64 * 1024) { set_error("dos renderer: transfer function larger than 4096 entries"); return CVR_ERR_UNSUPPORTED; }
+  PhongK ph;
+  if ((rc = phong_setup(c, P.apply_phong, P.gradient_type, P.ks, P.shininess, P.specular, rf, light, ph, st))) return rc;
   if (P.light_cache) {
     // PreComputeLightCache (dosrcrenderer.cpp:555-651), then the object-space march
     if ((rc = lc_prepare(c, P.light_cache_res))) return rc;
@@ -459,14 +473,14 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
                          c->d_pyr_quad, K, cam_up, rx, ry, rz, cell, half, c->d_lc);
     });
     CVR_HIP_CHECK(hipGetLastError());
-    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, out, st);
+    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
   }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   float4* o4 = reinterpret_cast<float4*>(out);
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
     hipLaunchKernelGGL((k_dos_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
-                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, o4, cnt);
+                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, ph, c->d_grad, o4, cnt);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return CVR_OK;

@@ -272,7 +272,8 @@ __device__ void ebs_build_rows(const EbsK& k, ShadowRow* s_rows, int* s_nrows) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
-                                              const float2* __restrict__ sat, float step, float et, float4* __restrict__ out,
+                                              const float2* __restrict__ sat, float step, float et, PhongK ph,
+                                              const float4* __restrict__ grad, float4* __restrict__ out,
                                               unsigned long long* __restrict__ counters) {
   extern __shared__ float4 smem[];
   __shared__ int s_nrows;
@@ -318,10 +319,21 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
         }
         if (k.apply_sdw)
           Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, p, nbox) : ebs_shadow_point(sat, k, p, nbox);
-        // (1/(ka+kd)) * (rgb*Io*ka + rgb*Is*kd), :547
-        src.x = inv_k * (src.x * Io * ka + src.x * Is * kd);
-        src.y = inv_k * (src.y * Io * ka + src.y * Is * kd);
-        src.z = inv_k * (src.z * Io * ka + src.z * Is * kd);
+        if (ph.on) {
+          // Phong branch (:523-543): a sample whose gradient vanishes keeps its colour
+          float dd, sp;
+          if (phong_terms(grad, k.g, ph, p, eye, dd, sp)) {
+            const float s = Is * ((k.apply_sdw ? ph.ks : 0.f) * sp);
+            src.x = inv_k * (src.x * Io * ka + Is * (src.x * kd * dd)) + s * ph.spec[0];
+            src.y = inv_k * (src.y * Io * ka + Is * (src.y * kd * dd)) + s * ph.spec[1];
+            src.z = inv_k * (src.z * Io * ka + Is * (src.z * kd * dd)) + s * ph.spec[2];
+          }
+        } else {
+          // (1/(ka+kd)) * (rgb*Io*ka + rgb*Is*kd), :547
+          src.x = inv_k * (src.x * Io * ka + src.x * Is * kd);
+          src.y = inv_k * (src.y * Io * ka + src.y * Is * kd);
+          src.z = inv_k * (src.z * Io * ka + src.z * Is * kd);
+        }
         const float a = 1.0f - __expf(-src.w * h);
         const float wgt = (1.0f - dst.w) * a;
         dst.x += wgt * src.x;
@@ -423,6 +435,8 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     k.nrows = int(n);
   }
   const float step = P.step_size > 0 ? P.step_size : default_step(c);
+  PhongK ph;
+  if ((rc = phong_setup(c, P.apply_phong, P.gradient_type, P.ks, P.shininess, P.specular, rf, light, ph, st))) return rc;
   if (P.light_cache) {
     // PreComputeLightCache (ebsrenderer.cpp:441-520), then the object-space march
     if ((rc = lc_prepare(c, P.light_cache_res))) return rc;
@@ -435,7 +449,7 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 255) / 256)), dim3(256), size_t(k.nrows) * sizeof(ShadowRow), st, k,
                        c->d_sat, rx, ry, rz, cell, c->d_lc);
     CVR_HIP_CHECK(hipGetLastError());
-    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, out, st);
+    return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
   }
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4) + size_t(k.nrows) * sizeof(ShadowRow);
@@ -446,10 +460,10 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
     hipLaunchKernelGGL(k_ebs<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
-                       step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+                       step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   else
     hipLaunchKernelGGL(k_ebs<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
-                       step, P.et_threshold, reinterpret_cast<float4*>(out), cnt);
+                       step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }

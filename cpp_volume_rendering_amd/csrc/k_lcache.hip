@@ -7,7 +7,9 @@
 // active (ebsrenderer.cpp:127-134, dosrcrenderer.cpp:134-143).  The frame is then marched
 // by _common_shaders/obj_ray_marching.comp:223-333: every opaque sample reads the cache
 // with GL_LINEAR / CLAMP_TO_EDGE at tex_pos / VolumeScaledSizes and shades
-// rgb <- (rgb*Iocc*ka + rgb*Isdw*kd) / (ka + kd) (non-Phong branch, :180-220).  Here the
+// rgb <- (rgb*Iocc*ka + rgb*Isdw*kd) / (ka + kd), or with Phong shading on
+// (rgb*Iocc*ka + Isdw*rgb*kd*max(0, n.l)) / (ka + kd) + Isdw*ks*specular*max(0, h.n)^n
+// (:186-211; a sample whose gradient vanishes keeps its colour).  Here the
 // cache is fp32 (the reference stores RG16F), laid out x fastest as float2; at the
 // default size it is 256 KiB and stays in L2, so its 8 taps per sample are cache hits.
 #include "cvr_ctx.hpp"
@@ -39,12 +41,14 @@ struct LcK {
   int rx, ry, rz;
   float tex_mul[3];  // cache texels per world unit: res / G
   float ka, kd;      // 0 for a term that is off
+  float ks;          // Phong specular weight, 0 without shadows (Kspecular counts with ApplyShadow)
 };
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_lc_render(RayFrame rf, VolGeom g, LcK L, const float2* __restrict__ lc,
                                                    const float4* __restrict__ tf, int ntf, float step, float et,
-                                                   float4* __restrict__ out, unsigned long long* __restrict__ counters) {
+                                                   PhongK ph, const float4* __restrict__ grad, float4* __restrict__ out,
+                                                   unsigned long long* __restrict__ counters) {
   extern __shared__ float4 s_tf[];
   for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
   __syncthreads();
@@ -74,10 +78,20 @@ __global__ __launch_bounds__(256) void k_lc_render(RayFrame rf, VolGeom g, LcK L
       if (src.w > 0.0f) {
         ++nshade;
         const float2 I = lc_sample(lc, L.rx, L.ry, L.rz, L.tex_mul, p);
-        const float m = inv_k * (I.x * L.ka + I.y * L.kd);
-        src.x *= m;
-        src.y *= m;
-        src.z *= m;
+        if (ph.on) {
+          float dd, sp;
+          if (phong_terms(grad, g, ph, p, eye, dd, sp)) {
+            const float s = I.y * (L.ks * sp);
+            src.x = inv_k * (src.x * I.x * L.ka + I.y * (src.x * L.kd * dd)) + s * ph.spec[0];
+            src.y = inv_k * (src.y * I.x * L.ka + I.y * (src.y * L.kd * dd)) + s * ph.spec[1];
+            src.z = inv_k * (src.z * I.x * L.ka + I.y * (src.z * L.kd * dd)) + s * ph.spec[2];
+          }
+        } else {
+          const float m = inv_k * (I.x * L.ka + I.y * L.kd);
+          src.x *= m;
+          src.y *= m;
+          src.z *= m;
+        }
         const float a = 1.0f - __expf(-src.w * h);
         const float wgt = (1.0f - dst.w) * a;
         dst.x += wgt * src.x;
@@ -115,7 +129,7 @@ int lc_prepare(cvr_ctx* c, const int res[3]) {
 }
 
 int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sdw, float ka, float kd, float step,
-                     float et, float* out, hipStream_t st) {
+                     float et, const PhongK& ph, float* out, hipStream_t st) {
   LcK L{};
   const VolGeom g = geom_of(c);
   L.rx = c->lc_res[0];
@@ -125,15 +139,16 @@ int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sd
   for (int i = 0; i < 3; ++i) L.tex_mul[i] = float(double(res[i]) / double(g.G[i]));
   L.ka = apply_occ ? ka : 0.f;
   L.kd = apply_sdw ? kd : 0.f;
+  L.ks = apply_sdw ? ph.ks : 0.f;
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
     hipLaunchKernelGGL(k_lc_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
-                       c->tf_size, step, et, reinterpret_cast<float4*>(out), cnt);
+                       c->tf_size, step, et, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   else
     hipLaunchKernelGGL(k_lc_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
-                       c->tf_size, step, et, reinterpret_cast<float4*>(out), cnt);
+                       c->tf_size, step, et, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   c->lc_valid = true;
   return CVR_OK;

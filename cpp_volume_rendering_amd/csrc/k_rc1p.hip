@@ -5,6 +5,11 @@
 // D - s); the TF holds (r, g, b, extinction); alpha = 1 - exp(-ext*h); composite
 // front to back; stop once accumulated alpha exceeds the threshold.
 //
+// With gradient shading on, every opaque sample's colour goes through ShadeBlinnPhong
+// (:48-81) before it is composited: rgb*(ka + kd*max(0, n.l)) + specular*ks*
+// max(0, h.n)^shininess, n the normalised GL_LINEAR gradient (samples whose gradient is
+// the zero vector keep their colour).
+//
 // Mapping: one lane per pixel, a wavefront covers an 8x8 pixel tile, a 256-lane
 // workgroup a 16x16 tile; workgroups are re-ordered so each XCD marches one
 // contiguous band of tiles.  The 1D transfer function is staged in LDS.  The
@@ -16,7 +21,8 @@ namespace cvr {
 
 template <typename T, bool TF_LDS>
 __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const float4* __restrict__ tf, int ntf,
-                                               float step, float et, float4* __restrict__ out,
+                                               float step, float et, PhongK ph, const float4* __restrict__ grad,
+                                               float ka, float kd, float4* __restrict__ out,
                                                unsigned long long* __restrict__ counters) {
   extern __shared__ float4 s_tf[];
   if (TF_LDS) {
@@ -50,6 +56,13 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
       ++nsamp;
       if (src.w > 0.0f) {
         ++nshade;
+        float dd, sp;
+        if (ph.on && phong_terms(grad, g, ph, p, eye, dd, sp)) {
+          const float m = ka + kd * dd, s = ph.ks * sp;
+          src.x = src.x * m + ph.spec[0] * s;
+          src.y = src.y * m + ph.spec[1] * s;
+          src.z = src.z * m + ph.spec[2] * s;
+        }
         const float a = 1.0f - __expf(-src.w * h);
         const float wgt = (1.0f - dst.w) * a;
         dst.x += wgt * src.x;
@@ -67,28 +80,55 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
   }
 }
 
-int launch_rc1p(cvr_ctx* c, const RayFrame& rf, float* out, hipStream_t st) {
+int phong_setup(cvr_ctx* c, int on, int gradient_type, float ks, float shininess, const float spec[3],
+                const RayFrame& rf, const cvr_light* light, PhongK& ph, hipStream_t st) {
+  ph = PhongK{};
+  if (!on) return 0;
+  if (!(shininess > 0.f)) {
+    set_error("cvr_render: Blinn-Phong shininess must be > 0");
+    return CVR_ERR_INVALID;
+  }
+  if (const int rc = build_gradient(c, gradient_type, st)) return rc;
+  ph.on = 1;
+  ph.ks = ks;
+  ph.shininess = shininess;
+  for (int i = 0; i < 3; ++i) {
+    ph.spec[i] = spec[i];
+    // without a light: the camera's own light at the eye (renderingmanager.cpp:421-426)
+    ph.lpos[i] = light ? light->position[i] : rf.eye[i];
+  }
+  return 0;
+}
+
+int launch_rc1p(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
+  const cvr_rc1p_params& P = c->rc1p;
   VolGeom g = geom_of(c);
-  float step = c->rc1p.step_size > 0 ? c->rc1p.step_size : default_step(c);
-  float et = c->rc1p.et_threshold;
+  float step = P.step_size > 0 ? P.step_size : default_step(c);
+  float et = P.et_threshold;
+  PhongK ph;
+  if (const int rc = phong_setup(c, P.apply_gradient_shading, P.gradient_type, P.ks, P.shininess, P.specular, rf, light,
+                                 ph, st))
+    return rc;
+  const float4* gr = ph.on ? c->d_grad : nullptr;
   int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   bool lds = c->tf_size <= 4096;
   size_t shm = lds ? size_t(c->tf_size) * sizeof(float4) : 0;
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
+  float4* o4 = reinterpret_cast<float4*>(out);
   if (c->dtype == CVR_U8) {
     if (lds)
       hipLaunchKernelGGL((k_rc1p<uint8_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, reinterpret_cast<float4*>(out), cnt);
+                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
     else
       hipLaunchKernelGGL((k_rc1p<uint8_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, reinterpret_cast<float4*>(out), cnt);
+                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
   } else {
     if (lds)
       hipLaunchKernelGGL((k_rc1p<uint16_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, reinterpret_cast<float4*>(out), cnt);
+                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
     else
       hipLaunchKernelGGL((k_rc1p<uint16_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, reinterpret_cast<float4*>(out), cnt);
+                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
   }
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -35,8 +35,13 @@ class Light(C.Structure):
                 ("right", C.c_float * 3)]
 
 
+GRADIENT_SOBEL, GRADIENT_FINITE_DIFF = 0, 1
+
+
 class Rc1pParams(C.Structure):
-    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float)]
+    _fields_ = [("step_size", C.c_float), ("et_threshold", C.c_float), ("apply_gradient_shading", C.c_int),
+                ("gradient_type", C.c_int), ("ka", C.c_float), ("kd", C.c_float), ("ks", C.c_float),
+                ("shininess", C.c_float), ("specular", C.c_float * 3)]
 
 
 class EbsParams(C.Structure):
@@ -44,7 +49,9 @@ class EbsParams(C.Structure):
                 ("apply_shadow", C.c_int), ("ao_shells", C.c_int), ("ao_radius", C.c_float),
                 ("sdw_angle_deg", C.c_float), ("sdw_interval", C.c_float), ("sdw_initial", C.c_float),
                 ("sdw_weight", C.c_float), ("sdw_max_dist", C.c_float), ("shadow_type", C.c_int),
-                ("ka", C.c_float), ("kd", C.c_float), ("light_cache", C.c_int), ("light_cache_res", C.c_int * 3)]
+                ("ka", C.c_float), ("kd", C.c_float), ("light_cache", C.c_int), ("light_cache_res", C.c_int * 3),
+                ("apply_phong", C.c_int), ("gradient_type", C.c_int), ("ks", C.c_float), ("shininess", C.c_float),
+                ("specular", C.c_float * 3)]
 
 
 class GtParams(C.Structure):
@@ -66,7 +73,8 @@ class DosParams(C.Structure):
                 ("apply_shadow", C.c_int), ("shadow_type", C.c_int), ("spot_max_deg", C.c_float),
                 ("n_levels", C.c_int), ("base_res", C.c_int * 3), ("sigma0", C.c_float), ("occ", DosCone),
                 ("sdw", DosCone), ("ka", C.c_float), ("kd", C.c_float), ("density_precision", C.c_int),
-                ("light_cache", C.c_int), ("light_cache_res", C.c_int * 3)]
+                ("light_cache", C.c_int), ("light_cache_res", C.c_int * 3), ("apply_phong", C.c_int),
+                ("gradient_type", C.c_int), ("ks", C.c_float), ("shininess", C.c_float), ("specular", C.c_float * 3)]
 
 
 class ConeParams(C.Structure):
@@ -112,7 +120,7 @@ PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, C
 EXPORTS = [
     "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
     "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
-    "cvr_render", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_render", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
     "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load", "cvr_dos_sections", "cvr_gt_rays",
@@ -157,6 +165,7 @@ def lib() -> C.CDLL:
         "cvr_render": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, vp, i, vp]),
         "cvr_get_sat": (i, [vp, vp, sz]),
         "cvr_get_light_cache": (i, [vp, vp, sz, vp]),
+        "cvr_get_gradient": (i, [vp, i, vp, sz]),
         "cvr_get_stats": (i, [vp, C.POINTER(Stats)]),
         "cvr_set_collect_stats": (i, [vp, i]),
         "cvr_pyramid_info": (i, [vp, vp, vp, i]),
@@ -292,6 +301,13 @@ class Renderer:
         is_dev = not isinstance(out, np.ndarray)
         check(lib().cvr_render(self.h, C.byref(cam), C.byref(light) if light is not None else None, width, height,
                                _ptr(out), int(is_dev), stream))
+        return out
+
+    def gradient(self, shape, kind=GRADIENT_SOBEL) -> np.ndarray:
+        """The volume gradient used by Blinn-Phong shading: (nz, ny, nx, 3) float32."""
+        nx, ny, nz = shape
+        out = np.empty((nz, ny, nx, 3), dtype=np.float32)
+        check(lib().cvr_get_gradient(self.h, kind, out.ctypes.data, out.size))
         return out
 
     def light_cache(self) -> np.ndarray:

@@ -73,9 +73,25 @@ typedef struct cvr_light {
 } cvr_light;
 
 /* RayCasting1Pass parameters (rc1prenderer.cpp:18-23, 62-63). */
+/* Volume gradient for Blinn-Phong shading (DataManager::GenerateStructuredGradientTexture,
+ * datamanager.cpp:332-352): Sobel-Feldman (utils.cpp:287-350 and the compute-shader
+ * form _gradient_shading/sobelfeldman_generator.comp, unnormalised, 0 outside the
+ * volume) or central finite differences (utils.cpp:146-284 with its defaults: step 1,
+ * no filter, normalised, 0 where the difference vanishes). */
+enum { CVR_GRADIENT_SOBEL = 0, CVR_GRADIENT_FINITE_DIFF = 1 };
+
 typedef struct cvr_rc1p_params {
   float step_size;    /* <= 0: reference default 0.5/sqrt(3)*|voxel scale| */
   float et_threshold; /* early termination, ray_marching_1p.comp:238 (0.99) */
+  /* ShadeBlinnPhong (ray_marching_1p.comp:48-81): rgb*(ka + kd*max(0, n.l)) +
+   * specular*ks*max(0, h.n)^shininess where the gradient is nonzero; the light is
+   * cvr_light.position (RenderingParameters, renderingparameters.cpp:17-31: 0.5, 0.5,
+   * 0.8, 30; specular (1, 1, 1), lightsourcelist.cpp:24).  Off by default
+   * (rc1prenderer.cpp:22). */
+  int apply_gradient_shading;
+  int gradient_type;  /* CVR_GRADIENT_SOBEL or CVR_GRADIENT_FINITE_DIFF */
+  float ka, kd, ks, shininess;
+  float specular[3];
 } cvr_rc1p_params;
 
 /* RC1PExtinctionBasedShading parameters (ebsrenderer.cpp:19-53, 99-105, 125-247). */
@@ -100,6 +116,14 @@ typedef struct cvr_ebs_params {
    * (_common_shaders/obj_ray_marching.comp:223-333).  Off, 32^3. */
   int light_cache;
   int light_cache_res[3];
+  /* ApplyPhongShading (ebs_ray_bbox_marching.comp:523-546, ebsrenderer.cpp:22, 221):
+   * rgb <- (rgb*Iocc*ka + Isdw*rgb*kd*max(0, n.l)) / (ka+kd) + Isdw*ks*specular*
+   * max(0, h.n)^shininess where the gradient is nonzero (ks counts only with shadows
+   * on); the light is cvr_light.position.  Off; 0.8, 30, (1, 1, 1). */
+  int apply_phong;
+  int gradient_type;
+  float ks, shininess;
+  float specular[3];
 } cvr_ebs_params;
 
 /* RC1PConeLightGroundTruthSteps parameters (crtgtrenderer.cpp:23-56, 106-113).  Each
@@ -157,6 +181,11 @@ typedef struct cvr_dos_params {
    * towards the eye in a frame built from the camera's up vector.  Off, 32^3. */
   int light_cache;
   int light_cache_res[3];
+  /* ApplyPhongShading as for CVR_EBS (ray_bbox_marching.comp:627-650). */
+  int apply_phong;
+  int gradient_type;
+  float ks, shininess;
+  float specular[3];
 } cvr_dos_params;
 
 /* Cone-traced lighting over the Gaussian extinction pyramid.  The pyramid follows
@@ -256,6 +285,10 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
  * ebsrenderer.cpp:624-723 + summedareatable.h:218-278), copied to the host as
  * double (hi + lo of the two-float storage).  Builds it if needed. */
 int cvr_get_sat(cvr_handle h, double* out_host, size_t count);
+
+/* The volume gradient of `type` (built if needed), 3 floats per voxel, x fastest
+ * (count = floats available).  Replaces reading back the gradient texture. */
+int cvr_get_gradient(cvr_handle h, int type, float* out_host, size_t count);
 
 /* The light cache of the last CVR_EBS / CVR_DOS render with light_cache on: res[3]
  * receives its size, out_host 2 floats (occlusion, shadow) per cell, x fastest

@@ -5,6 +5,7 @@
 #include "oracle_cone.hpp"
 #include "oracle_dos.hpp"
 #include "oracle_lc.hpp"
+#include "oracle_grad.hpp"
 #include <omp.h>
 
 using namespace orc;
@@ -447,6 +448,76 @@ void orc_render_lc_f32(int nx, int ny, int nz, int dtype, const void* data, doub
   threads(nthreads);
   render_lc<float>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), lc_from(cache, rx, ry, rz),
                    apply_occ, apply_sdw, ka, kd, mkcam(cam), W, H, step, et, out, nullptr, stride);
+}
+
+// Gradients (3 doubles per voxel, x fastest) and the Blinn-Phong renders: the gradient
+// of `type` (0 Sobel-Feldman, 1 finite differences) is built from the volume itself.
+struct PhongC {
+  int type;
+  double ka, kd, ks, shininess;
+  double spec[3], lpos[3];
+};
+static void grad_of(const Volume& v, int type, GradientO& g) {
+  if (type == 0) gradient_sobel(v, g);
+  else gradient_fd(v, g);
+}
+static PhongO phong_from(const PhongC* c, const GradientO* g) {
+  PhongO p;
+  p.g = g;
+  p.ka = c->ka; p.kd = c->kd; p.ks = c->ks; p.shininess = c->shininess;
+  for (int i = 0; i < 3; ++i) { p.spec[i] = c->spec[i]; p.lpos[i] = c->lpos[i]; }
+  return p;
+}
+void orc_gradient(int nx, int ny, int nz, int dtype, const void* data, int type, double* out, int nthreads) {
+  threads(nthreads);
+  GradientO g;
+  grad_of(mkvol(nx, ny, nz, dtype, data, 1, 1, 1), type, g);
+  for (size_t i = 0; i < g.gx.size(); ++i) { out[3 * i] = g.gx[i]; out[3 * i + 1] = g.gy[i]; out[3 * i + 2] = g.gz[i]; }
+}
+void orc_render_rc1p_phong_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                               void* tf, const PhongC* ph, const float* cam, int W, int H, double step, double et,
+                               double* out, double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  const Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  GradientO g;
+  grad_of(v, ph->type, g);
+  render_rc1p_phong<double>(v, *static_cast<TF*>(tf), phong_from(ph, &g), mkcam(cam), W, H, step, et, out, margin,
+                            stride);
+}
+void orc_render_ebs_phong_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                              void* tf, const double* sat, const EbsC* prm, const PhongC* ph, const float* cam, int W,
+                              int H, double step, double et, double* out, double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  const Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  GradientO g;
+  grad_of(v, ph->type, g);
+  const PhongO P = phong_from(ph, &g);
+  render_ebs<double, double>(v, *static_cast<TF*>(tf), sat, ebs_from(prm), mkcam(cam), W, H, step, et, out, margin,
+                             stride, &P);
+}
+void orc_render_dos_phong_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                              void* tf, void* pyr, const DosC* c, double sigma0, const PhongC* ph, const float* cam,
+                              int W, int H, double step, double et, double* out, double* margin, int nthreads,
+                              int stride) {
+  threads(nthreads);
+  const Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  GradientO g;
+  grad_of(v, ph->type, g);
+  const PhongO P = phong_from(ph, &g);
+  render_dos<double>(v, *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr), dos_from(c), sigma0, mkcam(cam), W, H, step,
+                     et, out, margin, stride, &P);
+}
+void orc_render_lc_phong_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz,
+                             void* tf, const double* cache, int rx, int ry, int rz, int apply_occ, int apply_sdw,
+                             double ka, double kd, const PhongC* ph, const float* cam, int W, int H, double step,
+                             double et, double* out, double* margin, int nthreads, int stride) {
+  threads(nthreads);
+  const Volume v = mkvol(nx, ny, nz, dtype, data, sx, sy, sz);
+  GradientO g;
+  grad_of(v, ph->type, g);
+  const PhongO P = phong_from(ph, &g);
+  render_lc<double>(v, *static_cast<TF*>(tf), lc_from(cache, rx, ry, rz), apply_occ, apply_sdw, ka, kd, mkcam(cam), W,
+                    H, step, et, out, margin, stride, &P);
 }
 
 int orc_mlp_nparams() { return kNParams; }

@@ -302,6 +302,62 @@ inline void march(const V3<R>& tex_pos, const V3<R>& dir, R D, R step, R et, con
   }
 }
 
+// ---------------------------------------------------------------- Blinn-Phong
+// Gradient-based shading: ShadeBlinnPhong (rc1pass/ray_marching_1p.comp:48-81) and the
+// Phong branches of ebs_ray_bbox_marching.comp:523-546, rc1pdosct/ray_bbox_marching.comp:
+// 627-650 and _common_shaders/obj_ray_marching.comp:186-211.  The gradient texture is
+// sampled with GL_LINEAR / CLAMP_TO_EDGE at tex_pos / G; a zero vector means the sample
+// keeps its colour.
+struct GradientO {
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<double> gx, gy, gz;  // x fastest
+};
+
+struct PhongO {
+  const GradientO* g = nullptr;
+  double ka = 0.5, kd = 0.5, ks = 0.8, shininess = 30;  // renderingparameters.cpp:23-26
+  double spec[3] = {1, 1, 1};                            // lightsourcelist.cpp:24
+  double lpos[3] = {0, 0, 0};
+  double eye[3] = {0, 0, 0};
+};
+
+// max(0, n.l) and max(0, h.n)^shininess at texture-space p; false where the GL_LINEAR
+// gradient is the zero vector.
+template <class R>
+inline bool phong_terms(const PhongO& P, const V3<R>& G, const V3<R>& p, R& dd, R& sp) {
+  const V3<R> uvw(p.x / G.x, p.y / G.y, p.z / G.z);
+  V3<R> n(tex3d_grid<R, double>(P.g->gx.data(), P.g->nx, P.g->ny, P.g->nz, uvw),
+          tex3d_grid<R, double>(P.g->gy.data(), P.g->nx, P.g->ny, P.g->nz, uvw),
+          tex3d_grid<R, double>(P.g->gz.data(), P.g->nx, P.g->ny, P.g->nz, uvw));
+  if (n.x == 0 && n.y == 0 && n.z == 0) return false;
+  n = normalize(n);
+  const V3<R> wp = p - G * R(0.5);
+  const V3<R> L = normalize(V3<R>(R(P.lpos[0]), R(P.lpos[1]), R(P.lpos[2])) - wp);
+  const V3<R> E = normalize(V3<R>(R(P.eye[0]), R(P.eye[1]), R(P.eye[2])) - wp);
+  const V3<R> H = normalize(E + L);
+  dd = std::max(R(0), dot(n, L));
+  sp = std::pow(std::max(R(0), dot(H, n)), R(P.shininess));
+  return true;
+}
+
+// ShadeBlinnPhong (ray_marching_1p.comp:48-81)
+template <class R> inline void shade_blinn_phong(const PhongO& P, const V3<R>& G, const V3<R>& p, R* rgb) {
+  R dd, sp;
+  if (!phong_terms(P, G, p, dd, sp)) return;
+  for (int c = 0; c < 3; ++c) rgb[c] = rgb[c] * (R(P.ka) + R(P.kd) * dd) + R(P.spec[c]) * R(P.ks) * sp;
+}
+
+template <class R>
+inline void shade_lit_phong(const PhongO& P, const V3<R>& G, const V3<R>& p, R Io, R Is, R ka, R kd, R ks, R* rgb) {
+  R dd, sp;
+  if (!phong_terms(P, G, p, dd, sp)) return;
+  for (int c = 0; c < 3; ++c)
+    rgb[c] = (R(1) / (ka + kd)) * (rgb[c] * Io * ka + Is * (rgb[c] * kd * dd)) + Is * (ks * R(P.spec[c]) * sp);
+}
+
+// The Phong branch of a lit marcher: occlusion Io, shadow Is, effective weights ka, kd,
+// ks (0 for a term that is off); the DOS shader writes the same quantity as
+// rgb*((Io*ka + Is*kd*dd)/(ka+kd)) + spec*Is*ks*sp.
 template <class R>
 inline void render_rc1p(const Volume& vol, const TF& tf, const CamIn& cam, int W, int H, R step, R et, R* out,
                         double* margin, int stride = 1) {
@@ -545,10 +601,12 @@ template <class R, class S> struct Ebs {
 // ebs_ray_bbox_marching.comp:553-627
 template <class R, class S>
 inline void render_ebs(const Volume& vol, const TF& tf, const S* sat, const EbsParams& prm, const CamIn& cam, int W,
-                       int H, R step, R et, R* out, double* margin, int stride = 1) {
+                       int H, R step, R et, R* out, double* margin, int stride = 1, const PhongO* ph = nullptr) {
   Ebs<R, S> ebs(sat, vol, prm);
   CamBasis<R> cb = cam_basis<R>(cam, W, H);
   V3<R> G = ebs.G;
+  PhongO P = ph ? *ph : PhongO{};
+  for (int i = 0; i < 3; ++i) P.eye[i] = cam.eye[i];
 #pragma omp parallel for schedule(dynamic, 1)
   for (int py = 0; py < H; ++py)
     for (int px = 0; px < W; ++px) {
@@ -561,7 +619,14 @@ inline void render_ebs(const Volume& vol, const TF& tf, const S* sat, const EbsP
       if (ray_aabb(cb.eye, d, G, tn, tf_)) {
         R D = std::fabs(tf_ - tn);
         V3<R> tp = cb.eye + d * tn + G * R(0.5);
-        auto sh = [&](const V3<R>& p, R* src) { ebs.shade(p, src, &mg); };
+        auto sh = [&](const V3<R>& p, R* src) {
+          if (!ph) return ebs.shade(p, src, &mg);
+          // Phong branch (:523-546)
+          R ka = 0, kd = 0, ks = 0, Io = 0, Is = 0;
+          if (prm.apply_occlusion) { ka = R(prm.ka); Io = ebs.ambient_occlusion(p); }
+          if (prm.apply_shadow) { kd = R(prm.kd); ks = R(P.ks); Is = ebs.shadows(p, &mg); }
+          shade_lit_phong(P, G, p, Io, Is, ka, kd, ks, src);
+        };
         march(tp, d, D, step, et, G, vol, tf.rgbt.data(), tf.size(), sh, o, &mg);
       }
       if (margin) { margin[(size_t(py) * W + px) * 2] = mg.a; margin[(size_t(py) * W + px) * 2 + 1] = mg.g; }

@@ -242,7 +242,10 @@ template <class R> struct DosTracer {
 
 template <class R>
 inline void render_dos(const Volume& vol, const TF& tf, const Pyramid& P, const DosParamsO& dp, double sigma0,
-                       const CamIn& cam, int W, int H, R step, R et, R* out, double* margin, int stride = 1) {
+                       const CamIn& cam, int W, int H, R step, R et, R* out, double* margin, int stride = 1,
+                       const PhongO* ph = nullptr) {
+  PhongO Ph = ph ? *ph : PhongO{};
+  for (int i = 0; i < 3; ++i) Ph.eye[i] = cam.eye[i];
   DosParamsO q = dp;
   for (int i = 0; i < 3; ++i) q.eye[i] = cam.eye[i];
   DosTracer<R> tr(P, q, sigma0);
@@ -266,6 +269,7 @@ inline void render_dos(const Volume& vol, const TF& tf, const Pyramid& P, const 
         auto sh = [&](const V3<R>& p, R* src) {
           R Io = q.apply_occ ? tr.occlusion(p, up, right) : R(0);
           R Is = q.apply_sdw ? tr.shadow(p, &mg) : R(0);
+          if (ph) return shade_lit_phong(Ph, G, p, Io, Is, ka, kd, q.apply_sdw ? R(Ph.ks) : R(0), src);  // :627-650
           for (int c = 0; c < 3; ++c) src[c] = (R(1) / (ka + kd)) * (src[c] * Io * ka + src[c] * Is * kd);
         };
         march(tp, d, D, step, et, G, vol, tf.rgbt.data(), tf.size(), sh, o, &mg);

@@ -79,7 +79,10 @@ inline void lc_dos(const Volume& v, const Pyramid& P, const DosParamsO& dp, doub
 // obj_ray_marching.comp:223-333 with the cache read by GL_LINEAR at tex_pos / G.
 template <class R>
 inline void render_lc(const Volume& vol, const TF& tf, const LightCacheO& lc, int apply_occ, int apply_sdw, double ka_,
-                      double kd_, const CamIn& cam, int W, int H, R step, R et, R* out, double* margin, int stride = 1) {
+                      double kd_, const CamIn& cam, int W, int H, R step, R et, R* out, double* margin, int stride = 1,
+                      const PhongO* ph = nullptr) {
+  PhongO Ph = ph ? *ph : PhongO{};
+  for (int i = 0; i < 3; ++i) Ph.eye[i] = cam.eye[i];
   CamBasis<R> cb = cam_basis<R>(cam, W, H);
   V3<R> G(R(vol.nx * vol.sx), R(vol.ny * vol.sy), R(vol.nz * vol.sz));
   V3<R> invG(R(1) / G.x, R(1) / G.y, R(1) / G.z);
@@ -100,6 +103,7 @@ inline void render_lc(const Volume& vol, const TF& tf, const LightCacheO& lc, in
           const V3<R> uvw = p * invG;
           const R Io = tex3d_grid<R, double>(lc.occ.data(), lc.rx, lc.ry, lc.rz, uvw);
           const R Is = tex3d_grid<R, double>(lc.sdw.data(), lc.rx, lc.ry, lc.rz, uvw);
+          if (ph) return shade_lit_phong(Ph, G, p, Io, Is, ka, kd, apply_sdw ? R(Ph.ks) : R(0), src);  // :186-211
           for (int c = 0; c < 3; ++c) src[c] = (R(1) / (ka + kd)) * (src[c] * Io * ka + src[c] * Is * kd);
         };
         march(tp, d, D, step, et, G, vol, tf.rgbt.data(), tf.size(), sh, o, &mg);

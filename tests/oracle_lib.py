@@ -79,6 +79,12 @@ def lib():
         "orc_render_lc_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, i, i, d, d, vp, vp, i,
                                      i]),
         "orc_render_lc_f32": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, i, i, f, f, vp, i, i]),
+        "orc_gradient": (None, [i, i, i, i, vp, i, vp, i]),
+        "orc_render_rc1p_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_ebs_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_dos_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, vp, vp, i, i, d, d, vp, vp, i, i]),
+        "orc_render_lc_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, vp, i, i, d, d, vp,
+                                           vp, i, i]),
         "orc_mlp_nparams": (i, []),
         "orc_mlp_forward": (None, [vp, i, vp, vp]),
         "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
@@ -544,4 +550,76 @@ def render_lc(vol, tf: OTF, cache, occ, sdw, ka, kd, cam, W, H, step=0.5, et=0.9
     margin = np.zeros((H, W, 2), np.float64)
     lib().orc_render_lc_f64(*va, tf.h, cache.ctypes.data, rx, ry, rz, int(occ), int(sdw), float(ka), float(kd),
                             c.ctypes.data, W, H, step, et, out.ctypes.data, margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+# ---------------------------------------------------------------- gradients, Blinn-Phong
+class PhongC(C.Structure):
+    _fields_ = [("type", C.c_int), ("ka", C.c_double), ("kd", C.c_double), ("ks", C.c_double),
+                ("shininess", C.c_double), ("spec", C.c_double * 3), ("lpos", C.c_double * 3)]
+
+
+def phong(kind=0, light=None, ka=0.5, kd=0.5, ks=0.8, shininess=30.0, spec=(1.0, 1.0, 1.0), lpos=None) -> PhongC:
+    """Blinn-Phong constants (reference defaults); the light position from `light` or `lpos`."""
+    p = PhongC()
+    p.type, p.ka, p.kd, p.ks, p.shininess = kind, _f32(ka), _f32(kd), _f32(ks), _f32(shininess)
+    p.spec[:] = [_f32(x) for x in spec]
+    pos = lpos if lpos is not None else (light.position if light is not None else (0.0, 0.0, 0.0))
+    p.lpos[:] = [float(np.float32(x)) for x in pos]
+    return p
+
+
+def gradient(vol, kind=0, threads=0):
+    """The volume gradient (nz, ny, nx, 3): kind 0 Sobel-Feldman, 1 finite differences."""
+    vol = np.ascontiguousarray(vol)
+    nz, ny, nx = vol.shape
+    out = np.zeros((nz, ny, nx, 3))
+    lib().orc_gradient(nx, ny, nz, 0 if vol.dtype == np.uint8 else 1, vol.ctypes.data, kind, out.ctypes.data, threads)
+    return out
+
+
+def _phong_out(W, H):
+    return np.zeros((H, W, 4), np.float64), np.zeros((H, W, 2), np.float64)
+
+
+def render_rc1p_phong(vol, tf: OTF, ph: PhongC, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1), threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    out, margin = _phong_out(W, H)
+    lib().orc_render_rc1p_phong_f64(*va, tf.h, C.byref(ph), c.ctypes.data, W, H, step, et, out.ctypes.data,
+                                    margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+def render_ebs_phong(vol, tf: OTF, sat_arr, prm: EbsC, ph: PhongC, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1),
+                     threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    s64 = np.ascontiguousarray(sat_arr, dtype=np.float64)
+    out, margin = _phong_out(W, H)
+    lib().orc_render_ebs_phong_f64(*va, tf.h, s64.ctypes.data, C.byref(prm), C.byref(ph), c.ctypes.data, W, H, step,
+                                   et, out.ctypes.data, margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+def render_dos_phong(vol, tf: OTF, pyr, prm, ph: PhongC, cam, W, H, sigma0=1.0, step=0.5, et=0.99, scale=(1, 1, 1),
+                     threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    out, margin = _phong_out(W, H)
+    lib().orc_render_dos_phong_f64(*va, tf.h, pyr.h, C.byref(prm), sigma0, C.byref(ph), c.ctypes.data, W, H, step, et,
+                                   out.ctypes.data, margin.ctypes.data, threads, stride)
+    return out, margin
+
+
+def render_lc_phong(vol, tf: OTF, cache, occ, sdw, ka, kd, ph: PhongC, cam, W, H, step=0.5, et=0.99, scale=(1, 1, 1),
+                    threads=0, stride=1):
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    cache = np.ascontiguousarray(cache, dtype=np.float64)
+    rz, ry, rx, _ = cache.shape
+    out, margin = _phong_out(W, H)
+    lib().orc_render_lc_phong_f64(*va, tf.h, cache.ctypes.data, rx, ry, rz, int(occ), int(sdw), float(ka), float(kd),
+                                  C.byref(ph), c.ctypes.data, W, H, step, et, out.ctypes.data, margin.ctypes.data,
+                                  threads, stride)
     return out, margin
