@@ -36,6 +36,9 @@ class Light(C.Structure):
 
 
 GRADIENT_SOBEL, GRADIENT_FINITE_DIFF = 0, 1
+(FILTER_BOX, FILTER_HAT, FILTER_CATMULL_ROM, FILTER_MITCHELL_NETRAVALI, FILTER_CARDINAL_BSPLINE3,
+ FILTER_CARDINAL_OMOMS3) = range(6)
+SCALE_MULTISAMPLE, SCALE_DOWNSCALE, SCALE_UPSCALE = 0, 1, 2
 
 
 class Rc1pParams(C.Structure):
@@ -121,6 +124,7 @@ EXPORTS = [
     "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
     "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
     "cvr_render", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_multiscale_size", "cvr_resample_frame",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
     "cvr_nn_checkpoint_size", "cvr_nn_save", "cvr_nn_load", "cvr_dos_sections", "cvr_gt_rays",
@@ -166,6 +170,8 @@ def lib() -> C.CDLL:
         "cvr_get_sat": (i, [vp, vp, sz]),
         "cvr_get_light_cache": (i, [vp, vp, sz, vp]),
         "cvr_get_gradient": (i, [vp, i, vp, sz]),
+        "cvr_multiscale_size": (i, [i, i, i, i, vp, vp]),
+        "cvr_resample_frame": (i, [i, vp, i, i, vp, i, i, i, i, vp, vp]),
         "cvr_get_stats": (i, [vp, C.POINTER(Stats)]),
         "cvr_set_collect_stats": (i, [vp, i]),
         "cvr_pyramid_info": (i, [vp, vp, vp, i]),
@@ -479,3 +485,19 @@ class Batch:
         s = Stats()
         check(lib().cvr_batch_get_stats(self.h, C.byref(s)))
         return s
+
+
+def multiscale_size(screen_w: int, screen_h: int, mw: int, mh: int):
+    """Frame size for a screen and multipliers (m > 0 multiplies, m < 0 divides)."""
+    w, h = C.c_int(), C.c_int()
+    check(lib().cvr_multiscale_size(screen_w, screen_h, mw, mh, C.byref(w), C.byref(h)))
+    return w.value, h.value
+
+
+def resample_frame(src, dst, mode=SCALE_DOWNSCALE, kernel=FILTER_HAT, work=None, stream=None, device=0):
+    """Resample the device frame `src` (torch (sh, sw, 4) float32) into `dst` ((dh, dw, 4))."""
+    sh, sw = int(src.shape[0]), int(src.shape[1])
+    dh, dw = int(dst.shape[0]), int(dst.shape[1])
+    check(lib().cvr_resample_frame(device, _ptr(src), sw, sh, _ptr(dst), dw, dh, mode, kernel,
+                                   None if work is None else _ptr(work), stream))
+    return dst

@@ -295,6 +295,32 @@ int cvr_get_gradient(cvr_handle h, int type, float* out_host, size_t count);
  * (count = floats available).  Replaces reading back the RG16F cache texture. */
 int cvr_get_light_cache(cvr_handle h, float* out_host, size_t count, int* res);
 
+/* Multi-scale output (RenderFrameToScreen, libs/vis_utils/renderoutputframe.cpp:96-535):
+ * a frame rendered at a multiple (or fraction) of the screen size resampled to it.
+ * Kernels follow vis_utils/filters/utils.hpp:8-15 (IMAGE_FILTER_KERNEL, default hat). */
+enum {
+  CVR_FILTER_BOX = 0,
+  CVR_FILTER_HAT = 1,
+  CVR_FILTER_CATMULL_ROM = 2,
+  CVR_FILTER_MITCHELL_NETRAVALI = 3,
+  CVR_FILTER_CARDINAL_BSPLINE3 = 4,
+  CVR_FILTER_CARDINAL_OMOMS3 = 5
+};
+enum {
+  CVR_SCALE_MULTISAMPLE = 0, /* DrawMultiSampleHigherResolutionMode: GL_LINEAR at screen pixel centres */
+  CVR_SCALE_DOWNSCALE = 1,   /* DrawHigherResolutionWithDownScale: kernel filter (+ digital filter) */
+  CVR_SCALE_UPSCALE = 2      /* DrawLowerResolutionWithUpScale: (digital filter +) kernel interpolation */
+};
+/* The frame size for screen (w, h) and multipliers (mw, mh): m > 0 multiplies, m < 0
+ * divides (UpdateScreenResolutionMultiScaling, renderoutputframe.cpp:96-112). */
+int cvr_multiscale_size(int screen_w, int screen_h, int mw, int mh, int* frame_w, int* frame_h);
+/* Resamples the RGBA float frame src (sw x sh, device) into dst (dw x dh, device) on
+ * `device` / hip_stream.  UPSCALE with a cardinal kernel filters a copy of src in
+ * `work` (sw*sh*4 floats, device; work == src filters src in place as the reference
+ * does); otherwise work may be NULL. */
+int cvr_resample_frame(int device, const float* src, int sw, int sh, float* dst, int dw, int dh, int mode, int kernel,
+                       float* work, void* hip_stream);
+
 /* Counters of the last render (samples marched, shaded samples). */
 typedef struct cvr_stats {
   uint64_t samples;          /* volume samples taken along camera rays */

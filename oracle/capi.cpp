@@ -6,6 +6,7 @@
 #include "oracle_dos.hpp"
 #include "oracle_lc.hpp"
 #include "oracle_grad.hpp"
+#include "oracle_scale.hpp"
 #include <omp.h>
 
 using namespace orc;
@@ -518,6 +519,16 @@ void orc_render_lc_phong_f64(int nx, int ny, int nz, int dtype, const void* data
   const PhongO P = phong_from(ph, &g);
   render_lc<double>(v, *static_cast<TF*>(tf), lc_from(cache, rx, ry, rz), apply_occ, apply_sdw, ka, kd, mkcam(cam), W,
                     H, step, et, out, margin, stride, &P);
+}
+
+// Multi-scale output: src (sh x sw x 4) resampled into dst (dh x dw x 4).
+void orc_resample(const double* src, int sw, int sh, double* dst, int dw, int dh, int mode, int kernel) {
+  Frame s, d;
+  s.w = sw; s.h = sh;
+  s.p.assign(src, src + size_t(sw) * sh * 4);
+  d.w = dw; d.h = dh;
+  resample(s, d, mode, kernel);
+  std::copy(d.p.begin(), d.p.end(), dst);
 }
 
 int orc_mlp_nparams() { return kNParams; }

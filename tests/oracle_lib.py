@@ -85,6 +85,7 @@ def lib():
         "orc_render_dos_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, d, vp, vp, i, i, d, d, vp, vp, i, i]),
         "orc_render_lc_phong_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, i, i, i, i, i, d, d, vp, vp, i, i, d, d, vp,
                                            vp, i, i]),
+        "orc_resample": (None, [vp, i, i, vp, i, i, i, i]),
         "orc_mlp_nparams": (i, []),
         "orc_mlp_forward": (None, [vp, i, vp, vp]),
         "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
@@ -623,3 +624,13 @@ def render_lc_phong(vol, tf: OTF, cache, occ, sdw, ka, kd, ph: PhongC, cam, W, H
                                   C.byref(ph), c.ctypes.data, W, H, step, et, out.ctypes.data, margin.ctypes.data,
                                   threads, stride)
     return out, margin
+
+
+# ---------------------------------------------------------------- multi-scale output
+def resample(frame, dw, dh, mode, kernel):
+    """An (H, W, 4) frame resampled to (dh, dw, 4): mode 0 multisample, 1 downscale, 2 upscale."""
+    src = np.ascontiguousarray(frame, dtype=np.float64)
+    sh, sw, _ = src.shape
+    out = np.zeros((dh, dw, 4))
+    lib().orc_resample(src.ctypes.data, sw, sh, out.ctypes.data, dw, dh, mode, kernel)
+    return out
