@@ -112,11 +112,10 @@ __device__ __forceinline__ void lin_axis(float x, int n, int& i0, int& i1, float
 // neighbour, edge-clamped): first cell i in [0, n-1] and weight f.  Below the first
 // texel centre both taps clamp to cell 0, so the weight is zeroed there.
 __device__ __forceinline__ void quad_axis(float x, int n, int& i, float& f) {
-  const float fl = floorf(x);
-  f = x - fl;
-  i = int(fl);
-  if (i < 0) { i = 0; f = 0.f; }
-  if (i > n - 1) i = n - 1;
+  const float xc = fmaxf(x, 0.f);  // below the first texel centre: cell 0 with weight 0
+  const float fl = floorf(xc);
+  f = xc - fl;
+  i = min(int(fl), n - 1);
 }
 
 // Trilinear from two quad loads (planes z0 and z1): c = (v00, v10, v01, v11) over (x, y).

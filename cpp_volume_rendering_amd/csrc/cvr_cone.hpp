@@ -61,37 +61,41 @@ struct ConeK {
   const FeatTap* ft;        // surrogate inputs: axial/ring pairs [2*kNOcc], shadow [kNSdw], local [1]
 };
 
-// One neighbour block of the pyramid in its storage format: fp32 (float4), or four
-// 16-bit values (fp16 or bf16) in 8 bytes.
+// One corner block of the pyramid in its storage format: the values at (x..x+1, y..y+1,
+// z..z+1) of a level, the +1 neighbours clamped to the edge; fp32 as two float4 (planes
+// z and z+1, each (x, y), (x+1, y), (x, y+1), (x+1, y+1)), fp16 / bf16 as eight 16-bit
+// values in 16 bytes.  A trilinear read is one record: one cache line, one address.
+template <int F> struct PyrOct { typedef uint4 type; };
+template <> struct PyrOct<0> { struct type { float4 a, b; }; };
+struct Oct8 { float4 a, b; };
+
 template <int F>
-__device__ __forceinline__ float4 pyr_block(const void* __restrict__ pyr, size_t i) {
-  if (F == 0) return static_cast<const float4*>(pyr)[i];
-  const uint2 r = static_cast<const uint2*>(pyr)[i];
-  if (F == 1)
-    return make_float4(__half2float(__ushort_as_half(ushort(r.x & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.x >> 16))),
-                       __half2float(__ushort_as_half(ushort(r.y & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.y >> 16))));
-  return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xFFFF0000u), __uint_as_float(r.y << 16),
-                     __uint_as_float(r.y & 0xFFFF0000u));
+__device__ __forceinline__ Oct8 pyr_decode(typename PyrOct<F>::type r) {
+  if constexpr (F == 0) {
+    return Oct8{r.a, r.b};
+  } else if constexpr (F == 1) {
+    auto h = [](unsigned int u, bool hi) { return __half2float(__ushort_as_half(ushort(hi ? u >> 16 : u & 0xFFFFu))); };
+    return Oct8{make_float4(h(r.x, false), h(r.x, true), h(r.y, false), h(r.y, true)),
+                make_float4(h(r.z, false), h(r.z, true), h(r.w, false), h(r.w, true))};
+  } else {
+    auto b = [](unsigned int u, bool hi) { return __uint_as_float(hi ? u & 0xFFFF0000u : u << 16); };
+    return Oct8{make_float4(b(r.x, false), b(r.x, true), b(r.y, false), b(r.y, true)),
+                make_float4(b(r.z, false), b(r.z, true), b(r.w, false), b(r.w, true))};
+  }
 }
 
-// GL_LINEAR lookup of pyramid level `l` at texture-space point tp through the neighbour
-// blocks (2 vector loads per trilinear read); the texel coordinate tp * dims / G - 1/2
-// is one fma per axis.
+// GL_LINEAR lookup of pyramid level `l` at texture-space point tp: the texel coordinate
+// tp * dims / G - 1/2 is one fma per axis, the read one corner block.
 template <int F>
 __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, f3 tp) {
-  const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
-  int x0, y0;
-  float fx, fy;
-  quad_axis(fmaf(tp.x, pg.ts[l][0], -0.5f), nx, x0, fx);
-  quad_axis(fmaf(tp.y, pg.ts[l][1], -0.5f), ny, y0, fy);
-  const float zt = fmaf(tp.z, pg.ts[l][2], -0.5f);
-  const float zf = floorf(zt);
-  const float fz = zt - zf;
-  const int zi = int(zf);
-  const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
-  const size_t base = pg.boff[l];
-  return quad_lerp(pyr_block<F>(pyr, base + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])),
-                   pyr_block<F>(pyr, base + brick_index(x0, y0, z1, pg.nb[l][0], pg.nb[l][1])), fx, fy, fz);
+  int x0, y0, z0;
+  float fx, fy, fz;
+  quad_axis(fmaf(tp.x, pg.ts[l][0], -0.5f), pg.dims[l][0], x0, fx);
+  quad_axis(fmaf(tp.y, pg.ts[l][1], -0.5f), pg.dims[l][1], y0, fy);
+  quad_axis(fmaf(tp.z, pg.ts[l][2], -0.5f), pg.dims[l][2], z0, fz);
+  typedef typename PyrOct<F>::type B;
+  const Oct8 o = pyr_decode<F>(static_cast<const B*>(pyr)[pg.boff[l] + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])]);
+  return quad_lerp(o.a, o.b, fx, fy, fz);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
@@ -163,26 +167,11 @@ __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, cons
 
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
-// opacities along the light (8..14), the local opacity (15).  All 32 block loads are
-// issued before the first is used (texel weights and falloff x segment length kept in
-// registers), so one memory round trip serves the whole stencil; each read is
-// gauss_ext_level's, with the segment length folded into the falloff factor.
-template <int F> struct PyrBlk { typedef uint2 type; };
-template <> struct PyrBlk<0> { typedef float4 type; };
-
-template <int F>
-__device__ __forceinline__ float4 pyr_decode(typename PyrBlk<F>::type r) {
-  if constexpr (F == 0) {
-    return r;
-  } else if constexpr (F == 1) {
-    return make_float4(__half2float(__ushort_as_half(ushort(r.x & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.x >> 16))),
-                       __half2float(__ushort_as_half(ushort(r.y & 0xFFFFu))), __half2float(__ushort_as_half(ushort(r.y >> 16))));
-  } else {
-    return make_float4(__uint_as_float(r.x << 16), __uint_as_float(r.x & 0xFFFF0000u), __uint_as_float(r.y << 16),
-                       __uint_as_float(r.y & 0xFFFF0000u));
-  }
-}
-
+// opacities along the light (8..14), the local opacity (15).  The corner-block loads of
+// a group of kFeatGroup inputs are issued before the first of them is used (texel
+// weights and falloff x segment length kept in registers), so one memory round trip
+// serves the group; each read is gauss_ext_level's, with the segment length folded into
+// the falloff factor.
 typedef const __attribute__((address_space(4))) FeatTap* ConstTaps;
 
 #ifndef CVR_FEAT_GROUP
@@ -193,37 +182,34 @@ constexpr int kFeatGroup = CVR_FEAT_GROUP;
 template <int F>
 __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const FeatTap* __restrict__ taps,
                                          f3 p, f3 k, f3 L, float* f) {
-  typedef typename PyrBlk<F>::type B;
+  typedef typename PyrOct<F>::type B;
   // the table is wave-uniform and read-only: constant address space -> scalar loads
   const ConstTaps ft = (ConstTaps)(uintptr_t)taps;
 #pragma unroll
   for (int g0 = 0; g0 < kNFeat; g0 += kFeatGroup) {
-    B b0[kFeatGroup], b1[kFeatGroup];
+    B b[kFeatGroup];
     float fx[kFeatGroup], fy[kFeatGroup], fz[kFeatGroup], w[kFeatGroup];
 #pragma unroll
     for (int j = 0; j < kFeatGroup; ++j) {
       const int i = g0 + j;
       const f3 a = i < 2 * kNOcc ? k : L;
       const f3 tp = i == kNFeat - 1 ? p : p + a * ft[i].dist;
-      int x0, y0;
+      int x0, y0, z0;
       quad_axis(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dims[0], x0, fx[j]);
       quad_axis(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dims[1], y0, fy[j]);
-      const float zt = fmaf(tp.z, ft[i].ts[2], -0.5f);
-      const float zf = floorf(zt);
-      fz[j] = zt - zf;
-      const int zi = int(zf), nz = ft[i].dims[2];
-      const int z0 = min(max(zi, 0), nz - 1), z1 = min(max(zi + 1, 0), nz - 1);
+      quad_axis(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dims[2], z0, fz[j]);
       const B* lv = static_cast<const B*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
-      b0[j] = lv[brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1])];
-      b1[j] = lv[brick_index(x0, y0, z1, ft[i].nb[0], ft[i].nb[1])];
+      b[j] = lv[brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1])];
       w[j] = border_falloff(pg, tp, ft[i].inv2sg2);
       if (i != kNFeat - 1) w[j] *= ft[i].seg;
     }
     // keep the machine scheduler from pulling the lerps up between the loads
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int j = 0; j < kFeatGroup; ++j)
-      f[g0 + j] = 1.f - __expf(-(quad_lerp(pyr_decode<F>(b0[j]), pyr_decode<F>(b1[j]), fx[j], fy[j], fz[j]) * w[j]));
+    for (int j = 0; j < kFeatGroup; ++j) {
+      const Oct8 o = pyr_decode<F>(b[j]);
+      f[g0 + j] = 1.f - __expf(-(quad_lerp(o.a, o.b, fx[j], fy[j], fz[j]) * w[j]));
+    }
   }
 }
 

@@ -10,10 +10,10 @@ constexpr int kMaxLevels = 12;
 // Mip chain of the extinction pyramid: level dims and offsets into one fp32 buffer.
 struct PyrGeom {
   int levels;
-  int fmt;  // storage of the neighbour blocks: 0 fp32 (float4), 1 fp16, 2 bf16 (4 x 16 bit)
+  int fmt;  // storage of the corner blocks: 0 fp32 (2 x float4), 1 fp16, 2 bf16 (8 x 16 bit)
   int dims[kMaxLevels][3];
   size_t off[kMaxLevels];   // level offsets in the linear fp32 copy
-  size_t boff[kMaxLevels];  // level offsets in the brick-ordered neighbour blocks
+  size_t boff[kMaxLevels];  // level offsets (in records) in the brick-ordered corner blocks
   int nb[kMaxLevels][2];    // bricks along x and y per level
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
@@ -62,7 +62,7 @@ struct cvr_ctx {
 
   // Gaussian extinction pyramid (cone tracers, surrogate)
   float* d_pyr = nullptr;
-  void* d_pyr_quad = nullptr;  // neighbour blocks of every level (PyrGeom::fmt)
+  void* d_pyr_oct = nullptr;   // corner blocks of every level (PyrGeom::fmt, k_quad.hip)
   cvr::PyrGeom pyr{};
   uint64_t pyr_built_for = ~0ull;
   void* d_cone_tab = nullptr;  // surrogate input taps + cone step schedules
@@ -126,7 +126,7 @@ int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sd
                      float et, const PhongK& ph, float* out, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
 int build_volume_quad(cvr_ctx* c, hipStream_t st);
-int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st);
+int build_pyramid_oct(cvr_ctx* c, size_t total, int fmt, hipStream_t st);
 int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st);
 int check_cone_params(const cvr_cone_params& P);
 int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);

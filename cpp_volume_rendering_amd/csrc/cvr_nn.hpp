@@ -77,39 +77,48 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
   const bf16x8v_t* W3 = W2 + kTiles2 * 64;
   const bf16x8v_t* W4 = W3 + kTiles3 * 64;
   const f32x4_t* B4 = reinterpret_cast<const f32x4_t*>(sB);  // 4 biases of rows 4q..4q+3 per (layer, row tile)
-  // Sample tiles are independent through the network: each 16-sample column tile goes
-  // through all four layers before the next, so only its activation tiles are live.
-#pragma unroll 1
-  for (int nt = 0; nt < 4; ++nt) {
+  // The four 16-sample column tiles go through the network together: every weight tile
+  // is read from LDS once for all four, and the four independent MFMA chains hide each
+  // other's latency.
+  f32x4_t h[4][4];  // [column tile][row tile]
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
     f32x4_t xv;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) xv[e] = sX[(4 * q + e) * 64 + nt * 16 + r];
+    for (int e = 0; e < 4; ++e) xv[e] = sX[(4 * q + e) * 64 + u * 16 + r];
     const bf16x4_t xb = __builtin_bit_cast(bf16x4_t, __builtin_convertvector(xv, bf16x4v_t));
-    f32x4_t h[4];
     // layer 1: K = 16
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) h[mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
-    // layers 2 and 3: K = 64 as two 32-wide chunks
+    for (int mt = 0; mt < 4; ++mt) h[u][mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
+  }
+  // layers 2 and 3: K = 64 as two 32-wide chunks
 #pragma unroll
-    for (int l = 0; l < 2; ++l) {
-      const bf16x8v_t* Wl = l == 0 ? W2 : W3;
-      const bf16x8v_t b0 = relu_pack(h[0], h[1]), b1 = relu_pack(h[2], h[3]);
+  for (int l = 0; l < 2; ++l) {
+    const bf16x8v_t* Wl = l == 0 ? W2 : W3;
+    bf16x8v_t b0[4], b1[4];
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        f32x4_t acc = B4[(l + 1) * 16 + mt * 4 + q];
-        acc = mfma32(Wl[(mt * 2) * 64 + lane], b0, acc);
-        h[mt] = mfma32(Wl[(mt * 2 + 1) * 64 + lane], b1, acc);
-      }
+    for (int u = 0; u < 4; ++u) {
+      b0[u] = relu_pack(h[u][0], h[u][1]);
+      b1[u] = relu_pack(h[u][2], h[u][3]);
     }
-    // layer 4: 16 outputs (rows 0, 1 used)
-    const bf16x8v_t b0 = relu_pack(h[0], h[1]), b1 = relu_pack(h[2], h[3]);
-    f32x4_t acc = B4[48 + q];
-    acc = mfma32(W4[lane], b0, acc);
-    acc = mfma32(W4[64 + lane], b1, acc);
-    // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples nt*16 + r
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const bf16x8v_t wa = Wl[(mt * 2) * 64 + lane], wb = Wl[(mt * 2 + 1) * 64 + lane];
+      const f32x4_t bias = B4[(l + 1) * 16 + mt * 4 + q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) h[u][mt] = mfma32(wb, b1[u], mfma32(wa, b0[u], bias));
+    }
+  }
+  // layer 4: 16 outputs (rows 0, 1 used)
+  const bf16x8v_t w40 = W4[lane], w41 = W4[64 + lane];
+  const f32x4_t b4 = B4[48 + q];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const f32x4_t acc = mfma32(w41, relu_pack(h[u][2], h[u][3]), mfma32(w40, relu_pack(h[u][0], h[u][1]), b4));
+    // lanes 0..15 (q = 0) hold outputs 0 and 1 of samples u * 16 + r
     if (q == 0) {
-      sX[kNFeat * 64 + (nt * 16 + r) * 2] = acc[0];
-      sX[kNFeat * 64 + (nt * 16 + r) * 2 + 1] = acc[1];
+      sX[kNFeat * 64 + (u * 16 + r) * 2] = acc[0];
+      sX[kNFeat * 64 + (u * 16 + r) * 2 + 1] = acc[1];
     }
   }
   // lanes 0..15 wrote outputs 0 and 1 of every sample; hand them back
@@ -168,7 +177,7 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
   // per-lane ray state (pix < 0: idle)
   int pix = -1, px = 0, py = 0, j = 0;
   float D = 0.f, h = 0.f;
-  f3 dir = mk3(0.f, 0.f, 1.f), tp = eye, p = eye, k, up, right;
+  f3 dir = mk3(0.f, 0.f, 1.f), tp = eye, p = eye;  // the occlusion axis is -dir
   float4 dst = make_float4(0.f, 0.f, 0.f, 0.f), src = dst;
   bool parked = false;
   int next = 0;  // wave-uniform: next pixel of the span to hand out
@@ -187,7 +196,6 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
           dir = camera_ray(rf, px, py);
           if (ray_box(eye, dir, g.G, tn, tf_)) {
             pix = mine;
-            occ_frame(dir, k, up, right);
             D = fabsf(tf_ - tn);
             tp = eye + dir * tn + half;
             j = 0;
@@ -228,7 +236,7 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
       bool ood = false;
       float bo = 0.f, bs = 0.f;
       if (parked) {
-        nn_feats<F>(pyr, pg, K.ft, p, k, L, f);
+        nn_feats<F>(pyr, pg, K.ft, p, mk3(-dir.x, -dir.y, -dir.z), L, f);
 #pragma unroll
         for (int c = 0; c < kNFeat; ++c) ood |= (f[c] < nk.lo[c]) | (f[c] > nk.hi[c]);
         // the network-free estimate now, so the features die inside the network's input staging
@@ -245,6 +253,8 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
         ++cnt[1];
         if (ood || nk.exact_all) {
           ++cnt[2];
+          f3 k, up, right;
+          occ_frame(dir, k, up, right);
           Io = K.apply_occ ? ct_occlusion<F>(pyr, pg, K, p, k, up, right) : 0.f;
           Is = K.apply_sdw ? cone_T<F>(pyr, pg, K.ss, K.n_steps, p, L) : 0.f;
         }

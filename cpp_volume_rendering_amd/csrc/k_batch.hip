@@ -186,7 +186,7 @@ int build_tables(cvr_batch* b) {
     VolDesc& d = vd[v];
     d.g = geom_of(c);
     d.pg = c->pyr;
-    d.pyr = c->d_pyr_quad;
+    d.pyr = c->d_pyr_oct;
     const float L0[3] = {0.f, 0.f, 1.f};
     d.K = cone_constants(c, N.cone, L0);
     d.nk = nn_consts(c);

@@ -252,7 +252,7 @@ int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* o
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
     hipLaunchKernelGGL((k_cone_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
-                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, o4, cnt);
+                       c->d_pyr_oct, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, o4, cnt);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
@@ -266,7 +266,7 @@ int cone_points(cvr_ctx* c, const cvr_cone_params& P, int n, const float* d_pos,
   ConeK K = cone_constants(c, P, L);
   with_fmt(c->pyr.fmt, [&](auto f) {
     hipLaunchKernelGGL((k_cone_points<decltype(f)::value>), dim3((n + 255) / 256), dim3(256), 0, st, c->pyr,
-                       c->d_pyr_quad, K, n, d_pos, d_dirs, d_lights, d_occ, d_sdw, d_feat);
+                       c->d_pyr_oct, K, n, d_pos, d_dirs, d_lights, d_occ, d_sdw, d_feat);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -470,7 +470,7 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     const f3 cell{cs[0], cs[1], cs[2]};
     with_fmt(c->pyr.fmt, [&](auto f) {
       hipLaunchKernelGGL((k_lc_dos<decltype(f)::value>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, c->pyr,
-                         c->d_pyr_quad, K, cam_up, rx, ry, rz, cell, half, c->d_lc);
+                         c->d_pyr_oct, K, cam_up, rx, ry, rz, cell, half, c->d_lc);
     });
     CVR_HIP_CHECK(hipGetLastError());
     return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
@@ -480,7 +480,7 @@ int launch_dos(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
     hipLaunchKernelGGL((k_dos_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
-                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, ph, c->d_grad, o4, cnt);
+                       c->d_pyr_oct, K, c->d_tf_rgbt, c->tf_size, step, P.et_threshold, ph, c->d_grad, o4, cnt);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return CVR_OK;

@@ -434,7 +434,7 @@ int nn_generate(cvr_ctx* c, const cvr_cone_params& P, uint64_t seed, int n, floa
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
     hipLaunchKernelGGL((k_nn_samples<T, decltype(f)::value>), dim3((n + 255) / 256), dim3(256), 0, st, g, c->pyr,
-                       c->d_pyr_quad, K, c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
+                       c->d_pyr_oct, K, c->d_tf_rgbt, c->tf_size, seed, n, d_X, d_T, d_pos);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
@@ -613,7 +613,7 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
     hipLaunchKernelGGL((k_nn_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
-                       c->d_pyr_quad, K, nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, cnt);
+                       c->d_pyr_oct, K, nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, cnt);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -189,7 +189,7 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   CVR_HIP_CHECK(hipFree(d_op));
   c->pyr = pg;
-  if (build_pyramid_quad(c, btotal, fmt, st) != CVR_OK) {
+  if (build_pyramid_oct(c, btotal, fmt, st) != CVR_OK) {
     c->pyr_built_for = ~0ull;
     return -5;
   }

@@ -1,11 +1,15 @@
-// Neighbour-block ("quad") copies of the sampled grids.  Cell (x, y, z) of a quad grid
-// holds the values at (x, y), (x+1, y), (x, y+1), (x+1, y+1) of plane z, with the +1
-// neighbours clamped to the edge, so a GL_LINEAR trilinear read is two vector loads
-// (planes z0 and z1) instead of eight scalar gathers.  Used for the raw volume
-// (uchar4 / ushort4 per voxel) and for every level of the extinction pyramid (float4,
-// or four fp16 / bf16 values in 8 bytes).  The blocks are stored in brick order
-// (4 x 4 x 4 cells per brick, brick_index), so that the samples of a wavefront, which
-// spread over a small 3D region, touch few cache lines.
+// Neighbour-block copies of the sampled grids, so that a GL_LINEAR trilinear read is one
+// or two vector loads instead of eight scalar gathers.
+//  - Raw volume, "quad" blocks: cell (x, y, z) holds the values at (x, y), (x+1, y),
+//    (x, y+1), (x+1, y+1) of plane z (uchar4 / ushort4), the +1 neighbours clamped to
+//    the edge; a read is two loads (planes z0 and z1).
+//  - Extinction pyramid, "corner" blocks: cell (x, y, z) holds all eight values of
+//    (x..x+1, y..y+1, z..z+1), clamped likewise (32 bytes in fp32, 16 in fp16 / bf16);
+//    a read is one record on one cache line.  Twice the quad footprint, half the loads
+//    and half the cache lines per read, which is what the cone-traced renderers and the
+//    surrogate's 16 feature reads per shaded sample are bound by.
+// Both are stored in brick order (4 x 4 x 4 cells per brick, brick_index), so that the
+// samples of a wavefront, which spread over a small 3D region, touch few cache lines.
 #include "cvr_ctx.hpp"
 #include <hip/hip_fp16.h>
 
@@ -66,48 +70,54 @@ __device__ __forceinline__ unsigned int to16(float x, int fmt) {
   return u >> 16;
 }
 
-// Narrows fp32 neighbour blocks to 4 x 16 bit.
-__global__ __launch_bounds__(256) void k_pack_quad16(const float4* __restrict__ src, uint2* __restrict__ dst, size_t n,
-                                                     int fmt) {
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float4 q = src[i];
-  dst[i] = make_uint2(to16(q.x, fmt) | (to16(q.y, fmt) << 16), to16(q.z, fmt) | (to16(q.w, fmt) << 16));
+// Corner blocks of one pyramid level: cell (x, y, z) holds the 2 x 2 x 2 values at
+// (x..x+1, y..y+1, z..z+1), the +1 neighbours clamped to the edge, plane z first, each
+// plane (x, y), (x+1, y), (x, y+1), (x+1, y+1); fp32 as 32 bytes, fp16 / bf16 as 16.
+template <int FMT>
+__global__ __launch_bounds__(256) void k_make_oct(const float* __restrict__ src, void* __restrict__ dst, int nx, int ny,
+                                                  int nz) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny || z >= nz) return;
+  const int x1 = min(x + 1, nx - 1), y1 = min(y + 1, ny - 1), z1 = min(z + 1, nz - 1);
+  const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  const size_t r00 = z * sz + y * sy, r01 = z * sz + y1 * sy, r10 = z1 * sz + y * sy, r11 = z1 * sz + y1 * sy;
+  const float v[8] = {src[r00 + x], src[r00 + x1], src[r01 + x], src[r01 + x1],
+                      src[r10 + x], src[r10 + x1], src[r11 + x], src[r11 + x1]};
+  const size_t i = brick_index(x, y, z, bricks(nx), bricks(ny));
+  if (FMT == 0) {
+    float4* d = static_cast<float4*>(dst) + 2 * i;
+    d[0] = make_float4(v[0], v[1], v[2], v[3]);
+    d[1] = make_float4(v[4], v[5], v[6], v[7]);
+  } else {
+    static_cast<uint4*>(dst)[i] = make_uint4(to16(v[0], FMT) | (to16(v[1], FMT) << 16), to16(v[2], FMT) | (to16(v[3], FMT) << 16),
+                                             to16(v[4], FMT) | (to16(v[5], FMT) << 16), to16(v[6], FMT) | (to16(v[7], FMT) << 16));
+  }
 }
 
-int build_pyramid_quad(cvr_ctx* c, size_t total, int fmt, hipStream_t st) {
-  if (c->d_pyr_quad) CVR_HIP_CHECK(hipFree(c->d_pyr_quad));
-  c->d_pyr_quad = nullptr;
-  float4* q32 = nullptr;
-  if (hipMalloc(&q32, total * sizeof(float4)) != hipSuccess) {
-    set_error("pyramid: out of device memory (quad layout)");
+int build_pyramid_oct(cvr_ctx* c, size_t total, int fmt, hipStream_t st) {
+  if (c->d_pyr_oct) CVR_HIP_CHECK(hipFree(c->d_pyr_oct));
+  c->d_pyr_oct = nullptr;
+  const size_t rec = fmt == 0 ? 32 : 16;
+  if (hipMalloc(&c->d_pyr_oct, total * rec) != hipSuccess) {
+    c->d_pyr_oct = nullptr;
+    set_error("pyramid: out of device memory (corner blocks)");
     return CVR_ERR_NOMEM;
   }
   const PyrGeom& pg = c->pyr;
   for (int l = 0; l < pg.levels; ++l) {
-    const int rc = make_quad(c->d_pyr + pg.off[l], q32 + pg.boff[l], pg.dims[l][0], pg.dims[l][1], pg.dims[l][2], st);
-    if (rc != CVR_OK) {
-      (void)hipFree(q32);
-      return rc;
-    }
+    const int nx = pg.dims[l][0], ny = pg.dims[l][1], nz = pg.dims[l][2];
+    if (nz > 65535) { set_error("corner blocks: depth above 65535"); return CVR_ERR_INVALID; }
+    const dim3 grid(unsigned((nx + 63) / 64), unsigned((ny + 3) / 4), unsigned(nz));
+    void* d = static_cast<char*>(c->d_pyr_oct) + pg.boff[l] * rec;
+    const float* s = c->d_pyr + pg.off[l];
+    if (fmt == 0) hipLaunchKernelGGL(k_make_oct<0>, grid, dim3(256), 0, st, s, d, nx, ny, nz);
+    else if (fmt == 1) hipLaunchKernelGGL(k_make_oct<1>, grid, dim3(256), 0, st, s, d, nx, ny, nz);
+    else hipLaunchKernelGGL(k_make_oct<2>, grid, dim3(256), 0, st, s, d, nx, ny, nz);
+    CVR_HIP_CHECK(hipGetLastError());
   }
-  if (fmt == 0) {
-    CVR_HIP_CHECK(hipStreamSynchronize(st));
-    c->d_pyr_quad = q32;
-    return CVR_OK;
-  }
-  void* q16 = nullptr;
-  if (hipMalloc(&q16, total * sizeof(uint2)) != hipSuccess) {
-    (void)hipFree(q32);
-    set_error("pyramid: out of device memory (quad layout)");
-    return CVR_ERR_NOMEM;
-  }
-  hipLaunchKernelGGL(k_pack_quad16, dim3(unsigned((total + 255) / 256)), dim3(256), 0, st, q32,
-                     static_cast<uint2*>(q16), total, fmt);
-  CVR_HIP_CHECK(hipGetLastError());
   CVR_HIP_CHECK(hipStreamSynchronize(st));
-  CVR_HIP_CHECK(hipFree(q32));
-  c->d_pyr_quad = q16;
   return CVR_OK;
 }
 
