@@ -212,6 +212,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_out) (void)hipFree(c->d_out);
   if (c->d_pyr) (void)hipFree(c->d_pyr);
   if (c->d_pyr_oct) (void)hipFree(c->d_pyr_oct);
+  if (c->d_nn_redo) (void)hipFree(c->d_nn_redo);
   if (c->d_vol_quad) (void)hipFree(c->d_vol_quad);
   if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
   if (c->d_dos_rows) (void)hipFree(c->d_dos_rows);

@@ -98,6 +98,8 @@ struct cvr_ctx {
   // stats
   bool collect_stats = false;
   unsigned long long* d_counters = nullptr;
+  unsigned int* d_nn_redo = nullptr;  // surrogate: tiles marked for the exact second pass
+  size_t nn_redo_cap = 0;
   cvr_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 

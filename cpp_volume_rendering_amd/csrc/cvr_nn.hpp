@@ -157,8 +157,13 @@ void light_dir(const RayFrame& rf, const cvr_light* light, float L[3]);
 // wave keeps its lanes busy until its pixels run out instead of idling behind its
 // longest ray.  sW/sB/s_tf: the network and transfer function staged in LDS; sXw: this
 // wave's LDS scratch.  Counters (samples, shaded, exact fallbacks) accumulate into cnt.
-template <typename T, int F>
-__device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, const PyrGeom& pg,
+// kExact: a sample whose inputs leave the training range takes the exact cone tracer's
+// answers; without it the tile is only marked (the return value: some lane of this wave
+// met such a sample) for a second pass with kExact, and its pixels are provisional.
+// Which samples a ray takes and their opacities do not depend on the shading, so both
+// passes march a tile identically.
+template <typename T, int F, bool kExact = true>
+__device__ __forceinline__ bool nn_tile(const RayFrame& rf, const VolGeom& g, const PyrGeom& pg,
                                         const void* __restrict__ pyr, const ConeK& K, const NnK& nk,
                                         const NnSched& sc, const short* sW, const float* sB, float* sXw,
                                         const float4* s_tf, int ntf, float step, float et, float4* __restrict__ out,
@@ -179,7 +184,7 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
   float D = 0.f, h = 0.f;
   f3 dir = mk3(0.f, 0.f, 1.f), tp = eye, p = eye;  // the occlusion axis is -dir
   float4 dst = make_float4(0.f, 0.f, 0.f, 0.f), src = dst;
-  bool parked = false;
+  bool parked = false, ood_seen = false;
   int next = 0;  // wave-uniform: next pixel of the span to hand out
   for (;;) {
     const uint64_t idle = __ballot(pix < 0);
@@ -251,7 +256,9 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
         Io += bo;
         Is += bs;
         ++cnt[1];
-        if (ood || nk.exact_all) {
+        if (!kExact) {
+          ood_seen |= ood;
+        } else if (ood || nk.exact_all) {
           ++cnt[2];
           f3 k, up, right;
           occ_frame(dir, k, up, right);
@@ -277,6 +284,7 @@ __device__ __forceinline__ void nn_tile(const RayFrame& rf, const VolGeom& g, co
       pix = -1;
     }
   }
+  return __any(ood_seen);
 }
 
 // Stages the packed network and the transfer function of one volume in LDS (all

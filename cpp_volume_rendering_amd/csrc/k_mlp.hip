@@ -302,28 +302,42 @@ __global__ void k_feat_range(const float* __restrict__ X, int n, unsigned int* _
 }
 
 // ---------------------------------------------------------------- surrogate renderer
-template <typename T, int F>
+// Two passes over the same tiles.  kExact = false renders every tile with the network
+// alone and marks (redo[tile] = 1) the tiles where some sample's inputs left the
+// training range; kExact = true then renders only the marked tiles again, exact cone
+// tracer for those samples, and clears their marks, so every mark is 0 between frames.
+// The first pass carries no exact-tracer code (its registers and scalar state serve the
+// march, the features and the network alone); the second costs one launch when nothing
+// is marked.  Counters: samples and shaded samples from the first pass, exact fallbacks
+// from the second (which marches a tile exactly as the first did).
+template <typename T, int F, bool kExact>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_nn_render(RayFrame rf, VolGeom g, PyrGeom pg, const void* __restrict__ pyr,
                                                    ConeK K, NnK nk, NnSched sc, const short* __restrict__ wpack,
                                                    const float* __restrict__ bpack, const float4* __restrict__ tf,
                                                    int ntf, float step, float et, float4* __restrict__ out,
-                                                   unsigned long long* __restrict__ counters) {
+                                                   unsigned int* __restrict__ redo, unsigned long long* __restrict__ counters) {
   __shared__ __align__(16) short sW[kPackShorts];
   __shared__ __align__(16) float sB[kNBias];
   __shared__ float sX[4][kWaveScratch];
   extern __shared__ float4 s_tf[];
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (kExact && redo[bid] == 0u) return;  // the same word for every thread of the block
   nn_stage(wpack, bpack, tf, ntf, sW, sB, s_tf);
   __syncthreads();
-  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  if (kExact && threadIdx.x == 0) redo[bid] = 0u;  // every thread has read it (barrier above)
   const int tw = 1 << sc.tw_log2;
   const int tiles_x = (rf.W + 2 * tw - 1) / (2 * tw);
   unsigned int cnt[3] = {0, 0, 0};
-  nn_tile<T, F>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step, et, out, bid % tiles_x,
-             bid / tiles_x, cnt);
+  const bool marked = nn_tile<T, F, kExact>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step,
+                                            et, out, bid % tiles_x, bid / tiles_x, cnt);
+  if (!kExact && marked && (threadIdx.x & 63) == 0) redo[bid] = 1u;
   if (counters) {
-    atomicAdd(&counters[0], (unsigned long long)cnt[0]);
-    atomicAdd(&counters[1], (unsigned long long)cnt[1]);
-    atomicAdd(&counters[2], (unsigned long long)cnt[2]);
+    if (!kExact) {
+      atomicAdd(&counters[0], (unsigned long long)cnt[0]);
+      atomicAdd(&counters[1], (unsigned long long)cnt[1]);
+    } else {
+      atomicAdd(&counters[2], (unsigned long long)cnt[2]);
+    }
   }
 }
 
@@ -610,10 +624,27 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   float* bp = reinterpret_cast<float*>(wp + kPackShorts);
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   float4* o4 = reinterpret_cast<float4*>(out);
+  // tile marks for the second pass: zero when allocated, and the second pass clears
+  // what the first sets
+  if (size_t(tiles) > c->nn_redo_cap) {
+    if (c->d_nn_redo) CVR_HIP_CHECK(hipFree(c->d_nn_redo));
+    c->d_nn_redo = nullptr;
+    c->nn_redo_cap = 0;
+    if (hipMalloc(&c->d_nn_redo, size_t(tiles) * sizeof(unsigned int)) != hipSuccess) {
+      c->d_nn_redo = nullptr;
+      set_error("surrogate: out of device memory");
+      return -5;
+    }
+    CVR_HIP_CHECK(hipMemset(c->d_nn_redo, 0, size_t(tiles) * sizeof(unsigned int)));
+    c->nn_redo_cap = size_t(tiles);
+  }
   with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
     using T = typename decltype(t)::type;
-    hipLaunchKernelGGL((k_nn_render<T, decltype(f)::value>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr,
-                       c->d_pyr_oct, K, nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, cnt);
+    constexpr int FF = decltype(f)::value;
+    hipLaunchKernelGGL((k_nn_render<T, FF, false>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
+                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, c->d_nn_redo, cnt);
+    hipLaunchKernelGGL((k_nn_render<T, FF, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
+                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, c->d_nn_redo, cnt);
   });
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -143,6 +143,49 @@ def test_surrogate_image_close_to_exact_tracer():
     assert np.array_equal(exact, ref)
 
 
+def test_out_of_range_samples_take_the_exact_tracer():
+    """The renderer's first pass uses the network alone and marks the tiles where a
+    sample's inputs leave the training range (+- ood_margin); a second pass renders the
+    marked tiles again with the exact tracer for those samples and clears the marks."""
+    wl, r, p = train_small()
+    cam, light = wl.camera(), wl.light()
+    r.collect_stats(True)
+    base = r.render(cam, 96, 96, light=light)
+    st0 = r.stats()
+    assert st0.fallback_samples == 0
+    ct = N.Renderer(N.CONE_TRACE)
+    ct.set_volume(wl.volume)
+    ct.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    ct.set_params(p.cone)
+    ref = ct.render(cam, 96, 96, light=light)
+    # an empty range: every shaded sample is out of it, every tile is rendered again and
+    # the frame is the exact tracer's
+    p.ood_margin = -2.0
+    r.set_params(p)
+    allx = r.render(cam, 96, 96, light=light)
+    st = r.stats()
+    print("all out of range: max diff", np.abs(allx - ref).max(), st.fallback_samples, st.shaded_samples)
+    assert st.fallback_samples == st.shaded_samples == st0.shaded_samples
+    assert st.samples == st0.samples
+    assert np.abs(allx - ref).max() <= 2e-5  # the two renderers round wgt * src * m in another order
+    # a narrowed range: some tiles marked, the others the network's
+    p.ood_margin = -0.02
+    r.set_params(p)
+    part = r.render(cam, 96, 96, light=light)
+    st = r.stats()
+    print("narrowed range: fallback", st.fallback_samples, "of", st.shaded_samples)
+    assert 0 < st.fallback_samples < st.shaded_samples
+    changed = np.any(part != base, axis=-1)
+    assert changed.any() and not changed.all()
+    assert np.array_equal(r.render(cam, 96, 96, light=light), part)  # marks were cleared in between
+    # the range restored: no marks left over, the first frame again
+    p.ood_margin = 0.02
+    r.set_params(p)
+    again = r.render(cam, 96, 96, light=light)
+    assert r.stats().fallback_samples == 0
+    assert np.array_equal(again, base)
+
+
 def test_checkpoint_restores_the_renderer_exactly():
     wl, r, p = train_small(n=32, n_train=1 << 15, epochs=2)
     blob = r.nn_save()
