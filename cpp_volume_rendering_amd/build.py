@@ -29,9 +29,11 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"-I{INCLUDE}", f"-I{CSRC}",
     "-Wall", "-Wno-unused-variable", "-Wno-unused-function", "-Wno-unused-but-set-variable",
 ]
-# Device code: gfx950 only; the packed-fp32 feature stays off (its host-pass warning is harmless).
+# Device code: gfx950 only; the packed-fp32 feature stays off (its host-pass warning is
+# harmless).  MFMA accumulators live in VGPRs: the network's ReLU and bf16 packing then
+# read them directly instead of copying each one out of an AGPR first.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-munsafe-fp-atomics"]
+          "-munsafe-fp-atomics", "-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 
 def _sources():

@@ -11,6 +11,10 @@ typedef float f32x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4v_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8v_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2v_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef short i16x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 64, kOut = 16;
 constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kH * kNFeat;
@@ -48,16 +52,18 @@ __device__ __forceinline__ f32x4_t mfma32(bf16x8v_t a, bf16x8v_t b, f32x4_t c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// ReLU of two D tiles and round to nearest even into one 8 x bf16 B operand
-// (v_cvt_pk_bf16_f32).
+// Round two D tiles to nearest even into one 8 x bf16 B operand, a pair at a time
+// (v_cvt_pk_bf16_f32), then ReLU on the bf16 bits: a negative bf16 (also -0) is a
+// negative int16, so a packed 16-bit integer max with 0 (v_pk_max_i16) gives exactly
+// bf16(max(x, 0)), without an fp32 max (and its input canonicalisation) per value.
+__device__ __forceinline__ unsigned int relu2(float x, float y) {
+  const f32x2_t p = {x, y};
+  const i16x2_t v = __builtin_bit_cast(i16x2_t, __builtin_convertvector(p, bf16x2v_t));
+  return __builtin_bit_cast(unsigned int, __builtin_elementwise_max(v, i16x2_t{0, 0}));
+}
 __device__ __forceinline__ bf16x8v_t relu_pack(f32x4_t a, f32x4_t b) {
-  f32x8_t v;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v[e] = fmaxf(a[e], 0.f);
-    v[4 + e] = fmaxf(b[e], 0.f);
-  }
-  return __builtin_convertvector(v, bf16x8v_t);
+  const u32x4_t w = {relu2(a[0], a[1]), relu2(a[2], a[3]), relu2(b[0], b[1]), relu2(b[2], b[3])};
+  return __builtin_bit_cast(bf16x8v_t, w);
 }
 
 // Wavefront-cooperative inference: every lane brings one sample's 16 features; the
