@@ -103,9 +103,10 @@ __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const Pyr
 // factor is exactly 1 there and no branch is needed (the loads of many lookups can then
 // be issued back to back).  inv2sg2 = 1 / (2 (2^lod)^2).
 __device__ __forceinline__ float border_falloff(const PyrGeom& pg, f3 tp, float inv2sg2) {
-  const float cx = fminf(fmaxf(tp.x, 0.f), pg.G[0]) - tp.x;
-  const float cy = fminf(fmaxf(tp.y, 0.f), pg.G[1]) - tp.y;
-  const float cz = fminf(fmaxf(tp.z, 0.f), pg.G[2]) - tp.z;
+  // clamp(p, 0, G) as one median-of-three (p is finite, G > 0)
+  const float cx = __builtin_amdgcn_fmed3f(tp.x, 0.f, pg.G[0]) - tp.x;
+  const float cy = __builtin_amdgcn_fmed3f(tp.y, 0.f, pg.G[1]) - tp.y;
+  const float cz = __builtin_amdgcn_fmed3f(tp.z, 0.f, pg.G[2]) - tp.z;
   return __expf(-(cx * cx + cy * cy + cz * cz) * inv2sg2);
 }
 

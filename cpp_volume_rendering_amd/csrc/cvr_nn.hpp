@@ -68,13 +68,22 @@ __device__ __forceinline__ bf16x8v_t relu_pack(f32x4_t a, f32x4_t b) {
 
 // Wavefront-cooperative inference: every lane brings one sample's 16 features; the
 // 64 samples go through the network as 4 column tiles of MFMA.  sW/sB: packed weights
-// and biases in LDS; sX: this wave's LDS scratch of kWaveScratch floats (16x64 inputs,
-// then 64x2 outputs).  Returns outputs 0 and 1.
+// and biases in LDS; sX: this wave's LDS scratch of kWaveScratch floats: the inputs as
+// 64 rows of 16 bf16 (rounded to nearest even once, here), kInRow dwords apart, then
+// 64x2 outputs at kNFeat * 64.  Returns outputs 0 and 1.
+constexpr int kInRow = 10;  // 8 dwords of inputs + 2 of padding: the operand reads hit 32 distinct banks
+static_assert(64 * kInRow <= kNFeat * 64, "input rows fit below the outputs");
 __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const float* __restrict__ sB, float* sX,
                                          const float* f, float& o0, float& o1) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+  u32x2_t* row = reinterpret_cast<u32x2_t*>(sX + lane * kInRow);
 #pragma unroll
-  for (int i = 0; i < kNFeat; ++i) sX[i * 64 + lane] = f[i];
+  for (int i = 0; i < kNFeat / 4; ++i) {
+    const f32x2_t a = {f[4 * i], f[4 * i + 1]}, b = {f[4 * i + 2], f[4 * i + 3]};
+    row[i] = u32x2_t{__builtin_bit_cast(unsigned int, __builtin_convertvector(a, bf16x2v_t)),
+                     __builtin_bit_cast(unsigned int, __builtin_convertvector(b, bf16x2v_t))};
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -89,10 +98,8 @@ __device__ __forceinline__ void mlp_wave(const short* __restrict__ sW, const flo
   f32x4_t h[4][4];  // [column tile][row tile]
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    f32x4_t xv;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) xv[e] = sX[(4 * q + e) * 64 + u * 16 + r];
-    const bf16x4_t xb = __builtin_bit_cast(bf16x4_t, __builtin_convertvector(xv, bf16x4v_t));
+    // lane 16q + r: inputs 4q..4q+3 of sample 16u + r
+    const bf16x4_t xb = *reinterpret_cast<const bf16x4_t*>(sX + (u * 16 + r) * kInRow + 2 * q);
     // layer 1: K = 16
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) h[u][mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                       unsigned long long* __restrict__ counters) {
   __shared__ __align__(16) short sW[kPackShorts];
   __shared__ __align__(16) float sB[kNBias];
-  __shared__ float sX[4][kWaveScratch];
+  __shared__ __align__(16) float sX[4][kWaveScratch];
   __shared__ int s_item;
   extern __shared__ float4 s_tf[];
   const int home = int(blockIdx.x) % nq;

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const short* __restrict__ wpa
                                                    const float* __restrict__ X, int n, float* __restrict__ Y) {
   __shared__ __align__(16) short sW[kPackShorts];
   __shared__ __align__(16) float sB[kNBias];
-  __shared__ float sX[4][kWaveScratch];
+  __shared__ __align__(16) float sX[4][kWaveScratch];
   for (int i = threadIdx.x; i < kPackShorts / 4; i += blockDim.x)
     reinterpret_cast<bf16x4_t*>(sW)[i] = reinterpret_cast<const bf16x4_t*>(wpack)[i];
   for (int i = threadIdx.x; i < kNBias; i += blockDim.x) sB[i] = bpack[i];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                    unsigned int* __restrict__ redo, unsigned long long* __restrict__ counters) {
   __shared__ __align__(16) short sW[kPackShorts];
   __shared__ __align__(16) float sB[kNBias];
-  __shared__ float sX[4][kWaveScratch];
+  __shared__ __align__(16) float sX[4][kWaveScratch];
   extern __shared__ float4 s_tf[];
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   if (kExact && redo[bid] == 0u) return;  // the same word for every thread of the block
