@@ -23,7 +23,7 @@ VolGeom geom_of(const cvr_ctx* c) {
     g.tex_mul[i] = float(double(n[i]) / double(g.G[i]));
   }
   g.vox = c->d_vol;
-  g.quad = c->d_vol_quad;
+  g.oct = c->d_vol_oct;
   g.nbx = bricks(c->nx);
   g.nby = bricks(c->ny);
   return g;
@@ -213,7 +213,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_pyr) (void)hipFree(c->d_pyr);
   if (c->d_pyr_oct) (void)hipFree(c->d_pyr_oct);
   if (c->d_nn_redo) (void)hipFree(c->d_nn_redo);
-  if (c->d_vol_quad) (void)hipFree(c->d_vol_quad);
+  if (c->d_vol_oct) (void)hipFree(c->d_vol_oct);
   if (c->d_cone_tab) (void)hipFree(c->d_cone_tab);
   if (c->d_dos_rows) (void)hipFree(c->d_dos_rows);
   if (c->d_gt_rays) (void)hipFree(c->d_gt_rays);
@@ -324,7 +324,7 @@ int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, fl
   }
   CVR_HIP_CHECK(hipMemcpy(h->d_vol, voxels, bytes, is_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
   h->nx = nx; h->ny = ny; h->nz = nz; h->dtype = dtype;
-  if (const int rc = build_volume_quad(h, nullptr)) return rc;
+  if (const int rc = build_volume_oct(h, nullptr)) return rc;
   h->sx = sx; h->sy = sy; h->sz = sz;
   h->vol_version++;
   h->nn_trained = false;

@@ -53,7 +53,7 @@ struct VolGeom {
   float scale[3];    // voxel size
   float tex_mul[3];  // n / G: world -> texel units
   const void* vox;   // raw voxels, x fastest
-  const void* quad;  // per voxel its (x, x+1) x (y, y+1) block, edge-clamped (4 values), brick order
+  const void* oct;   // per voxel its 2 x 2 x 2 corner block, edge-clamped (8 values), brick order
   int nbx, nby;      // bricks along x and y
 };
 
@@ -126,34 +126,38 @@ __device__ __forceinline__ float quad_lerp(float4 a, float4 b, float fx, float f
   return c0 + fz * (c1 - c0);
 }
 
-template <typename T> struct Quad;
-template <> struct Quad<uint8_t> {
-  typedef uchar4 type;
-  __device__ static float4 f(uchar4 q) { return make_float4(float(q.x), float(q.y), float(q.z), float(q.w)); }
+// Corner blocks of the raw volume: cell (x, y, z) holds the 8 voxels of (x..x+1,
+// y..y+1, z..z+1), the +1 neighbours clamped to the edge, plane z first, each plane
+// (x, y), (x+1, y), (x, y+1), (x+1, y+1): 8 bytes for u8 (uint2), 16 for u16 (uint4).
+template <typename T> struct VolOct;
+template <> struct VolOct<uint8_t> {
+  typedef uint2 type;
+  __device__ static void f(uint2 r, float4& a, float4& b) {
+    a = make_float4(float(r.x & 0xFFu), float((r.x >> 8) & 0xFFu), float((r.x >> 16) & 0xFFu), float(r.x >> 24));
+    b = make_float4(float(r.y & 0xFFu), float((r.y >> 8) & 0xFFu), float((r.y >> 16) & 0xFFu), float(r.y >> 24));
+  }
 };
-template <> struct Quad<uint16_t> {
-  typedef ushort4 type;
-  __device__ static float4 f(ushort4 q) { return make_float4(float(q.x), float(q.y), float(q.z), float(q.w)); }
+template <> struct VolOct<uint16_t> {
+  typedef uint4 type;
+  __device__ static void f(uint4 r, float4& a, float4& b) {
+    a = make_float4(float(r.x & 0xFFFFu), float(r.x >> 16), float(r.y & 0xFFFFu), float(r.y >> 16));
+    b = make_float4(float(r.z & 0xFFFFu), float(r.z >> 16), float(r.w & 0xFFFFu), float(r.w >> 16));
+  }
 };
 
-// Trilinear sample through the quad layout: 2 loads instead of 8, same result as
+// Trilinear sample through the corner blocks: 1 load instead of 8, same result as
 // sample_volume (same lerp order, same clamped taps).
 template <typename T>
-__device__ __forceinline__ float sample_volume_quad(const VolGeom& g, f3 p) {
-  typedef typename Quad<T>::type QT;
-  const QT* __restrict__ q = static_cast<const QT*>(g.quad);
+__device__ __forceinline__ float sample_volume_oct(const VolGeom& g, f3 p) {
+  typedef typename VolOct<T>::type RT;
+  const RT* __restrict__ q = static_cast<const RT*>(g.oct);
   int x0, y0, z0;
   float fx, fy, fz;
   quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
   quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
-  const float zt = p.z * g.tex_mul[2] - 0.5f;
-  const float zf = floorf(zt);
-  fz = zt - zf;
-  const int zi = int(zf);
-  z0 = min(max(zi, 0), g.nz - 1);
-  const int z1 = min(max(zi + 1, 0), g.nz - 1);
-  const float4 a = Quad<T>::f(q[brick_index(x0, y0, z0, g.nbx, g.nby)]);
-  const float4 b = Quad<T>::f(q[brick_index(x0, y0, z1, g.nbx, g.nby)]);
+  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  float4 a, b;
+  VolOct<T>::f(q[brick_index(x0, y0, z0, g.nbx, g.nby)], a, b);
   return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
 }
 

@@ -36,7 +36,7 @@ struct cvr_ctx {
   int nx = 0, ny = 0, nz = 0, dtype = 0;
   float sx = 1, sy = 1, sz = 1;
   void* d_vol = nullptr;
-  void* d_vol_quad = nullptr;  // uchar4 / ushort4 neighbour blocks (k_quad.hip)
+  void* d_vol_oct = nullptr;  // corner blocks of the voxels, 8 x u8 / 8 x u16 (k_quad.hip)
 
   // transfer function (host FP64 table as TransferFunction1D::BuildLinear builds it)
   int tf_max_density = 255;
@@ -127,7 +127,7 @@ int lc_prepare(cvr_ctx* c, const int res[3]);
 int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sdw, float ka, float kd, float step,
                      float et, const PhongK& ph, float* out, hipStream_t st);
 int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);
-int build_volume_quad(cvr_ctx* c, hipStream_t st);
+int build_volume_oct(cvr_ctx* c, hipStream_t st);
 int build_pyramid_oct(cvr_ctx* c, size_t total, int fmt, hipStream_t st);
 int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st);
 int check_cone_params(const cvr_cone_params& P);

@@ -240,7 +240,7 @@ __device__ __forceinline__ bool nn_tile(const RayFrame& rf, const VolGeom& g, co
       } else {
         h = fminf(step, D - s);
         p = tp + dir * (s + 0.5f * h);
-        src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
+        src = sample_tf(s_tf, ntf, sample_volume_oct<T>(g, p));
         ++cnt[0];
         if (src.w > 0.f) parked = true;
         else ++j;

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_dos_render(RayFrame rf, VolGeom g, PyrG
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      float4 src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
+      float4 src = sample_tf(s_tf, ntf, sample_volume_oct<T>(g, p));
       ++nsamp;
       if (src.w > 0.0f) {
         ++nshade;

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      const float dens = sample_volume_quad<T>(k.g, p);
+      const float dens = sample_volume_oct<T>(k.g, p);
       float4 src = sample_tf(s_tf, ntf, dens);
       ++nsamp;
       if (src.w > 0.0f) {

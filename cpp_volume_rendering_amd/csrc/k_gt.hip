@@ -32,14 +32,14 @@ template <typename T>
 __device__ __forceinline__ float gt_ray(const VolGeom& g, const float4* __restrict__ tf, int ntf, f3 p, f3 w,
                                         float dist, float gap, float rstep) {
   float Vt = 1.0f;
-  float st0 = sample_tf(tf, ntf, sample_volume_quad<T>(g, p + w * gap)).w;
+  float st0 = sample_tf(tf, ntf, sample_volume_oct<T>(g, p + w * gap)).w;
   for (int j = 0; j < kMaxSamples; ++j) {
     const float s = gap + float(j) * rstep;
     if (!(s < dist)) break;
     const float h = fminf(rstep, dist - s);
     const f3 at = p + w * (s + h);
     if (at.x < 0.f || at.x > g.G[0] || at.y < 0.f || at.y > g.G[1] || at.z < 0.f || at.z > g.G[2]) break;
-    const float st1 = sample_tf(tf, ntf, sample_volume_quad<T>(g, at)).w;
+    const float st1 = sample_tf(tf, ntf, sample_volume_oct<T>(g, at)).w;
     Vt *= __expf(-((st0 + st1) * 0.5f) * h);
     if ((1.0f - Vt) > 0.99f) break;
     st0 = st1;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_gt_render(RayFrame rf, VolGeom g, GtK K
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      float4 src = sample_tf(s_tf, ntf, sample_volume_quad<T>(g, p));
+      float4 src = sample_tf(s_tf, ntf, sample_volume_oct<T>(g, p));
       ++nsamp;
       if (src.w > 0.0f) {
         ++nshade;

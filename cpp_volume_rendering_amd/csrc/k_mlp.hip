@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_nn_samples(VolGeom g, PyrGeom pg, const
   for (int t = 0; t < 64; ++t) {
     const f3 c = mk3(u01(s) * g.G[0], u01(s) * g.G[1], u01(s) * g.G[2]);
     p = c;
-    if (sample_tf(tf, ntf, sample_volume_quad<T>(g, c)).w > 0.f) break;
+    if (sample_tf(tf, ntf, sample_volume_oct<T>(g, c)).w > 0.f) break;
   }
   const f3 dir = unit_sphere(s);
   const f3 L = unit_sphere(s);

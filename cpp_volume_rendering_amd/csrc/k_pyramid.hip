@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_pyr_level0(PyrBuild b, VolGeom g, const
     const float px = gx + float(kx) * b.sigma, py = gy + float(ky) * b.sigma, pz = gz + float(kz) * b.sigma;
     const float tx = px / b.G[0], ty = py / b.G[1], tz = pz / b.G[2];
     if (outside01(tx, ty, tz)) continue;
-    const float d = sample_volume_quad<T>(g, mk3(px, py, pz));
+    const float d = sample_volume_oct<T>(g, mk3(px, py, pz));
     acc += s_w[i] * sample_tf(tf_rgba, ntf, d).w;
   }
   out_op[size_t(x) + size_t(b.bx) * (y + size_t(b.by) * z)] = acc / s_wsum;

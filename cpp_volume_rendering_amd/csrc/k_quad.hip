@@ -1,62 +1,64 @@
-// Neighbour-block copies of the sampled grids, so that a GL_LINEAR trilinear read is one
-// or two vector loads instead of eight scalar gathers.
-//  - Raw volume, "quad" blocks: cell (x, y, z) holds the values at (x, y), (x+1, y),
-//    (x, y+1), (x+1, y+1) of plane z (uchar4 / ushort4), the +1 neighbours clamped to
-//    the edge; a read is two loads (planes z0 and z1).
-//  - Extinction pyramid, "corner" blocks: cell (x, y, z) holds all eight values of
-//    (x..x+1, y..y+1, z..z+1), clamped likewise (32 bytes in fp32, 16 in fp16 / bf16);
-//    a read is one record on one cache line.  Twice the quad footprint, half the loads
-//    and half the cache lines per read, which is what the cone-traced renderers and the
-//    surrogate's 16 feature reads per shaded sample are bound by.
-// Both are stored in brick order (4 x 4 x 4 cells per brick, brick_index), so that the
-// samples of a wavefront, which spread over a small 3D region, touch few cache lines.
+// Corner-block copies of the sampled grids, so that a GL_LINEAR trilinear read is one
+// vector load instead of eight scalar gathers: cell (x, y, z) holds all eight values of
+// (x..x+1, y..y+1, z..z+1), the +1 neighbours clamped to the edge, plane z first.
+//  - Raw volume: 8 x u8 (8 bytes) or 8 x u16 (16 bytes) per cell.
+//  - Extinction pyramid, every level: 32 bytes in fp32, 16 in fp16 / bf16.
+// A read is one record on one cache line: eight times the raw bytes of the volume (four
+// of a 2 x 2 plane block), for half the loads and half the cache lines of plane blocks,
+// which is what the marches and the cone-traced lookups are bound by.  Both are stored
+// in brick order (4 x 4 x 4 cells per brick, brick_index), so that the samples of a
+// wavefront, which spread over a small 3D region, touch few cache lines.
 #include "cvr_ctx.hpp"
 #include <hip/hip_fp16.h>
 
 namespace cvr {
 
-template <typename T, typename Q>
-__global__ __launch_bounds__(256) void k_make_quad(const T* __restrict__ src, Q* __restrict__ dst, int nx, int ny,
-                                                   int nz) {
+// Corner blocks of the raw voxels (u8: 8 bytes, u16: 16 bytes per cell).
+template <typename T>
+__global__ __launch_bounds__(256) void k_make_vol_oct(const T* __restrict__ src, void* __restrict__ dst, int nx, int ny,
+                                                      int nz) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63);
   const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
   const int z = blockIdx.z;
   if (x >= nx || y >= ny || z >= nz) return;
-  const int x1 = min(x + 1, nx - 1), y1 = min(y + 1, ny - 1);
-  const size_t r0 = (size_t(z) * ny + y) * nx, r1 = (size_t(z) * ny + y1) * nx;
-  Q q;
-  q.x = src[r0 + x];
-  q.y = src[r0 + x1];
-  q.z = src[r1 + x];
-  q.w = src[r1 + x1];
-  dst[brick_index(x, y, z, bricks(nx), bricks(ny))] = q;
+  const int x1 = min(x + 1, nx - 1), y1 = min(y + 1, ny - 1), z1 = min(z + 1, nz - 1);
+  const size_t sy = size_t(nx), sz = size_t(nx) * ny;
+  const size_t r00 = z * sz + y * sy, r01 = z * sz + y1 * sy, r10 = z1 * sz + y * sy, r11 = z1 * sz + y1 * sy;
+  const unsigned int v[8] = {src[r00 + x], src[r00 + x1], src[r01 + x], src[r01 + x1],
+                             src[r10 + x], src[r10 + x1], src[r11 + x], src[r11 + x1]};
+  const size_t i = brick_index(x, y, z, bricks(nx), bricks(ny));
+  if (sizeof(T) == 1) {
+    static_cast<uint2*>(dst)[i] = make_uint2(v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24),
+                                             v[4] | (v[5] << 8) | (v[6] << 16) | (v[7] << 24));
+  } else {
+    static_cast<uint4*>(dst)[i] = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
+  }
 }
 
-template <typename T, typename Q>
-static int make_quad(const T* src, Q* dst, int nx, int ny, int nz, hipStream_t st) {
-  if (nz > 65535) { set_error("quad layout: depth above 65535"); return CVR_ERR_INVALID; }
+template <typename T>
+static int make_vol_oct(const T* src, void* dst, int nx, int ny, int nz, hipStream_t st) {
+  if (nz > 65535) { set_error("corner blocks: depth above 65535"); return CVR_ERR_INVALID; }
   const dim3 grid(unsigned((nx + 63) / 64), unsigned((ny + 3) / 4), unsigned(nz));
-  hipLaunchKernelGGL((k_make_quad<T, Q>), grid, dim3(256), 0, st, src, dst, nx, ny, nz);
+  hipLaunchKernelGGL((k_make_vol_oct<T>), grid, dim3(256), 0, st, src, dst, nx, ny, nz);
   CVR_HIP_CHECK(hipGetLastError());
   return CVR_OK;
 }
 
-int build_volume_quad(cvr_ctx* c, hipStream_t st) {
+int build_volume_oct(cvr_ctx* c, hipStream_t st) {
   const size_t cells = brick_cells(c->nx, c->ny, c->nz);
-  const size_t bytes = cells * (c->dtype == CVR_U8 ? sizeof(uchar4) : sizeof(ushort4));
-  if (c->d_vol_quad) CVR_HIP_CHECK(hipFree(c->d_vol_quad));
-  c->d_vol_quad = nullptr;
-  if (hipMalloc(&c->d_vol_quad, bytes) != hipSuccess) {
-    c->d_vol_quad = nullptr;
-    set_error("cvr_set_volume: out of device memory (quad layout)");
+  const size_t bytes = cells * (c->dtype == CVR_U8 ? sizeof(uint2) : sizeof(uint4));
+  if (c->d_vol_oct) CVR_HIP_CHECK(hipFree(c->d_vol_oct));
+  c->d_vol_oct = nullptr;
+  if (hipMalloc(&c->d_vol_oct, bytes) != hipSuccess) {
+    c->d_vol_oct = nullptr;
+    set_error("cvr_set_volume: out of device memory (corner blocks)");
     return CVR_ERR_NOMEM;
   }
   int rc;
   if (c->dtype == CVR_U8)
-    rc = make_quad(static_cast<const uint8_t*>(c->d_vol), static_cast<uchar4*>(c->d_vol_quad), c->nx, c->ny, c->nz, st);
+    rc = make_vol_oct(static_cast<const uint8_t*>(c->d_vol), c->d_vol_oct, c->nx, c->ny, c->nz, st);
   else
-    rc = make_quad(static_cast<const uint16_t*>(c->d_vol), static_cast<ushort4*>(c->d_vol_quad), c->nx, c->ny, c->nz,
-                   st);
+    rc = make_vol_oct(static_cast<const uint16_t*>(c->d_vol), c->d_vol_oct, c->nx, c->ny, c->nz, st);
   if (rc != CVR_OK) return rc;
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   return CVR_OK;

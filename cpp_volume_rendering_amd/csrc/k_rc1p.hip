@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
       if (!(s < D)) break;
       const float h = fminf(step, D - s);
       const f3 p = tp + dir * (s + 0.5f * h);
-      const float dens = sample_volume_quad<T>(g, p);
+      const float dens = sample_volume_oct<T>(g, p);
       float4 src = sample_tf(tfp, ntf, dens);
       ++nsamp;
       if (src.w > 0.0f) {
