@@ -206,6 +206,30 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   (void)rem;
   return xcd * per + idx;
 }
+// Workgroup index -> tile of a tiles_x x tiles_y grid, in supertiles of 8 x 8 tiles
+// (row-major supertiles, row-major tiles inside; the supertiles at the right and bottom
+// edges are narrower or shorter).  After xcd_remap, the 64 workgroups an XCD runs at
+// once then share one compact screen region, and its L2 one region of the volume.
+// A bijection onto the grid for every size.
+__device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int& tx, int& ty) {
+  constexpr int S = 8;
+  const int full_rows = tiles_y / S, rowsz = tiles_x * S;
+  int sy, r, h;
+  if (bid < full_rows * rowsz) {
+    sy = bid / rowsz;
+    r = bid - sy * rowsz;
+    h = S;
+  } else {
+    sy = full_rows;
+    r = bid - full_rows * rowsz;
+    h = tiles_y - full_rows * S;
+  }
+  const int sx = r / (S * h);
+  const int w = sx < tiles_x / S ? S : tiles_x - (tiles_x / S) * S;
+  const int q = r - sx * S * h;
+  tx = sx * S + q % w;
+  ty = sy * S + q / w;
+}
 #endif
 
 }  // namespace cvr

@@ -328,8 +328,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int tw = 1 << sc.tw_log2;
   const int tiles_x = (rf.W + 2 * tw - 1) / (2 * tw);
   unsigned int cnt[3] = {0, 0, 0};
+  const int th = (64 >> sc.tw_log2) * sc.span;
+  int tx, ty;
+  super_tile(bid, tiles_x, (rf.H + 2 * th - 1) / (2 * th), tx, ty);
   const bool marked = nn_tile<T, F, kExact>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step,
-                                            et, out, bid % tiles_x, bid / tiles_x, cnt);
+                                            et, out, tx, ty, cnt);
   if (!kExact && marked && (threadIdx.x & 63) == 0) redo[bid] = 1u;
   if (counters) {
     if (!kExact) {
