@@ -197,7 +197,7 @@ __device__ __forceinline__ float4 sample_tf(const float4* __restrict__ tf, int T
 // (workgroup b to XCD b % 8), and each XCD has its own L2 while the MALL is shared.
 // Within every run of 512 workgroups, XCD x's k-th workgroup gets index
 // 512 (k / 64) + 64 x + k % 64: each XCD works through whole 64-index groups (one
-// supertile of super_tile below, which fits the 64 workgroups an XCD holds at once),
+// supertile of super_tile below, sized for the 64 workgroups an XCD holds at once),
 // and at any moment the 8 XCDs hold 8 consecutive supertiles, i.e. one compact region
 // of the image whose part of the volume the shared MALL then serves to all of them.
 // (Giving each XCD a contiguous eighth of the image instead was 22 % slower on the
@@ -209,29 +209,32 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return ((k >> 6) << 9) | (x << 6) | (k & 63);
 }
 
-// Workgroup index -> tile of a tiles_x x tiles_y grid, in supertiles of 8 x 8 tiles
-// (row-major supertiles, row-major tiles inside; the supertiles at the right and bottom
-// edges are narrower or shorter).  After xcd_remap, the 64 workgroups an XCD runs at
-// once then share one compact screen region, and its L2 one region of the volume.
-// A bijection onto the grid for every size.
+// Workgroup index -> tile of a tiles_x x tiles_y grid, in supertiles of 4 x 16 tiles
+// (64 workgroups: row-major supertiles, row-major tiles inside; the supertiles at the
+// right and bottom edges are narrower or shorter).  After xcd_remap, the 64 workgroups
+// an XCD holds at once render one supertile, and the 8 XCDs 8 supertiles side by side:
+// for the surrogate's 8 x 32-pixel workgroups a 32 x 512-pixel strip each, 256 x 512
+// pixels together, the fastest of the shapes tried (8 x 8 supertiles in a 512 x 256
+// region: +11 %; 2 x 4 of them in a 128 x 1024 region: +39 %).  A bijection onto the
+// grid for every size.
 __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int& tx, int& ty) {
-  constexpr int S = 8;
-  const int full_rows = tiles_y / S, rowsz = tiles_x * S;
+  constexpr int SW = 4, SH = 16;
+  const int full_rows = tiles_y / SH, rowsz = tiles_x * SH;
   int sy, r, h;
   if (bid < full_rows * rowsz) {
     sy = bid / rowsz;
     r = bid - sy * rowsz;
-    h = S;
+    h = SH;
   } else {
     sy = full_rows;
     r = bid - full_rows * rowsz;
-    h = tiles_y - full_rows * S;
+    h = tiles_y - full_rows * SH;
   }
-  const int sx = r / (S * h);
-  const int w = sx < tiles_x / S ? S : tiles_x - (tiles_x / S) * S;
-  const int q = r - sx * S * h;
-  tx = sx * S + q % w;
-  ty = sy * S + q / w;
+  const int sx = r / (SW * h);
+  const int w = sx < tiles_x / SW ? SW : tiles_x - (tiles_x / SW) * SW;
+  const int q = r - sx * SW * h;
+  tx = sx * SW + q % w;
+  ty = sy * SH + q / w;
 }
 #endif
 
