@@ -217,7 +217,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 // pixels together, the fastest of the shapes tried (8 x 8 supertiles in a 512 x 256
 // region: +11 %; 2 x 4 of them in a 128 x 1024 region: +39 %).  A bijection onto the
 // grid for every size.
-__device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int& tx, int& ty) {
+__host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int& tx, int& ty) {
   constexpr int SW = 4, SH = 16;
   const int full_rows = tiles_y / SH, rowsz = tiles_x * SH;
   int sy, r, h;

@@ -224,11 +224,16 @@ int build_tables(cvr_batch* b) {
   const int tl = tw_log2_of(b->tune.tile_w), tw = 1 << tl, th = (64 >> tl) * b->tune.span;
   const int tiles_x = (b->W + 2 * tw - 1) / (2 * tw), tiles_y = (b->H + 2 * th - 1) / (2 * th);
   const int nq = b->tune.queues;
+  // Every image's tiles in supertile order (super_tile); with 8 queues (one per XCD:
+  // workgroup b starts at queue b % 8) queue q takes supertiles q, q + 8, ... of each
+  // image, so the 8 XCDs render 8 adjacent supertiles of one image at a time, as the
+  // single-frame renderers' block order does.
   std::vector<std::vector<int2>> q(nq);
   for (int j : order)
-    for (int ty = 0; ty < tiles_y; ++ty) {
-      const int band = int((long long)ty * nq / tiles_y);
-      for (int tx = 0; tx < tiles_x; ++tx) q[band].push_back(make_int2(j, ty * tiles_x + tx));
+    for (int b = 0; b < tiles_x * tiles_y; ++b) {
+      int tx, ty;
+      super_tile(b, tiles_x, tiles_y, tx, ty);
+      q[(b >> 6) % nq].push_back(make_int2(j, ty * tiles_x + tx));
     }
   std::vector<int2> items;
   std::vector<int> qoff(kQueues + 1, 0);
