@@ -207,6 +207,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_tf_rgbt) (void)hipFree(c->d_tf_rgbt);
   if (c->d_tf_rgba) (void)hipFree(c->d_tf_rgba);
   if (c->d_sat) (void)hipFree(c->d_sat);
+  if (c->d_sat_oct) (void)hipFree(c->d_sat_oct);
   if (c->d_ext_lut) (void)hipFree(c->d_ext_lut);
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->d_out) (void)hipFree(c->d_out);
@@ -314,6 +315,8 @@ int cvr_set_volume(cvr_handle h, int nx, int ny, int nz, int dtype, float sx, fl
     h->d_vol = nullptr;
     if (h->d_sat) CVR_HIP_CHECK(hipFree(h->d_sat));
     h->d_sat = nullptr;
+    if (h->d_sat_oct) CVR_HIP_CHECK(hipFree(h->d_sat_oct));
+    h->d_sat_oct = nullptr;
     if (h->d_grad) CVR_HIP_CHECK(hipFree(h->d_grad));
     h->d_grad = nullptr;
     if (hipMalloc(&h->d_vol, bytes) != hipSuccess) {

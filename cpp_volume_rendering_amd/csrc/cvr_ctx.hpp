@@ -57,6 +57,7 @@ struct cvr_ctx {
 
   // EBS summed-area table, two-float (hi, lo) per cell
   float2* d_sat = nullptr;
+  void* d_sat_oct = nullptr;  // corner blocks of the SAT, 64 B per cell (k_sat.hip)
   float* d_ext_lut = nullptr;  // per raw value extinction (GetExtN), 256 or 65536 entries
   uint64_t sat_built_for = ~0ull;
 

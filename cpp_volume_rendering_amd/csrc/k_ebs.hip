@@ -131,24 +131,34 @@ struct EbsK {
   int axis;
 };
 
-__device__ __forceinline__ double sat_cell(const float2* __restrict__ sat, size_t i) {
-  float2 v = sat[i];
-  return double(v.x) + double(v.y);
+__device__ __forceinline__ double hilo(float hi, float lo) { return double(hi) + double(lo); }
+// Below the first texel both taps of an axis clamp to cell 0 with the weight kept
+// (CLAMP_TO_EDGE); the corner block at 0 holds cells 0 and 1, so the second is
+// replaced by the first there.
+__device__ __forceinline__ void sat_axis(double x, int n, int& i0, double& f, bool& below) {
+  const double fl = floor(x);
+  f = x - fl;
+  below = fl < 0.0;
+  i0 = min(max(int(fl), 0), n - 1);
 }
-// GL_LINEAR tap of the SAT at texel coordinate (x, y, z), :74-83, in FP64.
-__device__ __forceinline__ double sat_tap(const float2* __restrict__ sat, const EbsK& k, double x, double y,
+// GL_LINEAR tap of the SAT at texel coordinate (x, y, z), :74-83, in FP64, through the
+// corner blocks (one 64-byte record).
+__device__ __forceinline__ double sat_tap(const float4* __restrict__ sat, const EbsK& k, double x, double y,
                                           double z) {
-  double fx0 = floor(x), fy0 = floor(y), fz0 = floor(z);
-  double fx = x - fx0, fy = y - fy0, fz = z - fz0;
-  int x0 = min(max(int(fx0), 0), k.w - 1), x1 = min(max(int(fx0) + 1, 0), k.w - 1);
-  int y0 = min(max(int(fy0), 0), k.h - 1), y1 = min(max(int(fy0) + 1, 0), k.h - 1);
-  int z0 = min(max(int(fz0), 0), k.d - 1), z1 = min(max(int(fz0) + 1, 0), k.d - 1);
-  const size_t sy = size_t(k.w), sz = size_t(k.w) * k.h;
-  size_t r00 = y0 * sy + z0 * sz, r10 = y1 * sy + z0 * sz, r01 = y0 * sy + z1 * sz, r11 = y1 * sy + z1 * sz;
-  double c000 = sat_cell(sat, r00 + x0), c100 = sat_cell(sat, r00 + x1);
-  double c010 = sat_cell(sat, r10 + x0), c110 = sat_cell(sat, r10 + x1);
-  double c001 = sat_cell(sat, r01 + x0), c101 = sat_cell(sat, r01 + x1);
-  double c011 = sat_cell(sat, r11 + x0), c111 = sat_cell(sat, r11 + x1);
+  int x0, y0, z0;
+  double fx, fy, fz;
+  bool bx, by, bz;
+  sat_axis(x, k.w, x0, fx, bx);
+  sat_axis(y, k.h, y0, fy, by);
+  sat_axis(z, k.d, z0, fz, bz);
+  const float4* r = sat + 4 * ((size_t(z0) * k.h + y0) * k.w + x0);
+  const float4 q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
+  // (x, y) plane z0: q0 = (x0, y0), (x1, y0); q1 = (x0, y1), (x1, y1); plane z1: q2, q3
+  double c000 = hilo(q0.x, q0.y), c100 = hilo(q0.z, q0.w), c010 = hilo(q1.x, q1.y), c110 = hilo(q1.z, q1.w);
+  double c001 = hilo(q2.x, q2.y), c101 = hilo(q2.z, q2.w), c011 = hilo(q3.x, q3.y), c111 = hilo(q3.z, q3.w);
+  if (bz) { c001 = c000; c101 = c100; c011 = c010; c111 = c110; }
+  if (by) { c010 = c000; c110 = c100; c011 = c001; c111 = c101; }
+  if (bx) { c100 = c000; c110 = c010; c101 = c001; c111 = c011; }
   double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
   double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
   double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
@@ -156,7 +166,7 @@ __device__ __forceinline__ double sat_tap(const float2* __restrict__ sat, const 
 }
 
 // EvaluateSAT3D (:87-100) after the border offset and clamp (:102-110 / :181-183).
-__device__ __forceinline__ double box_sum(const float2* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
+__device__ __forceinline__ double box_sum(const float4* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
   p1 = mk3(fminf(fmaxf(p1.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
            fminf(fmaxf(p1.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
            fminf(fmaxf(p1.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
@@ -174,7 +184,7 @@ __device__ __forceinline__ double box_sum(const float2* __restrict__ sat, const 
 }
 
 // ExtinctionAmbientOcclusion (:112-146).
-__device__ float ebs_ao(const float2* __restrict__ sat, const EbsK& k, f3 tx) {
+__device__ float ebs_ao(const float4* __restrict__ sat, const EbsK& k, f3 tx) {
   const f3 sc = ld3(k.scale);
   const double rad = k.ao_radius;
   double S0 = box_sum(sat, k, tx - sc * k.ao_radius, tx + sc * k.ao_radius);
@@ -194,7 +204,7 @@ __device__ float ebs_ao(const float2* __restrict__ sat, const EbsK& k, f3 tx) {
 __device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
 // EvaluateShadowSAT3D (:148-188) summed along the cone; directional light from the table.
-__device__ float ebs_shadow_table(const float2* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
+__device__ float ebs_shadow_table(const float4* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
                                   f3 tx, unsigned int& nbox) {
   const float pa = comp(tx, k.axis), lo = k.minVol[k.axis], hi = k.maxVol[k.axis];
   double st = 0.0;
@@ -210,7 +220,7 @@ __device__ float ebs_shadow_table(const float2* __restrict__ sat, const EbsK& k,
 }
 
 // Point light: the cone axis depends on the sample (:437-438).
-__device__ float ebs_shadow_point(const float2* __restrict__ sat, const EbsK& k, f3 tx, unsigned int& nbox) {
+__device__ float ebs_shadow_point(const float4* __restrict__ sat, const EbsK& k, f3 tx, unsigned int& nbox) {
   f3 real = tx - mk3(0.5f * k.g.G[0], 0.5f * k.g.G[1], 0.5f * k.g.G[2]);
   f3 cvv = norm3(ld3(k.light_pos) - real);
   float cv[3] = {cvv.x, cvv.y, cvv.z};
@@ -272,7 +282,7 @@ __device__ void ebs_build_rows(const EbsK& k, ShadowRow* s_rows, int* s_nrows) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
-                                              const float2* __restrict__ sat, float step, float et, PhongK ph,
+                                              const float4* __restrict__ sat, float step, float et, PhongK ph,
                                               const float4* __restrict__ grad, float4* __restrict__ out,
                                               unsigned long long* __restrict__ counters) {
   extern __shared__ float4 smem[];
@@ -356,7 +366,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
 // Object-space light cache (rc1pextbsd/lightcachecomputation.comp:440-468): cell i holds
 // (AO, shadow) at texture-space position (i + 1/2) * cell, 1 for a term that is off,
 // from the renderer's own AO and shadow evaluations.  One lane per cell.
-__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const float2* __restrict__ sat, int rx, int ry, int rz, f3 cell,
+__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const float4* __restrict__ sat, int rx, int ry, int rz, f3 cell,
                                                 float2* __restrict__ lc) {
   extern __shared__ float4 smem[];
   __shared__ int s_nrows;
@@ -448,7 +458,7 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     const f3 cell{cs[0], cs[1], cs[2]};
     const size_t n = size_t(rx) * ry * rz;
     hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 255) / 256)), dim3(256), size_t(k.nrows) * sizeof(ShadowRow), st, k,
-                       c->d_sat, rx, ry, rz, cell, c->d_lc);
+                       static_cast<const float4*>(c->d_sat_oct), rx, ry, rz, cell, c->d_lc);
     CVR_HIP_CHECK(hipGetLastError());
     return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
   }
@@ -460,10 +470,10 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_ebs<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
+    hipLaunchKernelGGL(k_ebs<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, static_cast<const float4*>(c->d_sat_oct),
                        step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   else
-    hipLaunchKernelGGL(k_ebs<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, c->d_sat,
+    hipLaunchKernelGGL(k_ebs<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, static_cast<const float4*>(c->d_sat_oct),
                        step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;

@@ -115,6 +115,28 @@ static void build_ext_lut(const cvr_ctx* c, std::vector<float>& lut) {
   }
 }
 
+// Corner blocks of the SAT: record (x, y, z) holds the 8 cells of (x..x+1, y..y+1,
+// z..z+1), the +1 neighbours clamped to the grid, as 4 float4 (two (hi, lo) cells each):
+// planes z and z+1, each (x, y), (x+1, y) then (x, y+1), (x+1, y+1).  64 bytes, one
+// cache line: a GL_LINEAR tap of the SAT is one record instead of 8 scattered reads.
+__global__ __launch_bounds__(256) void k_sat_oct(const float2* __restrict__ sat, float4* __restrict__ oct, int w, int h,
+                                                 int d) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int z = blockIdx.z;
+  if (x >= w || y >= h || z >= d) return;
+  const int x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1), z1 = min(z + 1, d - 1);
+  const size_t sy = size_t(w), sz = size_t(w) * h;
+  const size_t r00 = z * sz + y * sy, r01 = z * sz + y1 * sy, r10 = z1 * sz + y * sy, r11 = z1 * sz + y1 * sy;
+  float4* o = oct + 4 * (r00 + x);
+  const float2 a = sat[r00 + x], b = sat[r00 + x1], c2 = sat[r01 + x], e = sat[r01 + x1];
+  const float2 f = sat[r10 + x], g = sat[r10 + x1], i = sat[r11 + x], j = sat[r11 + x1];
+  o[0] = make_float4(a.x, a.y, b.x, b.y);
+  o[1] = make_float4(c2.x, c2.y, e.x, e.y);
+  o[2] = make_float4(f.x, f.y, g.x, g.y);
+  o[3] = make_float4(i.x, i.y, j.x, j.y);
+}
+
 int build_sat(cvr_ctx* c, hipStream_t st) {
   const uint64_t key = (c->vol_version << 32) ^ c->tf_version;
   if (c->d_sat && c->sat_built_for == key) return 0;
@@ -148,6 +170,18 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
   // z: lines over (x, y), stride w*h
   hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, h), dim3(256), 0, st, c->d_sat, w, h, size_t(w), d,
                      size_t(w) * h);
+  CVR_HIP_CHECK(hipGetLastError());
+  // corner blocks for the render kernels' taps
+  if (!c->d_sat_oct) {
+    if (hipMalloc(&c->d_sat_oct, cells * 4 * sizeof(float4)) != hipSuccess) {
+      c->d_sat_oct = nullptr;
+      set_error("cvr_prepare: out of device memory for the summed-area table's corner blocks");
+      return -5;
+    }
+  }
+  if (d > 65535) { set_error("summed-area table: depth above 65535"); return -4; }
+  hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned((h + 3) / 4), unsigned(d)), dim3(256), 0, st,
+                     c->d_sat, static_cast<float4*>(c->d_sat_oct), w, h, d);
   CVR_HIP_CHECK(hipGetLastError());
   // the LUT is read by pass 1 only; keep it until the stream has consumed it
   CVR_HIP_CHECK(hipStreamSynchronize(st));
