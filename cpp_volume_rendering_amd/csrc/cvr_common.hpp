@@ -209,16 +209,17 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
   return ((k >> 6) << 9) | (x << 6) | (k & 63);
 }
 
-// Workgroup index -> tile of a tiles_x x tiles_y grid, in supertiles of 4 x 16 tiles
-// (64 workgroups: row-major supertiles, row-major tiles inside; the supertiles at the
-// right and bottom edges are narrower or shorter).  After xcd_remap, the 64 workgroups
-// an XCD holds at once render one supertile, and the 8 XCDs 8 supertiles side by side:
-// for the surrogate's 8 x 32-pixel workgroups a 32 x 512-pixel strip each, 256 x 512
-// pixels together, the fastest of the shapes tried (8 x 8 supertiles in a 512 x 256
-// region: +11 %; 2 x 4 of them in a 128 x 1024 region: +39 %).  A bijection onto the
-// grid for every size.
-__host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int& tx, int& ty) {
-  constexpr int SW = 4, SH = 16;
+// Workgroup index -> tile of a tiles_x x tiles_y grid of tile_px-wide tiles, in
+// supertiles of 64 tiles that are 32 pixels wide: SW = 32 / tile_px tiles across and
+// 64 / SW down (row-major supertiles, row-major tiles inside; the supertiles at the right
+// and bottom edges are narrower or shorter).  After xcd_remap, the 64 workgroups an
+// XCD holds at once render one 32-pixel-wide strip, and the 8 XCDs 8 strips side by
+// side: a 256-pixel-wide region (512 pixels tall for 16 x 16 or 8 x 32 tiles).  That was
+// the fastest of the shapes tried on the 1024^2 frames: the same 64 tiles as 64- or
+// 128-pixel-wide supertiles made the cone tracer 40 % slower, DOS 13 %, EBS 4 %, the
+// surrogate 11 %.  A bijection onto the grid for every size.
+__host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int tile_px, int& tx, int& ty) {
+  const int SW = tile_px >= 32 ? 1 : 32 / tile_px, SH = 64 / SW;
   const int full_rows = tiles_y / SH, rowsz = tiles_x * SH;
   int sy, r, h;
   if (bid < full_rows * rowsz) {
