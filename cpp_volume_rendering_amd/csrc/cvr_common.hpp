@@ -210,16 +210,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 }
 
 // Workgroup index -> tile of a tiles_x x tiles_y grid of tile_px-wide tiles, in
-// supertiles of 64 tiles that are 32 pixels wide: SW = 32 / tile_px tiles across and
-// 64 / SW down (row-major supertiles, row-major tiles inside; the supertiles at the right
-// and bottom edges are narrower or shorter).  After xcd_remap, the 64 workgroups an
-// XCD holds at once render one 32-pixel-wide strip, and the 8 XCDs 8 strips side by
-// side: a 256-pixel-wide region (512 pixels tall for 16 x 16 or 8 x 32 tiles).  That was
-// the fastest of the shapes tried on the 1024^2 frames: the same 64 tiles as 64- or
-// 128-pixel-wide supertiles made the cone tracer 40 % slower, DOS 13 %, EBS 4 %, the
-// surrogate 11 %.  A bijection onto the grid for every size.
-__host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int tile_px, int& tx, int& ty) {
-  const int SW = tile_px >= 32 ? 1 : 32 / tile_px, SH = 64 / SW;
+// supertiles of 64 tiles that are strip_px pixels wide: SW = strip_px / tile_px tiles
+// across (at least 1) and 64 / SW down (row-major supertiles, row-major tiles inside;
+// the supertiles at the right and bottom edges are narrower or shorter).  After
+// xcd_remap, the 64 workgroups an XCD holds at once render one tall strip, and the 8
+// XCDs 8 strips side by side.  The strip width is the caller's, from measurements on
+// the 1024^2 frames: 16 pixels for the surrogate, DOS and the cone tracers (against 32:
+// 5 %, 15 % and 15 % faster), 32 for EBS and the plain march (against 16: 7 % and 2 %
+// faster); 64- or 128-pixel strips were slower everywhere (up to 40 % for the cone
+// tracer).  A bijection onto the grid for every size.
+__host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int tile_px, int strip_px, int& tx,
+                                                    int& ty) {
+  const int SW = strip_px > tile_px ? strip_px / tile_px : 1, SH = 64 / SW;
   const int full_rows = tiles_y / SH, rowsz = tiles_x * SH;
   int sy, r, h;
   if (bid < full_rows * rowsz) {

@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void k_ebs(RayFrame rf, EbsK k, const float4* 
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles_x = (rf.W + 15) >> 4;
   int tx_, ty_;
-  super_tile(bid, tiles_x, (rf.H + 15) >> 4, 16, tx_, ty_);
+  super_tile(bid, tiles_x, (rf.H + 15) >> 4, 16, 32, tx_, ty_);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = tx_ * 16 + (w & 1) * 8 + (lane & 7);
   const int py = ty_ * 16 + (w >> 1) * 8 + (lane >> 3);

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_gt_render(RayFrame rf, VolGeom g, GtK K
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles_x = (rf.W + 15) >> 4;
   int tx, ty;
-  super_tile(bid, tiles_x, (rf.H + 15) >> 4, 16, tx, ty);
+  super_tile(bid, tiles_x, (rf.H + 15) >> 4, 16, 16, tx, ty);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int px = tx * 16 + (w & 1) * 8 + (lane & 7);
   const int py = ty * 16 + (w >> 1) * 8 + (lane >> 3);

@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   unsigned int cnt[3] = {0, 0, 0};
   const int th = (64 >> sc.tw_log2) * sc.span;
   int tx, ty;
-  super_tile(bid, tiles_x, (rf.H + 2 * th - 1) / (2 * th), 2 * tw, tx, ty);
+  super_tile(bid, tiles_x, (rf.H + 2 * th - 1) / (2 * th), 2 * tw, 16, tx, ty);
   const bool marked = nn_tile<T, F, kExact>(rf, g, pg, pyr, K, nk, sc, sW, sB, sX[threadIdx.x >> 6], s_tf, ntf, step,
                                             et, out, tx, ty, cnt);
   if (!kExact && marked && (threadIdx.x & 63) == 0) redo[bid] = 1u;
