@@ -216,9 +216,9 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
 // xcd_remap, the 64 workgroups an XCD holds at once render one tall strip, and the 8
 // XCDs 8 strips side by side.  The strip width is the caller's, from measurements on
 // the 1024^2 frames: 16 pixels for the surrogate, DOS and the cone tracers (against 32:
-// 5 %, 15 % and 15 % faster), 32 for EBS and the plain march (against 16: 7 % and 2 %
-// faster); 64- or 128-pixel strips were slower everywhere (up to 40 % for the cone
-// tracer).  A bijection onto the grid for every size.
+// 5 %, 15 % and 15 % faster) and the batched surrogate (2 %), 32 for EBS and the plain
+// march (against 16: 7 % and 2 % faster); 64- or 128-pixel strips were slower everywhere
+// (up to 40 % for the cone tracer).  A bijection onto the grid for every size.
 __host__ __device__ __forceinline__ void super_tile(int bid, int tiles_x, int tiles_y, int tile_px, int strip_px, int& tx,
                                                     int& ty) {
   const int SW = strip_px > tile_px ? strip_px / tile_px : 1, SH = 64 / SW;

@@ -232,7 +232,7 @@ int build_tables(cvr_batch* b) {
   for (int j : order)
     for (int b = 0; b < tiles_x * tiles_y; ++b) {
       int tx, ty;
-      super_tile(b, tiles_x, tiles_y, 2 * tw, 32, tx, ty);
+      super_tile(b, tiles_x, tiles_y, 2 * tw, 16, tx, ty);
       q[(b >> 6) % nq].push_back(make_int2(j, ty * tiles_x + tx));
     }
   std::vector<int2> items;
