@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -182,7 +183,9 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
     for _ in range(3):
         if t >= 0.5 * budget_s or stride <= 1:
             break
-        stride = max(1, min(stride - 1, int(stride * (t / budget_s) ** 0.5)))
+        # round up: near 1 a truncated stride would overshoot the budget (stride 1.6 -> 1 is
+        # 2.6x the pixels)
+        stride = max(1, min(stride - 1, math.ceil(stride * (t / budget_s) ** 0.5)))
         t0 = time.perf_counter()
         run(stride)
         t = time.perf_counter() - t0
