@@ -169,21 +169,27 @@ __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, cons
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
 // opacities along the light (8..14), the local opacity (15).  The corner-block loads of
-// a group of kFeatGroup inputs are issued before the first of them is used (texel
+// a group of feat_group<F>() inputs are issued before the first of them is used (texel
 // weights and falloff x segment length kept in registers), so one memory round trip
 // serves the group; each read is gauss_ext_level's, with the segment length folded into
-// the falloff factor.
+// the falloff factor.  Groups of 8 suit the 32-byte fp32 records and groups of 4 the
+// 16-byte ones (512^3 surrogate frame at 1024^2: fp32 6.98 -> 6.84 ms with 8, fp16
+// 6.57 -> 6.82 ms with 8; 16 spills).  The group only orders the loads: every input is
+// computed the same way whatever its size.
 typedef const __attribute__((address_space(4))) FeatTap* ConstTaps;
 
 #ifndef CVR_FEAT_GROUP
-#define CVR_FEAT_GROUP 4
+#define CVR_FEAT_GROUP 0
 #endif
-constexpr int kFeatGroup = CVR_FEAT_GROUP;
+template <int F>
+__host__ __device__ constexpr int feat_group() { return CVR_FEAT_GROUP ? CVR_FEAT_GROUP : (F == 0 ? 8 : 4); }
 
 template <int F>
 __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const FeatTap* __restrict__ taps,
                                          f3 p, f3 k, f3 L, float* f) {
   typedef typename PyrOct<F>::type B;
+  constexpr int kFeatGroup = feat_group<F>();
+  static_assert(kNFeat % kFeatGroup == 0, "the load groups must tile the inputs");
   // the table is wave-uniform and read-only: constant address space -> scalar loads
   const ConstTaps ft = (ConstTaps)(uintptr_t)taps;
 #pragma unroll
