@@ -62,6 +62,10 @@ __device__ __forceinline__ float dos_cone(const void* __restrict__ pyr, const Py
   od[0] = 0.f;
   lt[0] = 0.f;
   int i = 0;
+  // the single-cone sections read one level each and their positions do not depend on
+  // the reads: unrolled by 4, four reads are in flight at once and the sums keep their
+  // order (256^3 @ 1024^2 with shadows: 37.7 -> 35.9 ms, the same frame bit for bit)
+#pragma unroll 4
   for (; i < C.n1; ++i) {
     const float4 s = rows[i];
     const float at = dos_ext<F>(pyr, pg, p + k * track, s.y) * s.w;
