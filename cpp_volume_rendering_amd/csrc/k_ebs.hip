@@ -460,7 +460,9 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     for (int i = 0; i < 3; ++i) cs[i] = k.scale[i] * (float(dims[i]) / float(res[i]));  // VolumeScales * (dims / res)
     const f3 cell{cs[0], cs[1], cs[2]};
     const size_t n = size_t(rx) * ry * rz;
-    hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 255) / 256)), dim3(256), size_t(k.nrows) * sizeof(ShadowRow), st, k,
+    // one wave per workgroup: a 32^3 cache is 512 waves, which then spread over every CU
+    // instead of half of them (light-cache frame 0.927 -> 0.874 ms, the same cache)
+    hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 63) / 64)), dim3(64), size_t(k.nrows) * sizeof(ShadowRow), st, k,
                        static_cast<const float4*>(c->d_sat_oct), rx, ry, rz, cell, c->d_lc);
     CVR_HIP_CHECK(hipGetLastError());
     return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
