@@ -758,6 +758,7 @@ int cvr_nn_set_weights(cvr_handle h, const float* in, size_t count) {
   for (int i = 0; i < 16; ++i) { h->nn_lo[i] = -1e30f; h->nn_hi[i] = 1e30f; }
   h->nn_report = cvr_nn_report{};
   h->nn_trained = true;
+  h->nn_version++;
   return CVR_OK;
 }
 
@@ -862,6 +863,7 @@ int cvr_nn_load(cvr_handle h, const void* in, size_t bytes) {
   if (rc) return rc;
   CVR_HIP_CHECK(hipDeviceSynchronize());
   h->nn_trained = true;
+  h->nn_version++;
   return CVR_OK;
 }
 

@@ -18,6 +18,17 @@ struct PyrGeom {
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
 };
+// Inputs a built pyramid depends on, compared field by field before reuse.
+struct PyrKey {
+  uint64_t vol_v, tf_v;
+  int base[3];
+  uint32_t sigma_bits;  // bit pattern of sigma0
+  int nlev, fmt;
+  bool operator==(const PyrKey& o) const {
+    return vol_v == o.vol_v && tf_v == o.tf_v && base[0] == o.base[0] && base[1] == o.base[1] &&
+           base[2] == o.base[2] && sigma_bits == o.sigma_bits && nlev == o.nlev && fmt == o.fmt;
+  }
+};
 // Section table of one directional-occlusion cone: rows (interval, level, d_integral,
 // amplitude), section counts with 1/3/7 Gaussians, sub-cone axes.
 struct DosTable {
@@ -65,7 +76,8 @@ struct cvr_ctx {
   float* d_pyr = nullptr;
   void* d_pyr_oct = nullptr;   // corner blocks of every level (PyrGeom::fmt, k_quad.hip)
   cvr::PyrGeom pyr{};
-  uint64_t pyr_built_for = ~0ull;
+  cvr::PyrKey pyr_key{};
+  bool pyr_valid = false;  // d_pyr / d_pyr_oct hold the pyramid of pyr_key
   void* d_cone_tab = nullptr;  // surrogate input taps + cone step schedules
 
   // directional occlusion shading: section tables (occlusion rows, then shadow rows)
@@ -84,6 +96,7 @@ struct cvr_ctx {
   void* d_nn_pack = nullptr;
   float nn_lo[16] = {0}, nn_hi[16] = {0};
   bool nn_trained = false;
+  uint64_t nn_version = 0;  // bumped whenever the weights or the input range change
   cvr_nn_report nn_report{};
 
   // volume gradient (float4 per voxel, brick order like the neighbour blocks)

@@ -108,9 +108,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 using namespace cvr;
 
+// What a batch copied out of one member handle when it built its tables: the device
+// buffers its descriptors point to and the versions of the data behind them.  A member
+// whose signature changed (a new volume, transfer function, pyramid, network or
+// parameters) makes the batch rebuild its tables and graph before the next replay, so a
+// replay never reads a freed buffer or a stale network.
+struct MemberSig {
+  const void* vol_oct;
+  const void* pyr_oct;
+  const void* tf;
+  const void* pack;
+  const void* cone_tab;
+  uint64_t vol_v, tf_v, nn_v;
+  PyrKey pyr_key;
+  int pyr_valid;
+  cvr_nn_params prm;
+};
+
 struct cvr_batch {
   int device = 0, W = 0, H = 0;
   std::vector<cvr_handle> vols;
+  std::vector<MemberSig> sigs;  // per member, as of the last table build
   std::vector<cvr_job> jobs;
   cvr_batch_tuning tune{8, 0.5f, kQueues, 2, 1, 0.0};
   // device side
@@ -165,6 +183,36 @@ void drop_graph(cvr_batch* b) {
   b->graph_out = nullptr;
 }
 
+MemberSig sig_of(const cvr_ctx* c) {
+  MemberSig m;
+  std::memset(&m, 0, sizeof(m));
+  m.vol_oct = c->d_vol_oct;
+  m.pyr_oct = c->d_pyr_oct;
+  m.tf = c->d_tf_rgbt;
+  m.pack = c->d_nn_pack;
+  m.cone_tab = c->d_cone_tab;
+  m.vol_v = c->vol_version;
+  m.tf_v = c->tf_version;
+  m.pyr_key = c->pyr_key;
+  m.pyr_valid = c->pyr_valid ? 1 : 0;
+  m.nn_v = c->nn_trained ? c->nn_version : ~0ull;
+  std::memcpy(&m.prm, &c->nn, sizeof(cvr_nn_params));
+  return m;
+}
+
+// True when some member changed since the tables were built.
+bool same_sig(const MemberSig& a, const MemberSig& b) {
+  return a.vol_oct == b.vol_oct && a.pyr_oct == b.pyr_oct && a.tf == b.tf && a.pack == b.pack &&
+         a.cone_tab == b.cone_tab && a.vol_v == b.vol_v && a.tf_v == b.tf_v && a.nn_v == b.nn_v &&
+         a.pyr_key == b.pyr_key && a.pyr_valid == b.pyr_valid && std::memcmp(&a.prm, &b.prm, sizeof(a.prm)) == 0;
+}
+bool members_changed(const cvr_batch* b) {
+  if (b->sigs.size() != b->vols.size()) return true;
+  for (size_t v = 0; v < b->vols.size(); ++v)
+    if (!same_sig(sig_of(b->vols[v]), b->sigs[v])) return true;
+  return false;
+}
+
 int tw_log2_of(int tile_w) {
   int l = 0;
   while ((1 << l) < tile_w) ++l;
@@ -178,10 +226,12 @@ int build_tables(cvr_batch* b) {
   std::vector<VolDesc> vd(nv);
   std::vector<size_t> cells(nv);
   b->max_ntf = 0;
+  b->sigs.clear();
   for (int v = 0; v < nv; ++v) {
     cvr_ctx* c = b->vols[v];
     int rc = cvr_prepare(c, nullptr);
     if (rc) return rc;
+    b->sigs.push_back(sig_of(c));
     const cvr_nn_params& N = c->nn;
     VolDesc& d = vd[v];
     d.g = geom_of(c);
@@ -396,6 +446,12 @@ int cvr_batch_render(cvr_batch_handle b, float* out, void* hip_stream) {
   if (b->jobs.empty()) { set_error("cvr_batch_render: no jobs"); return CVR_ERR_STATE; }
   Guard gd(b->device);
   int rc;
+  if (!b->tables_dirty && members_changed(b)) {
+    // a member's buffers or network changed: wait for replays that may still read the
+    // old ones before the tables are rebuilt
+    CVR_HIP_CHECK(hipStreamSynchronize(static_cast<hipStream_t>(hip_stream)));
+    b->tables_dirty = true;
+  }
   if (b->tables_dirty) {
     if ((rc = build_tables(b))) return rc;
     b->dirty = true;

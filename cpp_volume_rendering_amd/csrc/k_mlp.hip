@@ -580,6 +580,7 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   c->nn_report.steps = total;
   c->nn_report.exact_fallback = (std::max(c->nn_report.val_rmse_occ, c->nn_report.val_rmse_sdw) > N.rmse_gate) ? 1 : 0;
   c->nn_trained = true;
+  c->nn_version++;
   return 0;
 }
 

@@ -11,6 +11,7 @@
 // One lane per output voxel, 8x8x4 voxel bricks per 256-lane workgroup; the 343
 // weights are the same for every voxel and sit in LDS.
 #include "cvr_ctx.hpp"
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -130,10 +131,16 @@ static int mip_count(int bx, int by, int bz) {
 int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st) {
   if (base_x < 1 || base_y < 1 || base_z < 1) { set_error("pyramid: base resolution must be >= 1"); return -1; }
   if (!(sigma0 > 0)) { set_error("pyramid: sigma0 must be > 0"); return -1; }
-  const uint64_t key = (c->vol_version << 32) ^ (c->tf_version << 8) ^ uint64_t(base_x * 131 + base_y * 7 + base_z) ^
-                       (uint64_t(nlev) << 56) ^ uint64_t(double(sigma0) * 1e6) ^ (uint64_t(fmt) << 60);
+  PyrKey key{};
+  key.vol_v = c->vol_version;
+  key.tf_v = c->tf_version;
+  key.base[0] = base_x; key.base[1] = base_y; key.base[2] = base_z;
+  std::memcpy(&key.sigma_bits, &sigma0, 4);
+  key.nlev = nlev;
+  key.fmt = fmt;
   if (fmt < 0 || fmt > 2) { set_error("pyramid: density_precision must be 0 (fp32), 1 (fp16) or 2 (bf16)"); return -1; }
-  if (c->d_pyr && c->pyr_built_for == key) return 0;
+  if (c->d_pyr && c->pyr_valid && c->pyr_key == key) return 0;
+  c->pyr_valid = false;
   const int full = mip_count(base_x, base_y, base_z);
   const int L = nlev > 0 ? std::min(nlev, full) : full;
   if (L > kMaxLevels) { set_error("pyramid: too many levels"); return -4; }
@@ -189,11 +196,9 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   CVR_HIP_CHECK(hipStreamSynchronize(st));
   CVR_HIP_CHECK(hipFree(d_op));
   c->pyr = pg;
-  if (build_pyramid_oct(c, btotal, fmt, st) != CVR_OK) {
-    c->pyr_built_for = ~0ull;
-    return -5;
-  }
-  c->pyr_built_for = key;
+  if (build_pyramid_oct(c, btotal, fmt, st) != CVR_OK) return -5;
+  c->pyr_key = key;
+  c->pyr_valid = true;
   return 0;
 }
 

@@ -226,6 +226,38 @@ def _ptr(a) -> int:
     return int(a.data_ptr())  # torch tensor
 
 
+def _out_arg(out, shape, device=None):
+    """Checks an output buffer before its address goes to the C ABI: float32,
+    C-contiguous, exactly `shape`; a numpy array is host memory, a torch tensor must live
+    on a GPU (on `device` when given).  Returns (pointer, is_device)."""
+    shape = tuple(int(x) for x in shape)
+    if isinstance(out, np.ndarray):
+        if out.dtype != np.float32 or not out.flags["C_CONTIGUOUS"] or out.shape != shape:
+            raise CvrError(ERR_INVALID, f"out must be a C-contiguous float32 array of shape {shape}")
+        return out.ctypes.data, False
+    try:
+        import torch
+    except ImportError as e:  # pragma: no cover - torch is how device buffers are passed
+        raise CvrError(ERR_INVALID, "out must be a numpy array or a torch tensor") from e
+    if not isinstance(out, torch.Tensor):
+        raise CvrError(ERR_INVALID, "out must be a numpy array or a torch tensor")
+    if out.dtype != torch.float32 or not out.is_contiguous() or tuple(out.shape) != shape:
+        raise CvrError(ERR_INVALID, f"out must be a contiguous float32 tensor of shape {shape}")
+    if not out.is_cuda:
+        raise CvrError(ERR_INVALID, "out tensor must be on a GPU (use a numpy array for host output)")
+    if device is not None and out.device.index != device:
+        raise CvrError(ERR_INVALID, f"out tensor is on cuda:{out.device.index}, the handle on device {device}")
+    return int(out.data_ptr()), True
+
+
+def _volume_dtype(dt_name: str) -> int:
+    if dt_name in ("uint8", "torch.uint8"):
+        return U8
+    if dt_name in ("uint16", "torch.uint16"):
+        return U16
+    raise CvrError(ERR_INVALID, f"volume dtype must be uint8 or uint16, not {dt_name}")
+
+
 def synth_blobs(shape, dtype=U8, n_blobs=8, seed=1, noise_weight=0.0, noise_octaves=4) -> np.ndarray:
     nx, ny, nz = shape
     out = np.empty((nz, ny, nx), dtype=np.uint8 if dtype == U8 else np.uint16)
@@ -247,6 +279,7 @@ class Renderer:
 
     def __init__(self, technique: int, device: int = 0):
         self.technique = technique
+        self.device = device
         h = C.c_void_p()
         check(lib().cvr_create(technique, device, C.byref(h)))
         self.h = h
@@ -264,14 +297,17 @@ class Renderer:
 
     def set_volume(self, vol, scale=(1.0, 1.0, 1.0)):
         """vol: numpy (nz, ny, nx) uint8/uint16 on the host, or a torch tensor on the device."""
+        if vol.ndim != 3:
+            raise CvrError(ERR_INVALID, "volume must be 3-dimensional (nz, ny, nx)")
+        nz, ny, nx = vol.shape
         if isinstance(vol, np.ndarray):
-            nz, ny, nx = vol.shape
-            dt = U8 if vol.dtype == np.uint8 else U16
+            dt = _volume_dtype(str(vol.dtype))
             vol = np.ascontiguousarray(vol)
             check(lib().cvr_set_volume(self.h, nx, ny, nz, dt, *scale, vol.ctypes.data, 0))
         else:
-            nz, ny, nx = vol.shape
-            dt = U8 if str(vol.dtype) == "torch.uint8" else U16
+            dt = _volume_dtype(str(vol.dtype))
+            if not vol.is_cuda or not vol.is_contiguous():
+                raise CvrError(ERR_INVALID, "a torch volume must be a contiguous GPU tensor")
             check(lib().cvr_set_volume(self.h, nx, ny, nz, dt, *scale, vol.data_ptr(), 1))
 
     def set_transfer_function(self, rgb_iso, alpha_iso, max_density=255, extinction_input=False):
@@ -304,9 +340,9 @@ class Renderer:
         """Render to a new host array (H, W, 4) float32, or into `out` (numpy or device torch tensor)."""
         if out is None:
             out = np.zeros((height, width, 4), dtype=np.float32)
-        is_dev = not isinstance(out, np.ndarray)
+        ptr, is_dev = _out_arg(out, (height, width, 4), self.device)
         check(lib().cvr_render(self.h, C.byref(cam), C.byref(light) if light is not None else None, width, height,
-                               _ptr(out), int(is_dev), stream))
+                               ptr, int(is_dev), stream))
         return out
 
     def gradient(self, shape, kind=GRADIENT_SOBEL) -> np.ndarray:
@@ -410,6 +446,15 @@ class Renderer:
     def nn_load(self, blob: bytes):
         check(lib().cvr_nn_load(self.h, blob, len(blob)))
 
+    def nn_range(self):
+        """The training-set input range (lo[16], hi[16]) stored in the checkpoint: the
+        renderer's exact tracer takes every sample with an input outside
+        [lo - ood_margin, hi + ood_margin]."""
+        blob = self.nn_save()
+        o = 16 + 3 * lib().cvr_nn_param_count() * 4
+        r = np.frombuffer(blob, np.float32, 32, o)
+        return r[:16].copy(), r[16:].copy()
+
     def nn_report(self) -> NnReport:
         r = NnReport()
         check(lib().cvr_nn_get_report(self.h, C.byref(r)))
@@ -430,7 +475,7 @@ class Batch:
     def __init__(self, width: int, height: int, device: int = 0):
         h = C.c_void_p()
         check(lib().cvr_batch_create(device, width, height, C.byref(h)))
-        self.h, self.width, self.height = h, width, height
+        self.h, self.width, self.height, self.device = h, width, height, device
         self._refs = []  # the renderers must outlive the batch
         self.n_jobs = 0
 
@@ -471,14 +516,22 @@ class Batch:
         check(lib().cvr_batch_set_tuning(self.h, C.byref(t)))
         return t
 
+    def _dev_out(self, out):
+        # with no jobs set the C ABI reports the state error, whatever the leading size
+        n = self.n_jobs if self.n_jobs else max(1, int(out.shape[0]))
+        ptr, is_dev = _out_arg(out, (n, self.height, self.width, 4), self.device)
+        if not is_dev:
+            raise CvrError(ERR_INVALID, "batch output must be a GPU tensor")
+        return ptr
+
     def autotune(self, out, stride=1, stream=None) -> BatchTuning:
         t = BatchTuning()
-        check(lib().cvr_batch_autotune(self.h, _ptr(out), stride, stream, C.byref(t)))
+        check(lib().cvr_batch_autotune(self.h, self._dev_out(out), stride, stream, C.byref(t)))
         return t
 
     def render(self, out, stream=None):
         """out: device tensor (n_jobs, height, width, 4) float32."""
-        check(lib().cvr_batch_render(self.h, _ptr(out), stream))
+        check(lib().cvr_batch_render(self.h, self._dev_out(out), stream))
         return out
 
     def stats(self) -> Stats:

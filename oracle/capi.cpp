@@ -538,5 +538,20 @@ void orc_mlp_forward(const double* prm, int n, const double* X, double* Y) {
 double orc_mlp_loss_grad(const double* prm, const double* X, const double* T, int n, double* grad) {
   return mlp_loss_grad(prm, X, T, n, grad);
 }
+void orc_mlp_forward_bf16(const double* prm, int n, const double* X, double* Y) {
+  for (int i = 0; i < n; ++i) mlp_forward_bf16(prm, X + size_t(i) * kNFeat, Y + size_t(i) * kOut);
+}
+// The surrogate-shaded frame (render_nn): prm[kNParams], lo/hi[16] (the range outside
+// which the exact tracer shades), bf16 selects the network's rounding points;
+// counts[2] receives (shaded samples, exact-tracer samples).
+void orc_render_nn_f64(int nx, int ny, int nz, int dtype, const void* data, double sx, double sy, double sz, void* tf,
+                       void* pyr, const ConeC* c, const double* prm, const double* lo, const double* hi, int bf16,
+                       const float* cam, int W, int H, double step, double et, double* out, double* margin,
+                       int nthreads, int stride, long long* counts) {
+  threads(nthreads);
+  NnNet net{prm, lo, hi, bf16};
+  render_nn<double>(mkvol(nx, ny, nz, dtype, data, sx, sy, sz), *static_cast<TF*>(tf), *static_cast<Pyramid*>(pyr),
+                    cone_from(c), net, mkcam(cam), W, H, step, et, out, margin, stride, counts);
+}
 
 }  // extern "C"

@@ -371,4 +371,109 @@ inline double mlp_loss_grad(const double* prm, const double* X, const double* T,
   return loss;
 }
 
+// bf16 round to nearest even of an fp32 value (the GPU rounds fp32 sums and inputs).
+inline double bf16_round(double x) {
+  float f = float(x);
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  std::memcpy(&f, &u, 4);
+  return double(f);
+}
+
+// Forward with the rounding points of a bf16 network with wide accumulation: the
+// inputs, the weights and every hidden activation rounded to bf16 (after the ReLU; the
+// order does not matter, bf16(max(x, 0)) = max(bf16(x), 0)), biases as given, products
+// summed in FP64.  The output layer stays unrounded.
+inline void mlp_forward_bf16(const double* prm, const double* x, double* y) {
+  MlpLayout L;
+  std::vector<double> a(kNFeat), b;
+  for (int i = 0; i < kNFeat; ++i) a[i] = bf16_round(x[i]);
+  for (int l = 0; l < 4; ++l) {
+    b.assign(L.out[l], 0.0);
+    for (int o = 0; o < L.out[l]; ++o) {
+      double s = prm[L.off_b[l] + o];
+      for (int i = 0; i < L.in[l]; ++i) s += bf16_round(prm[L.off_w[l] + o * L.in[l] + i]) * a[i];
+      b[o] = (l < 3) ? bf16_round(std::max(s, 0.0)) : s;
+    }
+    a.swap(b);
+  }
+  for (int o = 0; o < kOut; ++o) y[o] = a[o];
+}
+
+// ---------------------------------------------------------------- surrogate-shaded frame
+// The learned renderer's frame: the plain march of render_cone, where each opaque
+// sample's (occlusion, shadow) is the network's residual added to the network-free
+// estimate of its 16 inputs (nn_features, nn_baseline), clamped to [0, 1], and shades
+// the sample as rc1pdosct/ray_bbox_marching.comp:607-656 does with the exact cone
+// integrals: rgb <- rgb (Io ka + Is kd) / (ka + kd) (ka, kd zero for a term that is
+// off).  A sample whose inputs leave [lo, hi] (the training range widened by the
+// margin) takes the exact tracer's answers instead (ConeTracer).  `bf16` selects the
+// network's rounding points (mlp_forward_bf16) or plain FP64 (mlp_forward).  counts[0]
+// receives the shaded samples, counts[1] the samples the exact tracer shaded.
+struct NnNet {
+  const double* prm;  // kNParams, flattened as mlp_forward reads them
+  const double* lo;   // kNFeat
+  const double* hi;   // kNFeat
+  int bf16;
+};
+
+template <class R>
+inline void render_nn(const Volume& vol, const TF& tf, const Pyramid& P, const ConeParams& cp, const NnNet& net,
+                      const CamIn& cam, int W, int H, R step, R et, R* out, double* margin, int stride,
+                      long long* counts) {
+  ConeTracer<R> ct(P, cp);
+  CamBasis<R> cb = cam_basis<R>(cam, W, H);
+  V3<R> G(R(vol.nx * vol.sx), R(vol.ny * vol.sy), R(vol.nz * vol.sz));
+  V3<R> L = normalize(V3<R>(R(cp.light_fwd[0]), R(cp.light_fwd[1]), R(cp.light_fwd[2])));
+  const double ka = cp.apply_occ ? cp.ka : 0.0, kd = cp.apply_sdw ? cp.kd : 0.0;
+  long long n_shaded = 0, n_exact = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : n_shaded, n_exact)
+  for (int py = 0; py < H; ++py)
+    for (int px = 0; px < W; ++px) {
+      R* o = out + (size_t(py) * W + px) * 4;
+      o[0] = o[1] = o[2] = o[3] = 0;
+      if (px % stride || py % stride) continue;
+      Margin mg;
+      V3<R> d = ray_dir(cb, px, py, W, H);
+      R tn, tf_;
+      if (ray_aabb(cb.eye, d, G, tn, tf_)) {
+        R D = std::fabs(tf_ - tn);
+        V3<R> tp = cb.eye + d * tn + G * R(0.5);
+        V3<R> k, up, right;
+        occ_frame(d, k, up, right);
+        auto sh = [&](const V3<R>& p, R* src) {
+          R fr[kNFeat];
+          nn_features<R>(P, cp, p, k, L, fr);
+          double f[kNFeat], y[kOut], bo, bs;
+          bool ood = false;
+          for (int c = 0; c < kNFeat; ++c) {
+            f[c] = double(fr[c]);
+            ood |= (f[c] < net.lo[c]) | (f[c] > net.hi[c]);
+          }
+          double Io, Is;
+          ++n_shaded;
+          if (ood) {
+            ++n_exact;
+            Io = cp.apply_occ ? double(ct.occlusion(p, k, up, right)) : 0.0;
+            Is = cp.apply_sdw ? double(ct.shadow(p)) : 0.0;
+          } else {
+            if (net.bf16) mlp_forward_bf16(net.prm, f, y);
+            else mlp_forward(net.prm, f, y);
+            nn_baseline(cp, f, bo, bs);
+            Io = y[0] + bo;
+            Is = y[1] + bs;
+          }
+          Io = std::min(std::max(Io, 0.0), 1.0);
+          Is = std::min(std::max(Is, 0.0), 1.0);
+          const R m = R((Io * ka + Is * kd) / (ka + kd));
+          for (int c = 0; c < 3; ++c) src[c] = src[c] * m;
+        };
+        march(tp, d, D, step, et, G, vol, tf.rgbt.data(), tf.size(), sh, o, &mg);
+      }
+      if (margin) { margin[(size_t(py) * W + px) * 2] = mg.a; margin[(size_t(py) * W + px) * 2 + 1] = mg.g; }
+    }
+  if (counts) { counts[0] = n_shaded; counts[1] = n_exact; }
+}
+
 }  // namespace orc

@@ -89,6 +89,9 @@ def lib():
         "orc_mlp_nparams": (i, []),
         "orc_mlp_forward": (None, [vp, i, vp, vp]),
         "orc_mlp_loss_grad": (d, [vp, vp, vp, i, vp]),
+        "orc_mlp_forward_bf16": (None, [vp, i, vp, vp]),
+        "orc_render_nn_f64": (None, [i, i, i, i, vp, d, d, d, vp, vp, vp, vp, vp, vp, i, vp, i, i, d, d, vp, vp, i, i,
+                                     vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -364,6 +367,37 @@ def mlp_loss_grad(prm, X, T):
     g = np.zeros_like(prm)
     loss = lib().orc_mlp_loss_grad(prm.ctypes.data, X.ctypes.data, T.ctypes.data, len(X), g.ctypes.data)
     return loss, g
+
+
+def mlp_forward_bf16(prm, X):
+    """FP64 forward with a bf16 network's rounding points (inputs, weights and hidden
+    activations rounded to bf16, sums in FP64)."""
+    prm = np.ascontiguousarray(prm, np.float64)
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 16)
+    Y = np.zeros((len(X), 16))
+    lib().orc_mlp_forward_bf16(prm.ctypes.data, len(X), X.ctypes.data, Y.ctypes.data)
+    return Y
+
+
+def render_nn(vol, tf: OTF, pyr: OPyr, prm: ConeC, weights, cam, W, H, lo=None, hi=None, bf16=True, step=0.5,
+              et=0.99, scale=(1, 1, 1), threads=0, stride=1):
+    """The surrogate-shaded frame in FP64: network residual over the network-free
+    estimate, clamped, shading the march like the exact cone integrals; samples whose
+    inputs leave [lo, hi] take the exact tracer.  Returns (image, margin, counts) with
+    counts = (shaded samples, exact-tracer samples)."""
+    vol, va = _vol_args(vol, scale)
+    c = cam_array(cam)
+    w = np.ascontiguousarray(weights, np.float64)
+    assert w.size == lib().orc_mlp_nparams()
+    lo = np.full(16, -1e300) if lo is None else np.ascontiguousarray(lo, np.float64)
+    hi = np.full(16, 1e300) if hi is None else np.ascontiguousarray(hi, np.float64)
+    out = np.zeros((H, W, 4), np.float64)
+    margin = np.zeros((H, W, 2), np.float64)
+    counts = np.zeros(2, np.int64)
+    lib().orc_render_nn_f64(*va, tf.h, pyr.h, C.byref(prm), w.ctypes.data, lo.ctypes.data, hi.ctypes.data, int(bf16),
+                            c.ctypes.data, W, H, step, et, out.ctypes.data, margin.ctypes.data, threads, stride,
+                            counts.ctypes.data)
+    return out, margin, counts
 
 
 # ---------------------------------------------------------------------------- section tables (DOS)

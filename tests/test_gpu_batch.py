@@ -133,3 +133,97 @@ def test_batch_errors(scene):
     out = torch.zeros((1, HT, WD, 4), device="cuda")
     with pytest.raises(N.CvrError):
         b.render(out)  # no jobs yet
+
+
+def test_member_change_rebuilds_the_batch():
+    """Changing a member handle between two batch renders (a new volume of another size,
+    which frees and reallocates its device buffers and retrains its network; then a new
+    transfer function) must not replay the old graph against freed memory: the batch
+    picks up the change and still equals the single-frame renderer."""
+    vols = [N.synth_blobs((32, 32, 32), N.U8, n_blobs=6, seed=61), N.synth_blobs((24, 28, 20), N.U8, n_blobs=5, seed=62)]
+    rs = []
+    for i, v in enumerate(vols):
+        rgb, a = W.seeded_tf(70 + i, 4)
+        rs.append(surrogate(v, rgb, a, 80 + i))
+    grid = (32.0, 32.0, 32.0)
+    jobs = [(v, W.orbit_camera(grid, WD, HT, az, 25.0), W.light_from_direction((-0.45, 0.75, 0.5), grid))
+            for v in range(2) for az in (0.0, 120.0)]
+    b = make_batch(rs, jobs)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    b.render(out)
+    torch.cuda.synchronize()
+    # a new volume for member 0: its corner blocks, pyramid and network are rebuilt
+    rs[0].set_volume(N.synth_blobs((40, 36, 44), N.U8, n_blobs=9, seed=63))
+    rs[0].prepare()
+    for step in range(2):
+        out.zero_()
+        b.render(out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        ref = np.stack([rs[v].render(cam, WD, HT, light=l) for v, cam, l in jobs])
+        assert float(np.abs(got - ref).max()) <= EAGER_TOL, step
+        assert ref[..., 3].max() > 0.05
+        rgb, a = W.seeded_tf(90, 6)
+        rs[1].set_transfer_function(rgb, a)  # a new transfer function (and network) for the second round
+
+
+# ---------------------------------------------------------------------------- the full scene
+import oracle_lib as O  # noqa: E402
+import surrogate_oracle as SO  # noqa: E402
+
+FULL = 512
+
+
+@pytest.fixture(scope="module")
+def full_scene():
+    """The batched config at full size: 128^3 u8, 192x256x160 u16, 384^3 u8 and 512^3 u16
+    volumes, each with its own seeded TF and a surrogate trained on 2^22 exact-tracer
+    samples; 16 views x 4 lights per volume, 256 images of 512^2 in one launch."""
+    wls, jobs = W.batch_scene()
+    rs = []
+    for i, wl in enumerate(wls):
+        r = N.Renderer(N.CONE_NN)
+        r.set_volume(wl.volume, wl.scale)
+        r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        r.prepare()
+        rs.append(r)
+    b = N.Batch(FULL, FULL)
+    for r in rs:
+        b.add_volume(r)
+    b.set_jobs(jobs)
+    out = torch.zeros((len(jobs), FULL, FULL, 4), device="cuda")
+    b.render(out)
+    torch.cuda.synchronize()
+    return wls, rs, jobs, out
+
+
+def test_full_batch_equals_single_frames(full_scene):
+    """Every 8th of the 256 images against the single-frame renderer of its handle."""
+    wls, rs, jobs, out = full_scene
+    assert len(jobs) == 256 and out.shape == (256, FULL, FULL, 4)
+    worst = 0.0
+    for j in range(0, len(jobs), 8):
+        v, cam, light = jobs[j]
+        ref = rs[v].render(cam, FULL, FULL, light=light)
+        got = out[j].cpu().numpy()
+        err = float(np.abs(got - ref).max())
+        worst = max(worst, err)
+        assert err <= EAGER_TOL, (j, err)
+    print("max-abs vs single frames over 32 images", worst)
+    assert out[..., 3].amax().item() > 0.2
+
+
+def test_full_batch_one_job_per_volume_matches_fp64_oracle(full_scene):
+    """One image per volume (a different view and light each) against the FP64 surrogate
+    frame of its handle on every 16th pixel."""
+    wls, rs, jobs, out = full_scene
+    for v in range(len(wls)):
+        j = [i for i, jb in enumerate(jobs) if jb[0] == v][5 + 3 * v]
+        _, cam, light = jobs[j]
+        img = out[j].cpu().numpy()
+        ref, m, cnt = SO.nn_oracle_frame(wls[v], rs[v], cam, light, FULL, FULL, stride=16)
+        s = O.compare(img, ref, m, 2e-3, 16)
+        print(wls[v].name, "job", j, s, "oracle exact-tracer samples", cnt[1], "of", cnt[0])
+        assert ref[..., 3].max() > 0.1
+        assert s["max_abs"] <= 2e-3 or s["n_over_tol_nonstraddle"] <= 0.005 * s["n_pixels"], s
+        assert s["n_over_tol"] <= 0.005 * s["n_pixels"] + 1, s

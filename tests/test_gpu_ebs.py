@@ -107,3 +107,12 @@ def test_ebs_full_volume_sparse_pixels():
     wl = W.banded_noise(n=256, width=256, height=256)
     img, ref, m = ebs_pair(wl, 256, 256, stride=8, shadow_type=1)
     assert_parity(img, ref, m, stride=8)
+
+
+def test_ebs_full_config_1024():
+    """The EBS config at its own frame size: 256^3 uint16 banded noise, directional light,
+    1024^2; the oracle renders every 32nd pixel (x and y) on its FP64 SAT."""
+    wl = W.banded_noise(n=256, width=1024, height=1024)
+    img, ref, m = ebs_pair(wl, 1024, 1024, stride=32, shadow_type=1)
+    assert ref[..., 3].max() > 0.2
+    assert_parity(img, ref, m, stride=32)

@@ -202,3 +202,116 @@ def test_checkpoint_restores_the_renderer_exactly():
     assert np.array_equal(r2.nn_weights(), r.nn_weights())
     with pytest.raises(N.CvrError):
         r2.nn_load(b"X" * len(blob))
+
+
+# ---------------------------------------------------------------------------- frame parity
+import surrogate_oracle as SO  # noqa: E402
+
+
+def assert_frame_parity(img, ref, m, tol, stride, max_over_frac):
+    s = O.compare(img, ref, m, tol, stride)
+    print(s)
+    assert s["max_abs"] <= tol or s["n_over_tol_nonstraddle"] <= max_over_frac * s["n_pixels"], s
+    assert s["n_over_tol"] <= max_over_frac * s["n_pixels"] + 1, s
+    return s
+
+
+def test_surrogate_frame_matches_fp64_oracle():
+    """The surrogate renderer's whole frame against the FP64 surrogate frame on the same
+    pyramid, weights and input range: with the network's bf16 rounding points in the
+    oracle the frames agree within 2e-3 per channel; against the plain FP64 network (no
+    bf16 rounding) within 1e-2 (the bf16 network's own error)."""
+    wl, r, p = train_small()
+    cam, light = wl.camera(), wl.light()
+    r.collect_stats(True)
+    img = r.render(cam, 96, 96, light=light)
+    st = r.stats()
+    ref, m, cnt = SO.nn_oracle_frame(wl, r, cam, light, 96, 96)
+    print("shaded", st.shaded_samples, cnt[0], "exact", st.fallback_samples, cnt[1])
+    assert ref[..., 3].max() > 0.2
+    assert abs(int(cnt[0]) - int(st.shaded_samples)) <= 0.002 * st.shaded_samples
+    assert_frame_parity(img, ref, m, 2e-3, 1, 0.005)
+    ref64, m64, _ = SO.nn_oracle_frame(wl, r, cam, light, 96, 96, bf16=False)
+    assert_frame_parity(img, ref64, m64, 1e-2, 1, 0.005)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_surrogate_frame_16bit_pyramid_matches_fp64_oracle(prec):
+    """fp16 / bf16 pyramid storage: the oracle reads the same rounded values."""
+    wl = W.cone_blobs(n=48, width=64, height=64, seed=22)
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(wl.volume)
+    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    p = r.params()
+    p.n_train, p.n_val, p.epochs, p.batch = 1 << 17, 1 << 13, 2, 4096
+    p.cone.n_levels, p.cone.density_precision = 6, prec
+    r.set_params(p)
+    r.nn_train()
+    cam, light = wl.camera(), wl.light()
+    img = r.render(cam, 64, 64, light=light)
+    ref, m, _ = SO.nn_oracle_frame(wl, r, cam, light, 64, 64)
+    assert_frame_parity(img, ref, m, 2e-3, 1, 0.005)
+
+
+@pytest.fixture(scope="module")
+def cfg4():
+    """The surrogate config at full size: 512^3 u8 (64 blobs + 4-octave noise, seed 4),
+    1024^2, 7-level pyramid, network trained on 2^22 exact-tracer samples."""
+    wl = W.surrogate_blobs_noise()
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(wl.volume)
+    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    r.prepare()
+    return wl, r
+
+
+def test_cfg4_pyramid_sampled_voxels(cfg4):
+    wl, r = cfg4
+    levels = r.pyramid()
+    assert [a.shape for a in levels] == [(512 >> i,) * 3 for i in range(7)]
+    SO.check_pyramid_sparse(wl, levels, (512, 512, 512), np.random.default_rng(7))
+
+
+def test_cfg4_surrogate_frame_matches_fp64_oracle(cfg4):
+    """512^3 at 1024^2: the HIP surrogate frame against the FP64 surrogate frame on every
+    32nd pixel (x and y), and the sample counts of the march."""
+    wl, r = cfg4
+    assert r.nn_report().exact_fallback == 0
+    cam, light = wl.camera(), wl.light()
+    r.collect_stats(True)
+    img = r.render(cam, 1024, 1024, light=light)
+    st = r.stats()
+    print("samples", st.samples, "shaded", st.shaded_samples, "exact", st.fallback_samples)
+    assert st.fallback_samples <= 0.01 * st.shaded_samples
+    ref, m, cnt = SO.nn_oracle_frame(wl, r, cam, light, 1024, 1024, stride=32)
+    assert ref[..., 3].max() > 0.2
+    assert_frame_parity(img, ref, m, 2e-3, 32, 0.005)
+
+
+def test_cfg4_image_error_against_exact_tracer(cfg4):
+    """The acceptance figure of the surrogate: image RMSE <= 0.01 against the exact cone
+    tracer at 512^3 and 1024^2, over the whole frame and over RGB of covered pixels; and
+    what the trained network gains over the network-free estimate of the same inputs
+    (the same renderer with a zero network) on that frame."""
+    wl, r = cfg4
+    cam, light = wl.camera(), wl.light()
+    img = r.render(cam, 1024, 1024, light=light)
+    ct = N.Renderer(N.CONE_TRACE)
+    ct.set_volume(wl.volume)
+    ct.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    ct.set_params(r.params().cone)
+    ref = ct.render(cam, 1024, 1024, light=light)
+    whole, rgb, ncov = SO.image_rmse(img, ref)
+    print("image RMSE vs exact: whole frame", whole, "RGB over", ncov, "covered pixels", rgb)
+    assert whole <= 0.01 and rgb <= 0.01
+    assert ncov > 0.2 * 1024 * 1024
+    blob = r.nn_save()
+    try:
+        r.nn_set_weights(np.zeros(N.lib().cvr_nn_param_count(), np.float32))
+        base = r.render(cam, 1024, 1024, light=light)
+    finally:
+        r.nn_load(blob)
+    bw, brgb, _ = SO.image_rmse(base, ref)
+    print("network-free estimate: whole", bw, "RGB covered", brgb)
+    assert rgb < 0.7 * brgb
+    assert np.array_equal(r.render(cam, 1024, 1024, light=light), img)  # the checkpoint restored the network
