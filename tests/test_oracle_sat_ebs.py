@@ -105,3 +105,34 @@ def test_ebs_shadow_axis_selection_symmetry():
     _, a = O.ebs_point((n, n, n), (1, 1, 1), s1, p1, (11.3, 14.2, 6.7))
     _, b = O.ebs_point((n, n, n), (1, 1, 1), s2, p2, (6.7, 14.2, 11.3))
     assert a == pytest.approx(b, rel=1e-9)
+
+
+@pytest.mark.parametrize("light", [("dir", (0.15, 1.0, -0.3)), ("dir", (-0.2, -1.0, 0.25)), ("point", (9.0, 75.0, 12.0)),
+                                   ("point", (21.0, -40.0, 5.0))])
+def test_ebs_shadow_y_axis_equals_permuted_z_axis(light):
+    """A y-dominant light (the ConeYAxis branch) on a volume gives the shadow that the
+    z-dominant light of the y <-> z permutation (the ConeZAxis branch, pinned by the
+    closed form above) gives on the permuted volume at the permuted point; directional
+    and point lights, both signs."""
+    n = (26, 30, 22)  # nx, ny, nz: unequal sizes catch a dimension taken from the wrong axis
+    rng = np.random.default_rng(17)
+    base = rng.integers(0, 256, (n[2], n[1], n[0]), dtype=np.uint8)
+    tf = O.OTF([[1, 1, 1, 0], [1, 1, 1, 255]], [[0.0, 0], [0.25, 255]])
+    perm = np.ascontiguousarray(base.transpose(1, 0, 2))  # (z, y, x) -> (y, z, x): y and z swapped
+    s1, s2 = O.sat(base, tf), O.sat(perm, tf)
+    kind, v = light
+    sw = (v[0], v[2], v[1])
+    if kind == "dir":
+        p1 = O.ebs_params(apply_occlusion=0, shadow_type=1, light_fwd=v, sdw_max_dist=80)
+        p2 = O.ebs_params(apply_occlusion=0, shadow_type=1, light_fwd=sw, sdw_max_dist=80)
+    else:
+        # light positions are world coordinates, centred on the box
+        p1 = O.ebs_params(apply_occlusion=0, shadow_type=0, light_pos=v, sdw_max_dist=80)
+        p2 = O.ebs_params(apply_occlusion=0, shadow_type=0, light_pos=sw, sdw_max_dist=80)
+    shadows = []
+    for pos in ((11.3, 14.2, 6.7), (4.6, 22.9, 17.2), (20.1, 8.4, 11.0)):
+        _, a = O.ebs_point(n, (1, 1, 1), s1, p1, pos)
+        _, b = O.ebs_point((n[0], n[2], n[1]), (1, 1, 1), s2, p2, (pos[0], pos[2], pos[1]))
+        assert a == pytest.approx(b, rel=1e-9), pos
+        shadows.append(a)
+    assert min(shadows) < 0.95  # the medium casts a shadow at some of the points
