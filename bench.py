@@ -98,6 +98,67 @@ def gather_bytes(tech, st, r, bpv):
     return vol
 
 
+# Working sets of the gather-peak probe: each fits one level of the memory hierarchy
+# (L1 per CU, L2 per XCD (4 MB), the shared 256 MB MALL) or none (HBM only).
+PROBE_SETS = (("l1", 16 << 10), ("l2", 2 << 20), ("mall", 64 << 20), ("hbm", 4 << 30))
+LEVEL_CAPACITY = (("l2", 4 << 20), ("mall", 256 << 20), ("hbm", 1 << 62))
+FASTER = {"hbm": "mall", "mall": "l2", "l2": "l1"}
+
+
+def gather_peaks(N, stream):
+    """Measured ceilings (GB/s of independent 32-byte record reads) of each memory level."""
+    return {k: round(N.probe_gather(v, reps=3, stream=stream.cuda_stream), 1) for k, v in PROBE_SETS}
+
+
+def footprint_level(nbytes):
+    for name, cap in LEVEL_CAPACITY:
+        if nbytes <= cap:
+            return name
+    return "hbm"
+
+
+def gather_roofline(achieved, footprint, peaks, traffic, ms):
+    """Roofline of a gather-bound kernel.  Its reads come from the level that holds their
+    footprint (the bytes of every structure the kernel reads); HBM is priced at its
+    published 8 TB/s, a cache level at its measured gather ceiling.  A kernel whose rate
+    exceeds that level's ceiling is served by a faster level (locality), so the first
+    faster level whose ceiling is above the achieved rate bounds it instead."""
+    level = footprint_level(footprint)
+    peak = HBM_PEAK_GBS if level == "hbm" else peaks[level]
+    bound = level
+    while achieved > peak and bound in FASTER:
+        bound = FASTER[bound]
+        peak = peaks[bound]
+    roof = {"bound": bound, "achieved": round(achieved, 2), "peak": peak, "unit": "GB/s",
+            "frac": round(achieved / peak, 5), "traffic": traffic,
+            "footprint_bytes": int(footprint), "footprint_level": level, "gather_peaks_gbs": peaks}
+    if traffic:
+        # secondary figure: the measured L2-miss traffic (FETCH_SIZE + WRITE_SIZE) per frame
+        # against the published HBM peak
+        roof["traffic_gbs"] = round(traffic / (ms * 1e-3) / 1e9, 1)
+        roof["traffic_frac_of_hbm_peak"] = round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return roof
+
+
+def read_footprint(tech, r, wl, bpv, light_cache):
+    """Bytes of the structures the dominant kernel reads: the corner-block volume (8
+    values per voxel), the SAT's 64-byte corner blocks, the pyramid's corner blocks
+    (8 values per cell at the stored width), the frame written."""
+    from cpp_volume_rendering_amd import native as N
+    nz, ny, nx = wl.volume.shape
+    total = nx * ny * nz * 8 * bpv + wl.width * wl.height * 16
+    if light_cache:
+        return total + 32 ** 3 * 8
+    if tech == N.EBS:
+        total += (nx + 2) * (ny + 2) * (nz + 2) * 64
+    if tech in (N.DOS, N.CONE_TRACE, N.CONE_NN):
+        p = r.params()
+        prec = p.density_precision if tech == N.DOS else p.cone.density_precision
+        cells = sum(int(a.size) for a in r.pyramid())
+        total += cells * 8 * (4 if prec == 0 else 2)
+    return total
+
+
 def measured_traffic(kernel, workload, width):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), when one was
@@ -110,28 +171,58 @@ def measured_traffic(kernel, workload, width):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
+def host_cpu():
+    """nproc and the lscpu model name of the host running the CPU baseline."""
+    import subprocess
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.lower().startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return os.cpu_count() or 0, model
+
+
+def median_of_5(run, arg):
+    """One warm-up call, then the median wall time of 5 calls (BASELINE.md's protocol)."""
+    run(arg)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run(arg)
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[2], ts
+
+
+def cpu_baseline(kind, wl, cam, light, params=None, budget_s=18.0):
     """The oracle's FP32 instantiation on all host cores (OpenMP over rows) on a bounded
-    sample of the same frame: every `stride`-th pixel in x and y, frames/s scaled by the
-    fraction of pixels rendered (stride^2)."""
+    sample of the same frame: every `stride`-th pixel in x and y (whole frames for the
+    plain caster), one warm-up then the median of 5 runs, frames/s scaled by the fraction
+    of pixels rendered (stride^2)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import numpy as np
     import oracle_lib as O
 
     threads = O.lib().orc_max_threads()
+    nproc, model = host_cpu()
+    host = {"cores": threads, "nproc": nproc, "cpu_model": model, "kind": "port", "protocol": "1 warm-up, median of 5"}
     tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
     W_, H_ = wl.width, wl.height
     if kind == "rc1p":
+        def frames_run(k):
+            for _ in range(k):
+                O.render_rc1p(wl.volume, tf, cam, W_, H_, fp32=True, threads=threads)
         t0 = time.perf_counter()
-        frames = 0
-        while True:
-            O.render_rc1p(wl.volume, tf, cam, W_, H_, fp32=True, threads=threads)
-            frames += 1
-            t = time.perf_counter() - t0
-            if t > budget_s or frames >= 50:
-                break
-        return {"value": frames / t, "unit": "frames/s", "cores": threads, "kind": "port",
-                "sample": f"{frames} full {W_}x{H_} frames, oracle FP32, OpenMP over rows"}
+        frames_run(1)
+        per = max(1e-3, time.perf_counter() - t0)
+        k = max(1, int(budget_s / 6 / per))
+        med, ts = median_of_5(frames_run, k)
+        return {"value": k / med, "unit": "frames/s", **host,
+                "runs_s": [round(t, 3) for t in ts],
+                "sample": f"runs of {k} full {W_}x{H_} frames, oracle FP32, OpenMP over rows"}
     if kind in ("ebs-lc", "dos-lc"):
         # the cache (FP64 oracle) is part of every frame; the march over it in FP32
         if kind == "ebs-lc":
@@ -174,27 +265,30 @@ def cpu_baseline(kind, wl, cam, light, params=None, budget_s=15.0):
 
         def run(stride):
             O.render_cone(wl.volume, tf, pyr, prm, cam, W_, H_, fp32=True, threads=threads, stride=stride)
-    # a coarse probe, then pixel strides sized for about `budget_s` of CPU work (the
-    # work scales with the number of pixels, 1 / stride^2)
+    # a coarse probe, then a pixel stride sized so that one run takes about a sixth of
+    # `budget_s` (the work scales with the number of pixels, 1 / stride^2); then one
+    # warm-up and 5 timed runs at that stride
+    target = budget_s / 6.0
     stride = 64
     t0 = time.perf_counter()
     run(stride)
     t = time.perf_counter() - t0
     for _ in range(3):
-        if t >= 0.5 * budget_s or stride <= 1:
+        if t >= 0.5 * target or stride <= 1:
             break
         # round up: near 1 a truncated stride would overshoot the budget (stride 1.6 -> 1 is
         # 2.6x the pixels)
-        stride = max(1, min(stride - 1, math.ceil(stride * (t / budget_s) ** 0.5)))
+        stride = max(1, min(stride - 1, math.ceil(stride * (t / target) ** 0.5)))
         t0 = time.perf_counter()
         run(stride)
         t = time.perf_counter() - t0
-    frame_s = t * stride * stride
+    med, ts = median_of_5(run, stride)
+    frame_s = med * stride * stride
     if kind in ("ebs-lc", "dos-lc"):
         frame_s += t_cache
-    return {"value": 1.0 / frame_s, "unit": "frames/s", "cores": threads, "kind": "port",
+    return {"value": 1.0 / frame_s, "unit": "frames/s", **host, "runs_s": [round(x, 3) for x in ts],
             "sample": f"one pixel in {stride} along x and along y of one {W_}x{H_} frame (1/{stride * stride} of it, "
-                      f"{t:.1f} s), {what}, OpenMP over rows; scaled to whole frames"}
+                      f"median {med:.2f} s), {what}, OpenMP over rows; scaled to whole frames"}
 
 
 def bench_batch(args, N, W, torch, stream):
@@ -252,13 +346,50 @@ def bench_batch(args, N, W, torch, stream):
     print(json.dumps(line), flush=True)
 
 
+def bench_sat(args, N, W, stream):
+    """The extinction SAT build of the EBS config (256^3 u16 -> 258^3 two-float cells) on
+    its own: one step = one build (a transfer-function upload between steps invalidates
+    the table), device time of the three prefix passes from events around them.
+    Algorithmic bytes: the voxels read once, the (hi, lo) table written once."""
+    wl = W.banded_noise(n=args.size or 256, width=1024, height=1024, seed=2)
+    r = N.Renderer(N.EBS)
+    r.set_volume(wl.volume, wl.scale)
+    r.collect_stats(True)
+    scan, blocks = [], []
+    for i in range(args.warmup + args.steps):
+        r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        r.prepare(stream.cuda_stream)
+        st = r.stats()
+        if i >= args.warmup:
+            scan.append(st.sat_scan_ms)
+            blocks.append(st.sat_blocks_ms)
+    ms = sum(scan) / len(scan)
+    ms_b = sum(blocks) / len(blocks)
+    nz, ny, nx = wl.volume.shape
+    algo = nx * ny * nz * 2 + (nx + 2) * (ny + 2) * (nz + 2) * 8
+    achieved = algo / (ms * 1e-3) / 1e9
+    log(f"SAT build {nx}x{ny}x{nz} u16: prefix passes {ms:.4f} ms, corner blocks {ms_b:.4f} ms")
+    line = {
+        "metric": "SAT builds/s 256³ u16 -> 258³ two-float extinction prefix sums; HBM GB/s",
+        "value": round(1000.0 / ms, 3), "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (two-float)", "data": "synthetic (seeded)",
+        "config": {"workload": wl.name, "volume": f"{nx}x{ny}x{nz} u16", "sat": f"{nx + 2}x{ny + 2}x{nz + 2} float2",
+                   "algorithmic_bytes": algo, "corner_block_copy_ms": round(ms_b, 5),
+                   "kernels": "k_sat_x, k_sat_strided (y), k_sat_strided (z)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nn",
-                    choices=["nn", "cone", "rc1p", "ebs", "dos", "ebs-lc", "dos-lc", "batch"])
+                    choices=["nn", "cone", "rc1p", "ebs", "dos", "ebs-lc", "dos-lc", "batch", "sat"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=0, help="override the volume size (tests)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -279,6 +410,11 @@ def main():
     if args.workload == "batch":
         bench_batch(args, N, W, torch, stream)
         return
+    if args.workload == "sat":
+        bench_sat(args, N, W, stream)
+        return
+    peaks = gather_peaks(N, stream)
+    log(f"gather peaks (GB/s of 32-byte reads): {peaks}")
     t_setup = time.perf_counter()
     extra = {}
     if args.workload == "rc1p":
@@ -337,7 +473,14 @@ def main():
         r.render(cam, wl.width, wl.height, light=light, out=out, stream=stream.cuda_stream)
         stream.synchronize()
         rmse = float(torch.sqrt(torch.mean((out - out_ct) ** 2)).item())
-        log(f"exact cone tracer: {ms_ct:.3f} ms/frame; surrogate speedup {ms_ct / ms:.2f}x, image RMSE {rmse:.5f}")
+        # RGB over the pixels either frame covers: alpha and the background are the same
+        # in both by construction and would dilute the figure
+        cov = (out[..., 3] > 0) | (out_ct[..., 3] > 0)
+        d = (out - out_ct)[cov][:, :3]
+        rmse_rgb = float(torch.sqrt(torch.mean(d.double() ** 2)).item()) if d.numel() else 0.0
+        gate_ok = max(rmse, rmse_rgb) <= 0.01
+        log(f"exact cone tracer: {ms_ct:.3f} ms/frame; surrogate speedup {ms_ct / ms:.2f}x, image RMSE {rmse:.5f} "
+            f"(RGB over {int(cov.sum().item())} covered pixels {rmse_rgb:.5f}): {'pass' if gate_ok else 'FAIL'}")
         config.update({"pyramid_levels": p.cone.n_levels, "cones": "8 occlusion @30 deg + 1 shadow @10 deg, 64 steps",
                        "mlp": "16-64-64-64-16 ReLU, bf16 MFMA, fp32 accumulate",
                        "train_samples": p.n_train, "train_steps": rep.steps, "prepare_s": round(prep_s, 2),
@@ -346,6 +489,8 @@ def main():
                        "baseline_rmse_occ": round(rep.base_rmse_occ, 5), "baseline_rmse_sdw": round(rep.base_rmse_sdw, 5),
                        "exact_cone_fps": round(1000.0 / ms_ct, 4), "exact_cone_ms": round(ms_ct, 4),
                        "speedup_vs_exact": round(ms_ct / ms, 3), "image_rmse_vs_exact": round(rmse, 6),
+                       "image_rmse_rgb_covered_vs_exact": round(rmse_rgb, 6),
+                       "rmse_gate": "pass (<= 0.01)" if gate_ok else "FAIL (> 0.01)",
                        "exact_gate_fallback": int(rep.exact_fallback)})
         # the network's share of the frame: 20480 FLOP per shaded sample
         config["mlp_tflops"] = round(st.shaded_samples * MLP_FLOP_PER_SAMPLE / (ms * 1e-3) / 1e12, 3)
@@ -364,8 +509,8 @@ def main():
         kernel = "k_lc_render"
         config["light_cache"] = "32x32x32 (occlusion, shadow) rebuilt every frame"
     config["kernel"] = kernel
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(kernel, wl.name, wl.width)}
+    roof = gather_roofline(achieved, read_footprint(tech, r, wl, bpv, light_cache), peaks,
+                           measured_traffic(kernel, wl.name, wl.width), ms)
     line = {
         "metric": WORKLOAD_METRIC.get(args.workload, METRIC), "value": round(fps, 4), "unit": "frames/s",
         "n_gpus": 1, "steps": args.steps,
@@ -388,6 +533,9 @@ def main():
     print(json.dumps(line), flush=True)
     if args.calib_copy:
         calib_copy(torch)
+    if tech == N.CONE_NN and not gate_ok:
+        log("the surrogate's image RMSE against the exact tracer exceeds 0.01: the headline line is not valid")
+        sys.exit(3)
 
 
 def calib_copy(torch):

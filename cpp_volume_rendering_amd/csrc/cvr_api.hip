@@ -187,6 +187,9 @@ static void defaults(int technique, void* p) {
     cvr_nn_params* n = static_cast<cvr_nn_params*>(p);
     cone_defaults(&n->cone);
     n->cone.n_levels = 7;
+    // the 16 taps read an fp16 pyramid (the reference's R16F storage, volvis_utils/utils.h:15,
+    // extcoefvolumegenerator.cpp:113): 16-byte corner blocks, half the gathered bytes
+    n->cone.density_precision = 1;
     n->n_train = 1 << 22;
     n->n_val = 1 << 16;
     n->epochs = 4;
@@ -211,6 +214,7 @@ static void free_all(cvr_ctx* c) {
   if (c->d_ext_lut) (void)hipFree(c->d_ext_lut);
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->d_out) (void)hipFree(c->d_out);
+  if (c->d_ms_frame) (void)hipFree(c->d_ms_frame);
   if (c->d_pyr) (void)hipFree(c->d_pyr);
   if (c->d_pyr_oct) (void)hipFree(c->d_pyr_oct);
   if (c->d_nn_redo) (void)hipFree(c->d_nn_redo);
@@ -223,6 +227,8 @@ static void free_all(cvr_ctx* c) {
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
                   static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : c->ev_sat)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
 }
@@ -525,33 +531,45 @@ int cvr_prepare(cvr_handle h, void* hip_stream) {
   return CVR_OK;
 }
 
-int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
-               float* out_rgba, int out_is_device, void* hip_stream) {
-  if (!h || !cam || !out_rgba) { set_error("cvr_render: null argument"); return CVR_ERR_INVALID; }
+}  // extern "C"
+
+namespace cvr {
+
+static int check_frame_args(const char* who, const cvr_camera* cam, int width, int height) {
   if (width < 1 || height < 1 || size_t(width) * height > (size_t(1) << 28)) {
-    set_error("cvr_render: image size out of range");
+    set_error(std::string(who) + ": image size out of range");
     return CVR_ERR_INVALID;
   }
-  if (!(cam->fovy_deg > 0 && cam->fovy_deg < 180)) { set_error("cvr_render: fovy must be in (0, 180)"); return CVR_ERR_INVALID; }
-  int rc = check_inputs(h);
-  if (rc) return rc;
-  DeviceGuard dg(h->device);
-  hipStream_t st = static_cast<hipStream_t>(hip_stream);
-  const size_t bytes = size_t(width) * height * 4 * sizeof(float);
-  float* dst = out_rgba;
-  if (!out_is_device) {
-    if (h->d_out_bytes < bytes) {
-      if (h->d_out) CVR_HIP_CHECK(hipFree(h->d_out));
-      h->d_out = nullptr;
-      h->d_out_bytes = 0;
-      if (hipMalloc(&h->d_out, bytes) != hipSuccess) { set_error("cvr_render: out of device memory"); return CVR_ERR_NOMEM; }
-      h->d_out_bytes = bytes;
-    }
-    dst = h->d_out;
+  if (!(cam->fovy_deg > 0 && cam->fovy_deg < 180)) {
+    set_error(std::string(who) + ": fovy must be in (0, 180)");
+    return CVR_ERR_INVALID;
   }
+  return CVR_OK;
+}
+
+// Grows a device buffer of the handle to at least `bytes`.
+static int grow_buffer(float** p, size_t* cap, size_t bytes, const char* who) {
+  if (*cap >= bytes) return CVR_OK;
+  if (*p) CVR_HIP_CHECK(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string(who) + ": out of device memory");
+    return CVR_ERR_NOMEM;
+  }
+  *cap = bytes;
+  return CVR_OK;
+}
+
+// One frame of the technique into device memory `dst` (width x height RGBA floats),
+// with the stats and the event timing of the handle.
+static int render_device(cvr_ctx* h, const cvr_camera* cam, const cvr_light* light, int width, int height, float* dst,
+                         hipStream_t st) {
   if (h->collect_stats) CVR_HIP_CHECK(hipMemsetAsync(h->d_counters, 0, 4 * sizeof(unsigned long long), st));
   RayFrame rf = frame_of(cam, width, height);
   CVR_HIP_CHECK(hipEventRecord(h->ev0, st));
+  int rc;
   switch (h->technique) {
     case CVR_RC1P: rc = launch_rc1p(h, rf, light, dst, st); break;
     case CVR_EBS: rc = launch_ebs(h, rf, light, dst, st); break;
@@ -563,22 +581,102 @@ int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int 
   }
   if (rc) return rc;
   CVR_HIP_CHECK(hipEventRecord(h->ev1, st));
+  return CVR_OK;
+}
+
+// Reads the counters and the event time of the last render (waits for the stream).
+static int finish_stats(cvr_ctx* h, hipStream_t st) {
+  if (!h->collect_stats) return CVR_OK;
+  unsigned long long cnt[4];
+  CVR_HIP_CHECK(hipMemcpyAsync(cnt, h->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  CVR_HIP_CHECK(hipStreamSynchronize(st));
+  h->stats.samples = cnt[0];
+  h->stats.shaded_samples = cnt[1];
+  h->stats.fallback_samples = cnt[2];
+  h->stats.box_sums = cnt[3];
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_ms = ms;
+  return CVR_OK;
+}
+
+}  // namespace cvr
+
+extern "C" {
+
+int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
+               float* out_rgba, int out_is_device, void* hip_stream) {
+  if (!h || !cam || !out_rgba) { set_error("cvr_render: null argument"); return CVR_ERR_INVALID; }
+  int rc = check_frame_args("cvr_render", cam, width, height);
+  if (rc) return rc;
+  if ((rc = check_inputs(h))) return rc;
+  DeviceGuard dg(h->device);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  const size_t bytes = size_t(width) * height * 4 * sizeof(float);
+  float* dst = out_rgba;
+  if (!out_is_device) {
+    if ((rc = grow_buffer(&h->d_out, &h->d_out_bytes, bytes, "cvr_render"))) return rc;
+    dst = h->d_out;
+  }
+  if ((rc = render_device(h, cam, light, width, height, dst, st))) return rc;
   if (!out_is_device) {
     CVR_HIP_CHECK(hipMemcpyAsync(out_rgba, dst, bytes, hipMemcpyDeviceToHost, st));
     CVR_HIP_CHECK(hipStreamSynchronize(st));
   }
-  if (h->collect_stats) {
-    unsigned long long cnt[4];
-    CVR_HIP_CHECK(hipMemcpyAsync(cnt, h->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
-    CVR_HIP_CHECK(hipStreamSynchronize(st));
-    h->stats.samples = cnt[0];
-    h->stats.shaded_samples = cnt[1];
-    h->stats.fallback_samples = cnt[2];
-    h->stats.box_sums = cnt[3];
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) h->stats.last_ms = ms;
+  return finish_stats(h, st);
+}
+
+// BaseVolumeRenderer's multi-scale redraws (volrenderbase.h:28-33, 50-52): the technique
+// renders a frame of the size Reshape gives the mode (volrenderbase.cpp:42-66: screen
+// times the multipliers for multiple rays per pixel and down-scaling, screen divided by
+// them for up-scaling), and RenderFrameToScreen's draw resamples it to the screen
+// (DrawMultiSampleHigherResolutionMode, DrawHigherResolutionWithDownScale,
+// DrawLowerResolutionWithUpScale, renderoutputframe.cpp:265-535).  The camera keeps the
+// screen's aspect ratio, as the reference's camera follows the window.
+int cvr_render_multiscale(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int screen_w, int screen_h,
+                          int mode, int mult_w, int mult_h, int kernel, float* out_rgba, int out_is_device,
+                          void* hip_stream) {
+  if (!h || !cam || !out_rgba) { set_error("cvr_render_multiscale: null argument"); return CVR_ERR_INVALID; }
+  if (mode < CVR_SINGLE_RAY_PER_PIXEL || mode > CVR_UP_SCALING_RENDER) {
+    set_error("cvr_render_multiscale: unknown multi-scaling mode");
+    return CVR_ERR_INVALID;
   }
-  return CVR_OK;
+  if (mode == CVR_SINGLE_RAY_PER_PIXEL)
+    return cvr_render(h, cam, light, screen_w, screen_h, out_rgba, out_is_device, hip_stream);
+  if (mult_w < 1 || mult_h < 1 || kernel < CVR_FILTER_BOX || kernel > CVR_FILTER_CARDINAL_OMOMS3) {
+    set_error("cvr_render_multiscale: multipliers must be >= 1 and the kernel a CVR_FILTER_* value");
+    return CVR_ERR_INVALID;
+  }
+  int rc = check_frame_args("cvr_render_multiscale", cam, screen_w, screen_h);
+  if (rc) return rc;
+  if ((rc = check_inputs(h))) return rc;
+  const int sgn = mode == CVR_UP_SCALING_RENDER ? -1 : 1;
+  int fw = 0, fh = 0;
+  if ((rc = cvr_multiscale_size(screen_w, screen_h, sgn * mult_w, sgn * mult_h, &fw, &fh))) return rc;
+  if ((rc = check_frame_args("cvr_render_multiscale", cam, fw, fh))) return rc;
+  DeviceGuard dg(h->device);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  const size_t fbytes = size_t(fw) * fh * 4 * sizeof(float), sbytes = size_t(screen_w) * screen_h * 4 * sizeof(float);
+  if ((rc = grow_buffer(&h->d_ms_frame, &h->d_ms_frame_bytes, fbytes, "cvr_render_multiscale"))) return rc;
+  float* dst = out_rgba;
+  if (!out_is_device) {
+    if ((rc = grow_buffer(&h->d_out, &h->d_out_bytes, sbytes, "cvr_render_multiscale"))) return rc;
+    dst = h->d_out;
+  }
+  cvr_camera fc = *cam;
+  if (!(fc.aspect > 0)) fc.aspect = float(screen_w) / float(screen_h);
+  if ((rc = render_device(h, &fc, light, fw, fh, h->d_ms_frame, st))) return rc;
+  const int smode = mode == CVR_MULTIPLE_RAYS_PER_PIXEL ? CVR_SCALE_MULTISAMPLE
+                    : mode == CVR_DOWN_SCALING_RENDER   ? CVR_SCALE_DOWNSCALE
+                                                        : CVR_SCALE_UPSCALE;
+  // the cardinal up-scaling prefilters the rendered frame in place, as the reference does
+  if ((rc = cvr_resample_frame(h->device, h->d_ms_frame, fw, fh, dst, screen_w, screen_h, smode, kernel,
+                               h->d_ms_frame, hip_stream)))
+    return rc;
+  if (!out_is_device) {
+    CVR_HIP_CHECK(hipMemcpyAsync(out_rgba, dst, sbytes, hipMemcpyDeviceToHost, st));
+    CVR_HIP_CHECK(hipStreamSynchronize(st));
+  }
+  return finish_stats(h, st);
 }
 
 int cvr_get_sat(cvr_handle h, double* out_host, size_t count) {

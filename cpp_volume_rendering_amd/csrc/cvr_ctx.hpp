@@ -79,6 +79,9 @@ struct cvr_ctx {
   cvr::PyrKey pyr_key{};
   bool pyr_valid = false;  // d_pyr / d_pyr_oct hold the pyramid of pyr_key
   void* d_cone_tab = nullptr;  // surrogate input taps + cone step schedules
+  bool cone_tab_valid = false; // d_cone_tab holds the table of cone_tab_for over pyr_key
+  cvr_cone_params cone_tab_for{};
+  cvr::PyrKey cone_tab_pyr{};  // the pyramid the table's level geometry describes
 
   // directional occlusion shading: section tables (occlusion rows, then shadow rows)
   cvr::DosTable dos_occ, dos_sdw;
@@ -116,10 +119,14 @@ struct cvr_ctx {
   size_t nn_redo_cap = 0;
   cvr_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_sat[3] = {nullptr, nullptr, nullptr};  // SAT build timing (stats on)
 
   // scratch output when rendering to a host buffer
   float* d_out = nullptr;
   size_t d_out_bytes = 0;
+  // frame rendered at the multi-scale size before it is resampled to the screen
+  float* d_ms_frame = nullptr;
+  size_t d_ms_frame_bytes = 0;
 };
 
 namespace cvr {
@@ -144,6 +151,9 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
 int build_volume_oct(cvr_ctx* c, hipStream_t st);
 int build_pyramid_oct(cvr_ctx* c, size_t total, int fmt, hipStream_t st);
 int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt, hipStream_t st);
+PyrKey pyr_key_for(const cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, int nlev, int fmt);
+// true when the pyramid and the cone table of P are built (a render allocates nothing)
+bool cone_ready(const cvr_ctx* c, const cvr_cone_params& P);
 int check_cone_params(const cvr_cone_params& P);
 int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st);
 int launch_cone(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st);

@@ -549,6 +549,106 @@ void RenderingParameters::SetBlinnPhongLightSourceCameraVectors(const std::array
   light_right_ = right;
 }
 
+void RenderingParameters::SetLightSource(const LightSourceData& l) {
+  light_pos_ = l.position;
+  light_fwd_ = l.Forward();
+  light_up_ = l.y_axis;
+  light_right_ = l.x_axis;
+}
+
+// ------------------------------------------------------------------ camera states, light lists
+namespace {
+// Whitespace-separated numbers from a stream; false when fewer than n could be read.
+bool read_floats(std::istream& in, float* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!(in >> v[i])) return false;
+  return true;
+}
+// Next line that is not blank (false at the end of the stream).
+bool next_nonblank(std::istream& in, std::string& line) {
+  while (std::getline(in, line))
+    if (!trim(line).empty()) return true;
+  return false;
+}
+bool list_error(std::string* err, const std::string& msg) {
+  fail(err, msg);
+  return false;
+}
+}  // namespace
+
+bool CameraStateList::ReadCameraStates(const std::string& filepath, std::string* err) {
+  states_.clear();
+  std::ifstream f(filepath);
+  if (!f) return list_error(err, "cannot open camera states " + filepath);
+  std::string name, type;
+  while (next_nonblank(f, name)) {
+    CameraData c;
+    c.cam_setup_name = trim(name);
+    if (!std::getline(f, type)) return list_error(err, "camera state '" + c.cam_setup_name + "': missing type line");
+    type = trim(type);
+    if (type == "FLIGHT") {
+      c.c_type = CameraData::FLIGHT;
+    } else if (type == "ARCBALL") {
+      c.c_type = CameraData::ARCBALL;
+      float v[9];
+      if (!read_floats(f, v, 9)) return list_error(err, "camera state '" + c.cam_setup_name + "': needs eye, center, up");
+      c.eye = {v[0], v[1], v[2]};
+      c.center = {v[3], v[4], v[5]};
+      c.up = {v[6], v[7], v[8]};
+      std::string rest;
+      std::getline(f, rest);  // the end of the up line
+    } else {
+      return list_error(err, "camera state '" + c.cam_setup_name + "': type must be ARCBALL or FLIGHT, not '" + type + "'");
+    }
+    states_.push_back(c);
+  }
+  if (states_.empty()) return list_error(err, "no camera states in " + filepath);
+  return true;
+}
+
+int CameraStateList::FindCameraState(const std::string& name) const {
+  for (int i = 0; i < NumberOfCameraStates(); ++i)
+    if (states_[i].cam_setup_name == name) return i;
+  return -1;
+}
+
+bool LightSourceList::ReadLightSourceLists(const std::string& filepath, std::string* err) {
+  lists_.clear();
+  std::ifstream f(filepath);
+  if (!f) return list_error(err, "cannot open light source lists " + filepath);
+  std::string name;
+  while (next_nonblank(f, name)) {
+    LightSourceListItem item;
+    item.l_name = trim(name);
+    int n = 0;
+    if (!(f >> n) || n < 0 || n > 64) return list_error(err, "light source list '" + item.l_name + "': bad light count");
+    for (int i = 0; i < n; ++i) {
+      float v[13];
+      if (!read_floats(f, v, 13))
+        return list_error(err, "light source list '" + item.l_name + "': needs position, forward, up, right, spot angle");
+      LightSourceData l;
+      l.position = {v[0], v[1], v[2]};
+      l.z_axis = {-v[3], -v[4], -v[5]};
+      l.y_axis = {v[6], v[7], v[8]};
+      l.x_axis = {v[9], v[10], v[11]};
+      l.spot_light_angle = v[12];
+      l.spot_light_angle_rad = (v[12] * 3.14159265358979f) / 180.0f;
+      item.m_lightsources.push_back(l);
+    }
+    std::string rest;
+    std::getline(f, rest);  // the end of the spot-angle line
+    lists_.push_back(item);
+  }
+  if (lists_.empty()) return list_error(err, "no light source lists in " + filepath);
+  return true;
+}
+
+int LightSourceList::FindList(const std::string& name) const {
+  for (int i = 0; i < NumberOfLists(); ++i)
+    if (lists_[i].l_name == name) return i;
+  return -1;
+}
+
 cvr_light RenderingParameters::ToLight() const {
   cvr_light l;
   for (int i = 0; i < 3; ++i) {

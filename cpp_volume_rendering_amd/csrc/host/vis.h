@@ -160,6 +160,56 @@ class Camera {
   float fovy_ = 45.0f, aspect_ = 1.0f;
 };
 
+// CameraData / CameraStateList (vis_utils/camera.h, volvis_utils/camerastatelist.cpp:26-88):
+// named camera states, each "name", "ARCBALL" (then eye, center, up as 9 numbers) or
+// "FLIGHT" (no vectors), as data/#list_camera_states holds them.
+struct CameraData {
+  enum Type { FLIGHT = 0, ARCBALL = 1 };
+  std::string cam_setup_name;
+  int c_type = ARCBALL;
+  std::array<float, 3> eye{0, 0, 0}, center{0, 0, 0}, up{0, 1, 0};
+};
+
+class CameraStateList {
+ public:
+  // False (with *err) when the file cannot be read or a record is malformed; a blank
+  // trailing record (a final newline) is not a state.
+  bool ReadCameraStates(const std::string& filepath, std::string* err = nullptr);
+  int NumberOfCameraStates() const { return int(states_.size()); }
+  const CameraData* GetCameraState(int i) const { return i >= 0 && i < NumberOfCameraStates() ? &states_[i] : nullptr; }
+  int FindCameraState(const std::string& name) const;
+
+ private:
+  std::vector<CameraData> states_;
+};
+
+// LightSourceData / LightSourceList (volvis_utils/lightsourcelist.h, lightsourcelist.cpp:81-148):
+// named lists of light sources, each light as position, forward (towards the light),
+// up, right and the spot angle in degrees, as data/#list_light_sources holds them.  The
+// light stores z_axis = -forward, as the reference does.
+struct LightSourceData {
+  std::array<float, 3> position{0, 0, 0};
+  std::array<float, 3> z_axis{0, 0, 1}, y_axis{0, 1, 0}, x_axis{1, 0, 0};
+  float spot_light_angle = 4.0f, spot_light_angle_rad = 4.0f * 3.14159265f / 180.0f;
+  std::array<float, 3> Forward() const { return {-z_axis[0], -z_axis[1], -z_axis[2]}; }
+};
+
+struct LightSourceListItem {
+  std::string l_name;
+  std::vector<LightSourceData> m_lightsources;
+};
+
+class LightSourceList {
+ public:
+  bool ReadLightSourceLists(const std::string& filepath, std::string* err = nullptr);
+  int NumberOfLists() const { return int(lists_.size()); }
+  const LightSourceListItem* GetList(int i) const { return i >= 0 && i < NumberOfLists() ? &lists_[i] : nullptr; }
+  int FindList(const std::string& name) const;
+
+ private:
+  std::vector<LightSourceListItem> lists_;
+};
+
 // volvis_utils/renderingparameters.cpp:17-31, 147-167 (headless: one light source).
 class RenderingParameters {
  public:
@@ -177,6 +227,8 @@ class RenderingParameters {
   void SetBlinnPhongLightSourceCameraVectors(const std::array<float, 3>& fwd, const std::array<float, 3>& up,
                                              const std::array<float, 3>& right);
   std::array<float, 3> GetBlinnPhongLightSourceCameraForward() const { return light_fwd_; }
+  // CreateNewLightSource(LightSourceData) as the current light (renderingparameters.cpp:100-106)
+  void SetLightSource(const LightSourceData& l);
   cvr_light ToLight() const;
 
  private:

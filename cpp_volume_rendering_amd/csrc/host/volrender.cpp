@@ -21,10 +21,29 @@ void BaseVolumeRenderer::Clean() {
 }
 
 void BaseVolumeRenderer::Reshape(int w, int h) {
-  if (w == m_width && h == m_height) return;
-  m_width = w;
-  m_height = h;
-  m_frame.assign(size_t(w) * h * 4, 0.f);
+  int rw = w, rh = h;
+  const int mode = GetCurrentMultiScalingMode();
+  if (mode == MULTIPLE_RAYS_PER_PIXEL || mode == DOWN_SCALING_RENDER)
+    (void)cvr_multiscale_size(w, h, kMultiSampleW, kMultiSampleH, &rw, &rh);
+  else if (mode == UP_SCALING_RENDER)
+    (void)cvr_multiscale_size(w, h, -kMultiSampleW, -kMultiSampleH, &rw, &rh);
+  m_render_w = rw;
+  m_render_h = rh;
+  if (w != m_width || h != m_height) {
+    m_width = w;
+    m_height = h;
+    m_frame.assign(size_t(w) * h * 4, 0.f);
+  }
+  SetOutdated();
+}
+
+void BaseVolumeRenderer::SetCurrentMultiScalingMode(int f) {
+  vr_pixel_multiscaling_mode = f;
+  if (m_width > 0) Reshape(m_width, m_height);
+}
+
+void BaseVolumeRenderer::SetImageKernelFilter(unsigned int k) {
+  m_kernel_filter = k;
   SetOutdated();
 }
 
@@ -36,7 +55,12 @@ void BaseVolumeRenderer::PrepareRender(vis::Camera* camera) {
 }
 
 // ------------------------------------------------------------------ GpuVolumeRenderer
-GpuVolumeRenderer::GpuVolumeRenderer(int technique, int device) : m_technique(technique), m_device(device) {}
+// Every technique renders at any frame size, so all support the multi-scale modes (the
+// reference's structured renderers set vr_pixel_multiscaling_support, e.g.
+// rc1prenderer.cpp:24-26).
+GpuVolumeRenderer::GpuVolumeRenderer(int technique, int device) : m_technique(technique), m_device(device) {
+  vr_pixel_multiscaling_support = true;
+}
 
 GpuVolumeRenderer::~GpuVolumeRenderer() { Clean(); }
 
@@ -105,6 +129,18 @@ void GpuVolumeRenderer::Redraw() {
   Check(cvr_render(m_handle, &m_cam, m_ext_rendering_parameters ? &m_light : nullptr, m_width, m_height,
                    m_frame.data(), 0, nullptr));
 }
+
+// MultiSampleRedraw / DownScalingRedraw / UpScalingRedraw (rc1prenderer.cpp:153-190): the
+// dispatch at the render size, then RenderFrameToScreen's draw of the mode; the
+// screen-sized result lands in GetFrame().
+void GpuVolumeRenderer::MultiScaleRedraw(int mode) {
+  if (!m_handle) return;
+  Check(cvr_render_multiscale(m_handle, &m_cam, m_ext_rendering_parameters ? &m_light : nullptr, m_width, m_height,
+                              mode, kMultiSampleW, kMultiSampleH, int(m_kernel_filter), m_frame.data(), 0, nullptr));
+}
+void GpuVolumeRenderer::MultiSampleRedraw() { MultiScaleRedraw(MULTIPLE_RAYS_PER_PIXEL); }
+void GpuVolumeRenderer::DownScalingRedraw() { MultiScaleRedraw(DOWN_SCALING_RENDER); }
+void GpuVolumeRenderer::UpScalingRedraw() { MultiScaleRedraw(UP_SCALING_RENDER); }
 
 // ------------------------------------------------------------------ RenderingManager
 RenderingManager* RenderingManager::s_instance = nullptr;
@@ -176,10 +212,17 @@ void RenderingManager::SetLightDirection(const std::array<float, 3>& d) {
   m_params.SetBlinnPhongLightingPosition(eye);
 }
 
+// renderingmanager.cpp:199-208 with f_render (:1201-1229): the redraw of the current
+// multi-scaling mode.
 bool RenderingManager::Display() {
   if (!m_current || !m_current->IsBuilt()) return false;
   m_current->PrepareRender(&m_camera);
-  m_current->Redraw();
+  switch (m_current->GetCurrentMultiScalingMode()) {
+    case BaseVolumeRenderer::MULTIPLE_RAYS_PER_PIXEL: m_current->MultiSampleRedraw(); break;
+    case BaseVolumeRenderer::DOWN_SCALING_RENDER: m_current->DownScalingRedraw(); break;
+    case BaseVolumeRenderer::UP_SCALING_RENDER: m_current->UpScalingRedraw(); break;
+    default: m_current->Redraw();
+  }
   return m_current->GetLastError().empty();
 }
 

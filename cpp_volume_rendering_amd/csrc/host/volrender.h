@@ -11,6 +11,16 @@
 
 class BaseVolumeRenderer {
  public:
+  // volrenderbase.h:28-33
+  enum MULTISCALING {
+    SINGLE_RAY_PER_PIXEL = CVR_SINGLE_RAY_PER_PIXEL,
+    MULTIPLE_RAYS_PER_PIXEL = CVR_MULTIPLE_RAYS_PER_PIXEL,
+    DOWN_SCALING_RENDER = CVR_DOWN_SCALING_RENDER,
+    UP_SCALING_RENDER = CVR_UP_SCALING_RENDER,
+  };
+  // MULTISAMPLE_NUMBEROFSAMPLES_W / _H (defines.h:16-17)
+  static constexpr int kMultiSampleW = 2, kMultiSampleH = 2;
+
   BaseVolumeRenderer();
   virtual ~BaseVolumeRenderer();
 
@@ -25,6 +35,12 @@ class BaseVolumeRenderer {
   virtual bool Init(int shader_width, int shader_height) = 0;
   virtual bool Update(vis::Camera* camera) = 0;
   virtual void Redraw() = 0;
+  // volrenderbase.h:50-52: the frame rendered at the mode's size and resampled to the
+  // screen (the base versions do nothing, as the reference's)
+  virtual void MultiSampleRedraw() {}
+  virtual void DownScalingRedraw() {}
+  virtual void UpScalingRedraw() {}
+  // volrenderbase.cpp:42-66: the render size follows the multi-scaling mode
   virtual void Reshape(int w, int h);
   virtual vis::GRID_VOLUME_DATA_TYPE GetDataTypeSupport() { return vis::GRID_VOLUME_DATA_TYPE::STRUCTURED; }
 
@@ -33,6 +49,17 @@ class BaseVolumeRenderer {
   virtual void SetOutdated() { vr_outdated = true; }
   bool IsOutdated() const { return vr_outdated; }
   bool IsBuilt() const { return vr_built; }
+
+  // volrenderbase.cpp:103-121
+  bool IsPixelMultiScalingSupported() const { return vr_pixel_multiscaling_support; }
+  int GetCurrentMultiScalingMode() const { return IsPixelMultiScalingSupported() ? vr_pixel_multiscaling_mode : 0; }
+  void SetCurrentMultiScalingMode(int f);
+  // RenderFrameToScreen::Get/SetImageKernelFilter (renderoutputframe.h:53-57): a CVR_FILTER_* kernel
+  unsigned int GetImageKernelFilter() const { return m_kernel_filter; }
+  void SetImageKernelFilter(unsigned int k);
+  // The size the technique renders at (the screen size, or its multiple / fraction).
+  int GetRenderWidth() const { return m_render_w; }
+  int GetRenderHeight() const { return m_render_h; }
 
   // The rendered frame: width * height premultiplied RGBA floats, row 0 at the bottom.
   const std::vector<float>& GetFrame() const { return m_frame; }
@@ -45,9 +72,13 @@ class BaseVolumeRenderer {
   void SetBuilt(bool b) { vr_built = b; }
 
   bool vr_built = false, vr_outdated = true;
+  bool vr_pixel_multiscaling_support = false;
+  int vr_pixel_multiscaling_mode = 0;
+  unsigned int m_kernel_filter = CVR_FILTER_HAT;  // IMAGE_FILTER_KERNEL default (filters/utils.hpp:8-15)
   vis::DataManager* m_ext_data_manager = nullptr;
   vis::RenderingParameters* m_ext_rendering_parameters = nullptr;
-  int m_width = 0, m_height = 0;
+  int m_width = 0, m_height = 0;      // screen
+  int m_render_w = 0, m_render_h = 0;  // frame the technique renders
   std::vector<float> m_frame;
   std::string m_error;
 };
@@ -61,6 +92,9 @@ class GpuVolumeRenderer : public BaseVolumeRenderer {
   bool Init(int shader_width, int shader_height) override;
   bool Update(vis::Camera* camera) override;
   void Redraw() override;
+  void MultiSampleRedraw() override;
+  void DownScalingRedraw() override;
+  void UpScalingRedraw() override;
   // Technique parameters (the cvr_*_params POD of this technique).
   bool SetParams(const void* params, size_t bytes);
   bool GetParams(void* params, size_t bytes);
@@ -68,6 +102,7 @@ class GpuVolumeRenderer : public BaseVolumeRenderer {
 
  protected:
   bool Check(int rc);
+  void MultiScaleRedraw(int mode);
   int m_technique, m_device;
   cvr_handle m_handle = nullptr;
   cvr_camera m_cam{};

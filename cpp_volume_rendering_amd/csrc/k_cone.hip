@@ -178,6 +178,18 @@ int check_cone_params(const cvr_cone_params& P) {
 }
 
 // Builds the pyramid (if needed) and the per-volume cone tables; fills K's pointers.
+static PyrKey cone_pyr_key(const cvr_ctx* c, const cvr_cone_params& P) {
+  const int bx = P.base_res[0] > 0 ? P.base_res[0] : c->nx;
+  const int by = P.base_res[1] > 0 ? P.base_res[1] : c->ny;
+  const int bz = P.base_res[2] > 0 ? P.base_res[2] : c->nz;
+  return pyr_key_for(c, bx, by, bz, P.sigma0, P.n_levels, P.density_precision);
+}
+
+bool cone_ready(const cvr_ctx* c, const cvr_cone_params& P) {
+  return c->d_pyr && c->pyr_valid && c->pyr_key == cone_pyr_key(c, P) && c->d_cone_tab && c->cone_tab_valid &&
+         c->cone_tab_pyr == c->pyr_key && std::memcmp(&c->cone_tab_for, &P, sizeof(P)) == 0;
+}
+
 int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   int rc = check_cone_params(P);
   if (rc) return rc;
@@ -186,6 +198,10 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   const int bz = P.base_res[2] > 0 ? P.base_res[2] : c->nz;
   rc = build_pyramid(c, bx, by, bz, P.sigma0, P.n_levels, P.density_precision, st);
   if (rc) return rc;
+  // the tap table and the step schedules depend on the parameters and the pyramid (its
+  // levels, the volume's size) only: rebuilt (a synchronous upload) when one changes
+  if (cone_ready(c, P)) return 0;
+  c->cone_tab_valid = false;
   const double diag = volume_diagonal(c);
   const double occ_max = P.occ_max_dist > 0 ? P.occ_max_dist : 0.5 * float(diag);
   const double sdw_max = P.sdw_max_dist > 0 ? P.sdw_max_dist : 0.75 * float(diag);
@@ -204,12 +220,16 @@ int prepare_cone(cvr_ctx* c, const cvr_cone_params& P, hipStream_t st) {
   // [taps (kNFeat x 64 B)] [occlusion schedule] [shadow schedule]
   const size_t bytes = kNFeat * sizeof(FeatTap) + (so.size() + ss.size()) * sizeof(ConeStepF);
   if (!c->d_cone_tab) CVR_HIP_CHECK(hipMalloc(&c->d_cone_tab, kNFeat * sizeof(FeatTap) + 2 * kMaxSteps * sizeof(ConeStepF)));
+  else CVR_HIP_CHECK(hipDeviceSynchronize());  // earlier renders may still read the old table
   std::vector<char> buf(bytes);
   std::memcpy(buf.data(), taps.data(), kNFeat * sizeof(FeatTap));
   std::memcpy(buf.data() + kNFeat * sizeof(FeatTap), so.data(), so.size() * sizeof(ConeStepF));
   std::memcpy(buf.data() + kNFeat * sizeof(FeatTap) + so.size() * sizeof(ConeStepF), ss.data(), ss.size() * sizeof(ConeStepF));
   CVR_HIP_CHECK(hipMemcpy(c->d_cone_tab, buf.data(), bytes, hipMemcpyHostToDevice));
   c->cone_n_steps = P.n_steps;
+  c->cone_tab_for = P;
+  c->cone_tab_pyr = c->pyr_key;
+  c->cone_tab_valid = true;
   return 0;
 }
 

@@ -18,6 +18,7 @@
 #include "cvr_nn.hpp"
 #include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 namespace cvr {
@@ -596,14 +597,23 @@ NnK nn_consts(cvr_ctx* c) {
   return nk;
 }
 
-int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
+// Work shape of a surrogate frame: the schedule and the number of workgroup tiles.
+static NnSched nn_sched(const cvr_nn_params& N) {
+  NnSched sc{0, N.shade_batch, N.span};
+  while ((1 << sc.tw_log2) < N.tile_w) ++sc.tw_log2;
+  return sc;
+}
+static int nn_tiles(const NnSched& sc, const RayFrame& rf) {
+  const int tw = 1 << sc.tw_log2, th = (64 / tw) * sc.span;
+  return ((rf.W + 2 * tw - 1) / (2 * tw)) * ((rf.H + 2 * th - 1) / (2 * th));
+}
+
+// The frame of a prepared, trained handle: the exact tracer when the validation gate is
+// closed, else the two passes of k_nn_render over `redo` (one word per workgroup tile,
+// zero on entry and left zero by the second pass).  Allocates nothing.
+static int nn_frame(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, unsigned int* redo,
+                    unsigned long long* cnt, hipStream_t st) {
   const cvr_nn_params& N = c->nn;
-  int rc = prepare_cone(c, N.cone, st);
-  if (rc) return rc;
-  if (!c->nn_trained) {
-    rc = nn_train(c, st);
-    if (rc) return rc;
-  }
   float L[3];
   light_dir(rf, light, L);
   ConeK K = cone_constants(c, N.cone, L);
@@ -612,22 +622,40 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
     // the validation gate failed: the exact tracer renders the frame
     cvr_cone_params saved = c->cone;
     c->cone = N.cone;
-    rc = launch_cone(c, rf, light, out, st);
+    const int rc = launch_cone(c, rf, light, out, st);
     c->cone = saved;
     return rc;
   }
   VolGeom g = geom_of(c);
   const float step = N.cone.step_size > 0 ? N.cone.step_size : default_step(c);
-  NnSched sc{0, N.shade_batch, N.span};
-  while ((1 << sc.tw_log2) < N.tile_w) ++sc.tw_log2;
-  const int tw = 1 << sc.tw_log2, th = (64 / tw) * N.span;
-  const int tiles = ((rf.W + 2 * tw - 1) / (2 * tw)) * ((rf.H + 2 * th - 1) / (2 * th));
+  const NnSched sc = nn_sched(N);
+  const int tiles = nn_tiles(sc, rf);
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
   if (shm > 32 * 1024) { set_error("surrogate renderer: transfer function larger than 2048 entries"); return -4; }
   short* wp = static_cast<short*>(c->d_nn_pack);
   float* bp = reinterpret_cast<float*>(wp + kPackShorts);
-  unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   float4* o4 = reinterpret_cast<float4*>(out);
+  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
+    using T = typename decltype(t)::type;
+    constexpr int FF = decltype(f)::value;
+    hipLaunchKernelGGL((k_nn_render<T, FF, false>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
+                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, redo, cnt);
+    hipLaunchKernelGGL((k_nn_render<T, FF, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
+                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, redo, cnt);
+  });
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out, hipStream_t st) {
+  const cvr_nn_params& N = c->nn;
+  int rc = prepare_cone(c, N.cone, st);
+  if (rc) return rc;
+  if (!c->nn_trained) {
+    rc = nn_train(c, st);
+    if (rc) return rc;
+  }
+  const int tiles = nn_tiles(nn_sched(N), rf);
   // tile marks for the second pass: zero when allocated, and the second pass clears
   // what the first sets
   if (size_t(tiles) > c->nn_redo_cap) {
@@ -642,18 +670,10 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
     CVR_HIP_CHECK(hipMemset(c->d_nn_redo, 0, size_t(tiles) * sizeof(unsigned int)));
     c->nn_redo_cap = size_t(tiles);
   }
-  with_types(c->dtype, c->pyr.fmt, [&](auto t, auto f) {
-    using T = typename decltype(t)::type;
-    constexpr int FF = decltype(f)::value;
-    hipLaunchKernelGGL((k_nn_render<T, FF, false>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
-                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, c->d_nn_redo, cnt);
-    hipLaunchKernelGGL((k_nn_render<T, FF, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->pyr, c->d_pyr_oct, K,
-                       nk, sc, wp, bp, c->d_tf_rgbt, c->tf_size, step, N.cone.et_threshold, o4, c->d_nn_redo, cnt);
-  });
-  CVR_HIP_CHECK(hipGetLastError());
-  return 0;
+  return nn_frame(c, rf, light, out, c->d_nn_redo, c->collect_stats ? c->d_counters : nullptr, st);
 }
 
 int nn_param_count() { return kNParams; }
 
 }  // namespace cvr
+

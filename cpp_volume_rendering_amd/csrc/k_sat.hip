@@ -155,6 +155,12 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
   c->d_ext_lut = nullptr;
   CVR_HIP_CHECK(hipMalloc(&c->d_ext_lut, lut.size() * sizeof(float)));
   CVR_HIP_CHECK(hipMemcpyAsync(c->d_ext_lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  const bool timed = c->collect_stats;
+  if (timed) {
+    for (hipEvent_t* e : {&c->ev_sat[0], &c->ev_sat[1], &c->ev_sat[2]})
+      if (!*e) CVR_HIP_CHECK(hipEventCreate(e));
+    CVR_HIP_CHECK(hipEventRecord(c->ev_sat[0], st));
+  }
   const int rows = h * d;
   if (c->dtype == CVR_U8)
     hipLaunchKernelGGL(k_sat_x<uint8_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
@@ -171,6 +177,7 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
   hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, h), dim3(256), 0, st, c->d_sat, w, h, size_t(w), d,
                      size_t(w) * h);
   CVR_HIP_CHECK(hipGetLastError());
+  if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[1], st));
   // corner blocks for the render kernels' taps
   if (!c->d_sat_oct) {
     if (hipMalloc(&c->d_sat_oct, cells * 4 * sizeof(float4)) != hipSuccess) {
@@ -183,8 +190,16 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
   hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned((h + 3) / 4), unsigned(d)), dim3(256), 0, st,
                      c->d_sat, static_cast<float4*>(c->d_sat_oct), w, h, d);
   CVR_HIP_CHECK(hipGetLastError());
+  if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[2], st));
   // the LUT is read by pass 1 only; keep it until the stream has consumed it
   CVR_HIP_CHECK(hipStreamSynchronize(st));
+  if (timed) {
+    float a = 0.f, b = 0.f;
+    CVR_HIP_CHECK(hipEventElapsedTime(&a, c->ev_sat[0], c->ev_sat[1]));
+    CVR_HIP_CHECK(hipEventElapsedTime(&b, c->ev_sat[1], c->ev_sat[2]));
+    c->stats.sat_scan_ms = a;
+    c->stats.sat_blocks_ms = b;
+  }
   c->sat_built_for = key;
   return 0;
 }

@@ -39,6 +39,7 @@ GRADIENT_SOBEL, GRADIENT_FINITE_DIFF = 0, 1
 (FILTER_BOX, FILTER_HAT, FILTER_CATMULL_ROM, FILTER_MITCHELL_NETRAVALI, FILTER_CARDINAL_BSPLINE3,
  FILTER_CARDINAL_OMOMS3) = range(6)
 SCALE_MULTISAMPLE, SCALE_DOWNSCALE, SCALE_UPSCALE = 0, 1, 2
+SINGLE_RAY_PER_PIXEL, MULTIPLE_RAYS_PER_PIXEL, DOWN_SCALING_RENDER, UP_SCALING_RENDER = 0, 1, 2, 3
 
 
 class Rc1pParams(C.Structure):
@@ -113,7 +114,8 @@ class BatchTuning(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("shaded_samples", C.c_uint64), ("fallback_samples", C.c_uint64),
-                ("last_ms", C.c_double), ("box_sums", C.c_uint64)]
+                ("last_ms", C.c_double), ("box_sums", C.c_uint64), ("sat_scan_ms", C.c_double),
+                ("sat_blocks_ms", C.c_double)]
 
 
 PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, CONE_TRACE: ConeParams,
@@ -123,7 +125,7 @@ PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, C
 EXPORTS = [
     "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
     "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
-    "cvr_render", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_render", "cvr_render_multiscale", "cvr_probe_gather", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
     "cvr_multiscale_size", "cvr_resample_frame",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
@@ -167,6 +169,8 @@ def lib() -> C.CDLL:
         "cvr_get_params": (i, [vp, vp, sz]),
         "cvr_prepare": (i, [vp, vp]),
         "cvr_render": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, vp, i, vp]),
+        "cvr_probe_gather": (i, [i, C.c_size_t, i, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]),
+        "cvr_render_multiscale": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, i, i, i, i, vp, i, vp]),
         "cvr_get_sat": (i, [vp, vp, sz]),
         "cvr_get_light_cache": (i, [vp, vp, sz, vp]),
         "cvr_get_gradient": (i, [vp, i, vp, sz]),
@@ -343,6 +347,18 @@ class Renderer:
         ptr, is_dev = _out_arg(out, (height, width, 4), self.device)
         check(lib().cvr_render(self.h, C.byref(cam), C.byref(light) if light is not None else None, width, height,
                                ptr, int(is_dev), stream))
+        return out
+
+    def render_multiscale(self, cam: Camera, width: int, height: int, mode: int, light: Light | None = None,
+                          mult=(2, 2), kernel=None, out=None, stream=None):
+        """cvr_render_multiscale: render at the mode's multi-scale frame size and resample
+        to (height, width, 4) (a new host array, or `out`)."""
+        if out is None:
+            out = np.zeros((height, width, 4), dtype=np.float32)
+        ptr, is_dev = _out_arg(out, (height, width, 4), self.device)
+        k = FILTER_HAT if kernel is None else kernel
+        check(lib().cvr_render_multiscale(self.h, C.byref(cam), C.byref(light) if light is not None else None, width,
+                                          height, mode, mult[0], mult[1], k, ptr, int(is_dev), stream))
         return out
 
     def gradient(self, shape, kind=GRADIENT_SOBEL) -> np.ndarray:
@@ -538,6 +554,13 @@ class Batch:
         s = Stats()
         check(lib().cvr_batch_get_stats(self.h, C.byref(s)))
         return s
+
+
+def probe_gather(working_set_bytes: int, reps: int = 3, device: int = 0, stream=None):
+    """GB/s of independent 32-byte record reads from a working set of the given size."""
+    g, n = C.c_double(), C.c_double()
+    check(lib().cvr_probe_gather(device, int(working_set_bytes), reps, C.byref(g), C.byref(n), stream))
+    return g.value
 
 
 def multiscale_size(screen_w: int, screen_h: int, mw: int, mh: int):

@@ -211,7 +211,8 @@ typedef struct cvr_cone_params {
   float sdw_max_dist;   /* <= 0: 0.75 * diagonal */
   float r_sigma;        /* cone radius / Gaussian sigma at the level read (2.0) */
   float ka, kd;         /* 0.5, 0.5 */
-  int density_precision; /* pyramid storage read by the cones: 0 fp32, 1 fp16, 2 bf16 (0) */
+  int density_precision; /* pyramid storage read by the cones: 0 fp32, 1 fp16, 2 bf16
+                            (0; 1 in the surrogate's cvr_nn_params.cone) */
 } cvr_cone_params;
 
 /* Surrogate: the cone parameters it replaces (and falls back to) plus training. */
@@ -321,6 +322,36 @@ int cvr_multiscale_size(int screen_w, int screen_h, int mw, int mh, int* frame_w
 int cvr_resample_frame(int device, const float* src, int sw, int sh, float* dst, int dw, int dh, int mode, int kernel,
                        float* work, void* hip_stream);
 
+/* Multi-scale rendering of a technique (BaseVolumeRenderer::MULTISCALING and the
+ * MultiSampleRedraw / DownScalingRedraw / UpScalingRedraw it selects,
+ * cppvolrend/volrenderbase.h:28-33, 50-52, dispatched by RenderingManager::Display,
+ * renderingmanager.cpp:204, 1201-1229).  The technique renders a frame of
+ * cvr_multiscale_size(screen, +mult) for CVR_MULTIPLE_RAYS_PER_PIXEL and
+ * CVR_DOWN_SCALING_RENDER, of cvr_multiscale_size(screen, -mult) for
+ * CVR_UP_SCALING_RENDER (Reshape, volrenderbase.cpp:42-66; the reference's multipliers
+ * are 2 x 2, defines.h:16-17), which is resampled to the screen size with `kernel`
+ * (cvr_resample_frame's CVR_SCALE_MULTISAMPLE / DOWNSCALE / UPSCALE) into out_rgba
+ * (screen_w x screen_h RGBA floats, host or device).  CVR_SINGLE_RAY_PER_PIXEL is
+ * cvr_render at the screen size.  A camera aspect <= 0 means the screen's w/h.  The
+ * handle keeps the multi-scale frame in device memory between calls. */
+enum {
+  CVR_SINGLE_RAY_PER_PIXEL = 0,
+  CVR_MULTIPLE_RAYS_PER_PIXEL = 1,
+  CVR_DOWN_SCALING_RENDER = 2,
+  CVR_UP_SCALING_RENDER = 3
+};
+int cvr_render_multiscale(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int screen_w, int screen_h,
+                          int mode, int mult_w, int mult_h, int kernel, float* out_rgba, int out_is_device,
+                          void* hip_stream);
+
+/* Measurement helper, no reference counterpart: the rate (GB/s of 32-byte reads) at which
+ * lanes reading independent 32-byte records at pseudo-random indices get them back from a
+ * working set of `working_set_bytes` (16 KiB: L1, 2 MiB: L2, 64 MiB: MALL, 4 GiB: HBM),
+ * best of `reps` launches after a warm-up.  The ceiling of the renderers' trilinear
+ * corner-block reads from each level, used for their roofline figures. */
+int cvr_probe_gather(int device, size_t working_set_bytes, int reps, double* gbps, double* reads_per_launch,
+                     void* hip_stream);
+
 /* Counters of the last render (samples marched, shaded samples). */
 typedef struct cvr_stats {
   uint64_t samples;          /* volume samples taken along camera rays */
@@ -328,6 +359,10 @@ typedef struct cvr_stats {
   uint64_t fallback_samples; /* surrogate: samples the exact tracer shaded */
   double last_ms;            /* device time of the last render's main kernel */
   uint64_t box_sums;         /* CVR_EBS: SAT box sums evaluated (8 trilinear SAT reads each) */
+  /* CVR_EBS with stats on: device time of the last SAT build, its three prefix passes
+   * and the corner-block copy the render kernels read (events around the kernels). */
+  double sat_scan_ms;
+  double sat_blocks_ms;
 } cvr_stats;
 int cvr_get_stats(cvr_handle h, cvr_stats* out);
 int cvr_set_collect_stats(cvr_handle h, int enable);
