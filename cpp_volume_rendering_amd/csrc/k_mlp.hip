@@ -18,6 +18,7 @@
 #include "cvr_nn.hpp"
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -673,7 +674,86 @@ int launch_nn(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* out
   return nn_frame(c, rf, light, out, c->d_nn_redo, c->collect_stats ? c->d_counters : nullptr, st);
 }
 
+// ---------------------------------------------------------------------------------
+// Allocation-free rendering (cvr_render_noalloc): the tile marks live in a fixed array
+// of the library on every GPU instead of a buffer of the handle.  Each GPU holds its
+// own copy of `scratch` (256 KiB) until the process ends; host code uses only the
+// address hipGetSymbolAddress gives for the current device, never the array's name.
+constexpr int kNoAllocMaxTiles = 1 << 16;
+static __device__ __align__(256) unsigned int scratch[kNoAllocMaxTiles];
+constexpr int kMaxDevices = 64;
+static unsigned int* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `dev`, which must be the current device; looked up
+// once per device.
+static int scratch_of(int dev, unsigned int** p) {
+  if (dev < 0 || dev >= kMaxDevices) { set_error("cvr_render_noalloc: device index above 63"); return CVR_ERR_UNSUPPORTED; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      set_error(std::string("cvr_render_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+      return CVR_ERR_HIP;
+    }
+    g_scratch[dev] = static_cast<unsigned int*>(a);
+  }
+  *p = g_scratch[dev];
+  return CVR_OK;
+}
+
+__global__ __launch_bounds__(256) void k_clear_marks(unsigned int* __restrict__ p, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 int nn_param_count() { return kNParams; }
 
 }  // namespace cvr
 
+using namespace cvr;
+
+extern "C" int cvr_render_noalloc(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
+                                  float* out_device, void* hip_stream) {
+  if (!h || !cam || !out_device) { set_error("cvr_render_noalloc: null argument"); return CVR_ERR_INVALID; }
+  if (h->technique != CVR_CONE_NN) { set_error("cvr_render_noalloc: only CVR_CONE_NN handles"); return CVR_ERR_UNSUPPORTED; }
+  if (width < 1 || height < 1 || size_t(width) * height > (size_t(1) << 28)) {
+    set_error("cvr_render_noalloc: image size out of range");
+    return CVR_ERR_INVALID;
+  }
+  if (!(cam->fovy_deg > 0 && cam->fovy_deg < 180)) { set_error("cvr_render_noalloc: fovy must be in (0, 180)"); return CVR_ERR_INVALID; }
+  if (!h->d_vol || !h->d_tf_rgbt) { set_error("cvr_render_noalloc: volume and transfer function must be set"); return CVR_ERR_STATE; }
+  if (!h->nn_trained || !h->d_nn_pack || !cone_ready(h, h->nn.cone)) {
+    set_error("cvr_render_noalloc: call cvr_prepare first (pyramid, cone tables and network must be in place)");
+    return CVR_ERR_STATE;
+  }
+  const RayFrame rf = frame_of(cam, width, height);
+  const int tiles = nn_tiles(nn_sched(h->nn), rf);
+  if (tiles > kNoAllocMaxTiles) {
+    set_error("cvr_render_noalloc: the frame needs " + std::to_string(tiles) + " workgroup tiles, above the limit of " +
+              std::to_string(kNoAllocMaxTiles) + " (4096 x 4096 pixels at 4 x 16-pixel wave tiles)");
+    return CVR_ERR_INVALID;
+  }
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (prev != h->device) CVR_HIP_CHECK(hipSetDevice(h->device));
+  struct Restore {
+    int d;
+    ~Restore() { if (d >= 0) (void)hipSetDevice(d); }
+  } restore{prev != h->device ? prev : -1};
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, out_device) != hipSuccess || at.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    set_error("cvr_render_noalloc: out_device must be device memory");
+    return CVR_ERR_INVALID;
+  }
+  unsigned int* marks = nullptr;
+  int rc = scratch_of(h->device, &marks);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  // the marks must start at zero: a fill kernel (not a memset node) keeps the call capturable
+  hipLaunchKernelGGL(k_clear_marks, dim3((tiles + 255) / 256), dim3(256), 0, st, marks, tiles);
+  CVR_HIP_CHECK(hipGetLastError());
+  return nn_frame(h, rf, light, out_device, marks, nullptr, st);
+}

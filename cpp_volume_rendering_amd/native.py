@@ -125,7 +125,7 @@ PARAMS = {RC1P: Rc1pParams, EBS: EbsParams, CONE_GT: GtParams, DOS: DosParams, C
 EXPORTS = [
     "cvr_last_error", "cvr_technique_name", "cvr_create", "cvr_destroy", "cvr_set_volume",
     "cvr_set_transfer_function", "cvr_default_params", "cvr_set_params", "cvr_get_params", "cvr_prepare",
-    "cvr_render", "cvr_render_multiscale", "cvr_probe_gather", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
+    "cvr_render", "cvr_render_noalloc", "cvr_render_multiscale", "cvr_probe_gather", "cvr_get_sat", "cvr_get_light_cache", "cvr_get_gradient", "cvr_get_stats", "cvr_set_collect_stats",
     "cvr_multiscale_size", "cvr_resample_frame",
     "cvr_pyramid_info", "cvr_get_pyramid_level", "cvr_cone_eval", "cvr_nn_param_count", "cvr_nn_train",
     "cvr_nn_get_weights", "cvr_nn_set_weights", "cvr_nn_eval", "cvr_nn_gradient", "cvr_nn_get_report",
@@ -169,6 +169,7 @@ def lib() -> C.CDLL:
         "cvr_get_params": (i, [vp, vp, sz]),
         "cvr_prepare": (i, [vp, vp]),
         "cvr_render": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, vp, i, vp]),
+        "cvr_render_noalloc": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, vp, vp]),
         "cvr_probe_gather": (i, [i, C.c_size_t, i, C.POINTER(C.c_double), C.POINTER(C.c_double), vp]),
         "cvr_render_multiscale": (i, [vp, C.POINTER(Camera), C.POINTER(Light), i, i, i, i, i, i, vp, i, vp]),
         "cvr_get_sat": (i, [vp, vp, sz]),
@@ -359,6 +360,16 @@ class Renderer:
         k = FILTER_HAT if kernel is None else kernel
         check(lib().cvr_render_multiscale(self.h, C.byref(cam), C.byref(light) if light is not None else None, width,
                                           height, mode, mult[0], mult[1], k, ptr, int(is_dev), stream))
+        return out
+
+    def render_noalloc(self, cam: Camera, width: int, height: int, out, light: Light | None = None, stream=None):
+        """cvr_render_noalloc: render a prepared CVR_CONE_NN handle into the device tensor
+        `out` ((H, W, 4) float32) without any device allocation."""
+        ptr, is_dev = _out_arg(out, (height, width, 4), self.device)
+        if not is_dev:
+            raise CvrError(ERR_INVALID, "cvr_render_noalloc writes device memory only")
+        check(lib().cvr_render_noalloc(self.h, C.byref(cam), C.byref(light) if light is not None else None, width,
+                                       height, ptr, stream))
         return out
 
     def gradient(self, shape, kind=GRADIENT_SOBEL) -> np.ndarray:

@@ -282,6 +282,19 @@ int cvr_prepare(cvr_handle h, void* hip_stream);
 int cvr_render(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
                float* out_rgba, int out_is_device, void* hip_stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): cvr_render of
+ * a CVR_CONE_NN handle into device memory that calls no allocating routine.  The handle
+ * must be prepared first (cvr_prepare: pyramid, cone tables and trained network in
+ * place; otherwise CVR_ERR_STATE).  The two-pass render's tile marks live in a fixed
+ * library array of 65536 words per GPU (256 KiB, held until the process ends), so a
+ * frame may have at most 65536 workgroup tiles (4096 x 4096 pixels at the default 4 x 16
+ * wave tiles; larger frames give CVR_ERR_INVALID).  Every call on one GPU shares that
+ * array: calls that overlap in time on one GPU race.  The pixels equal cvr_render's;
+ * stats are not collected. */
+int cvr_render_noalloc(cvr_handle h, const cvr_camera* cam, const cvr_light* light, int width, int height,
+                       float* out_device, void* hip_stream);
+
 /* The extinction SAT built on the GPU for CVR_EBS ((nx+2)(ny+2)(nz+2) cells,
  * ebsrenderer.cpp:624-723 + summedareatable.h:218-278), copied to the host as
  * double (hi + lo of the two-float storage).  Builds it if needed. */
