@@ -225,7 +225,8 @@ static void free_all(cvr_ctx* c) {
   if (c->d_lc) (void)hipFree(c->d_lc);
   if (c->d_grad) (void)hipFree(c->d_grad);
   for (void* p : {static_cast<void*>(c->d_nn_w), static_cast<void*>(c->d_nn_g), static_cast<void*>(c->d_nn_m),
-                  static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack})
+                  static_cast<void*>(c->d_nn_v), static_cast<void*>(c->d_nn_scalar), c->d_nn_pack, c->d_nn_tpack,
+                  static_cast<void*>(c->d_nn_slabs)})
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev_sat)
     if (e) (void)hipEventDestroy(e);

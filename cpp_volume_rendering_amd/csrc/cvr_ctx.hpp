@@ -97,6 +97,9 @@ struct cvr_ctx {
   // surrogate network: fp32 master weights, gradient, Adam moments, packed bf16 copy
   float *d_nn_w = nullptr, *d_nn_g = nullptr, *d_nn_m = nullptr, *d_nn_v = nullptr, *d_nn_scalar = nullptr;
   void* d_nn_pack = nullptr;
+  void* d_nn_tpack = nullptr;      // bf16 training packs (forward, biases, W^T tiles)
+  float* d_nn_slabs = nullptr;     // per-64-sample gradient slabs of a training step
+  size_t nn_slabs_cap = 0;
   float nn_lo[16] = {0}, nn_hi[16] = {0};
   bool nn_trained = false;
   uint64_t nn_version = 0;  // bumped whenever the weights or the input range change

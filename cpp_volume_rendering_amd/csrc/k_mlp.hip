@@ -11,10 +11,11 @@
 // exactly the B operand of the next layer: inference keeps all activations in
 // registers; only the input features pass through LDS once.
 //
-// Training (k_mlp_train): one wavefront per workgroup walks `chunks` groups of 64
-// samples, keeps the activations of one group in LDS, runs the backward GEMMs
-// (W^T dY and dY H^T, operands gathered from LDS with transposed strides) and
-// accumulates the weight gradients in LDS; one atomic flush per workgroup.
+// Training (k_mlp_train, k_mlp_update): one wavefront per 64-sample chunk runs the
+// forward on the register path, the activation gradients through W^T tiles in
+// registers, and the weight gradients as MFMA over samples from sample-major bf16
+// copies in LDS; it writes its chunk's gradient slab (no atomics).  k_mlp_update sums
+// the slabs, takes the Adam step and writes the bf16 packs of the new weights.
 #include "cvr_nn.hpp"
 #include <cmath>
 #include <cstring>
@@ -59,108 +60,382 @@ __device__ void layer_fwd(const float* __restrict__ W, const float* __restrict__
     }
 }
 
-// Gradients of one layer: dW += G_out H_in^T, db += row sums of G_out, and (if gIn)
-// gIn = (W^T G_out) masked by H_in > 0.
-__device__ void layer_bwd(const float* __restrict__ W, int in, int out, const float* gOut, const float* hIn,
-                          float* gIn, float* sGradW, float* sGradB) {
-  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
-  if (lane < out) {
-    float s = 0.f;
-    for (int j = 0; j < 64; ++j) s += gOut[lane * 64 + j];
-    sGradB[lane] += s;
+// ---------------------------------------------------------------- training (MFMA register path)
+// One bf16 copy of the network for training, in MFMA A-operand order:
+//   [forward tiles, kPackShorts, as mlp_wave reads them] [fp32 biases, kNBias]
+//   [backward tiles of W^T, kBwdShorts]
+// Backward tiles, the A operands of the activation gradients G_in = W^T G_out: layer 4,
+// W4^T (64 inputs x K = 16 outputs), 4 tiles of 16x16x16 where lane (r, q) holds
+// W4[4q + e][16 mt + r]; layers 3 and 2, W^T (64 x K = 64), 4 x 2 tiles of 16x16x32 where
+// lane (r, q) holds W[chunk_unit(c, q, e)][16 mt + r] -- the K order in which a layer's
+// gradient D tiles pack into the next B operand, as the forward's activations do.  Layer
+// 1's input gradient is never needed.  The update kernel writes these packs after every
+// optimiser step, so the training kernel only copies them into LDS.
+constexpr int kBTiles4 = 4, kBTiles3 = 8, kBTiles2 = 8;
+constexpr int kBwdShorts = kBTiles4 * 64 * 4 + (kBTiles3 + kBTiles2) * 64 * 8;
+constexpr int kBwdBase3 = kBTiles4 * 64 * 4, kBwdBase2 = kBwdBase3 + kBTiles3 * 64 * 8;
+constexpr size_t kTrainPackBytes = size_t(kPackShorts) * 2 + size_t(kNBias) * 4 + size_t(kBwdShorts) * 2;
+// Per-chunk gradient slab: the kNParams gradients of a chunk's samples, then their
+// squared-error sum.  A chunk is one training wavefront's samples: 32 gives every CU two
+// wavefronts on a 16384-sample step.
+constexpr int kSlab = kNParams + 16;
+constexpr int kTrainChunk = 32;
+// LDS row stride (bf16) of the sample-major copies of activations and gradients: 64 samples
+// + 8 of padding, so that 16 rows read at one column fall on distinct banks.
+constexpr int kTS = 72;
+
+// Pack slots of flattened parameter i: short index in the forward pack (or -1), in the
+// backward pack (or -1), float index in the bias pack (or -1).
+__host__ __device__ inline void pack_slots(int i, int& fwd, int& bwd, int& bias) {
+  fwd = bwd = bias = -1;
+  if (i < kOffB1) {  // W1[o][k], 64 x 16: 16x16x16 tiles, lane (r, q) holds W1[16 t + r][4q + e]
+    const int o = i / kNFeat, k = i % kNFeat;
+    fwd = ((o / 16) * 64 + 16 * (k / 4) + o % 16) * 4 + k % 4;
+    return;
   }
-  for (int mt = 0; mt < out / 16; ++mt)
+  if (i < kOffW2) { bias = i - kOffB1; return; }
+  int l, off;
+  if (i < kOffB2) { l = 2; off = kOffW2; }
+  else if (i < kOffW3) { bias = kH + i - kOffB2; return; }
+  else if (i < kOffB3) { l = 3; off = kOffW3; }
+  else if (i < kOffW4) { bias = 2 * kH + i - kOffB3; return; }
+  else if (i < kOffB4) { l = 4; off = kOffW4; }
+  else { bias = 3 * kH + i - kOffB4; return; }
+  const int j = i - off, o = j / kH, k = j % kH;  // W_l[o][k] over 64 inputs
+  // chunk_unit(c, q, e) = k: c = k / 32, q = (k % 16) / 4, e = 4 ((k % 32) / 16) + k % 4
+  const int c = k / 32, q = (k % 16) / 4, e = 4 * ((k % 32) / 16) + k % 4;
+  const int t = (l == 2 ? 0 : l == 3 ? kTiles2 : kTiles2 + kTiles3) + (o / 16) * 2 + c;
+  fwd = kTiles1 * 64 * 4 + (t * 64 + 16 * q + o % 16) * 8 + e;
+  if (l == 4) {  // W4^T: row = input k, K = output o (16x16x16)
+    bwd = ((k / 16) * 64 + 16 * (o / 4) + k % 16) * 4 + o % 4;
+  } else {       // W^T: row = input k, K = output o in chunk order (16x16x32)
+    const int c2 = o / 32, q2 = (o % 16) / 4, e2 = 4 * ((o % 32) / 16) + o % 4;
+    bwd = (l == 3 ? kBwdBase3 : kBwdBase2) + (((k / 16) * 2 + c2) * 64 + 16 * q2 + k % 16) * 8 + e2;
+  }
+}
+
+__device__ __forceinline__ short bf16_bits(float x) {
+  return __builtin_bit_cast(short, __builtin_convertvector(f32x2_t{x, 0.f}, bf16x2v_t)[0]);
+}
+// fp32 D tiles a (rows 4q..4q+3 of tile 2c) and b (of tile 2c+1) as one 8 x bf16 B operand
+// (hardware round to nearest even), the order relu_pack uses, without the ReLU.
+__device__ __forceinline__ bf16x8v_t grad_pack(f32x4_t a, f32x4_t b) {
+  const f32x2_t p0 = {a[0], a[1]}, p1 = {a[2], a[3]}, p2 = {b[0], b[1]}, p3 = {b[2], b[3]};
+  const u32x4_t w = {__builtin_bit_cast(unsigned int, __builtin_convertvector(p0, bf16x2v_t)),
+                     __builtin_bit_cast(unsigned int, __builtin_convertvector(p1, bf16x2v_t)),
+                     __builtin_bit_cast(unsigned int, __builtin_convertvector(p2, bf16x2v_t)),
+                     __builtin_bit_cast(unsigned int, __builtin_convertvector(p3, bf16x2v_t))};
+  return __builtin_bit_cast(bf16x8v_t, w);
+}
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// D tile d of column tile u (rows unit0 + 4q + e, column = sample 16u + r) into the
+// sample-major LDS copy sT[unit][sample], masked by the post-ReLU activation bits `act`
+// (element e of the tile's rows; nullptr: no mask).  Returns the masked tile.
+__device__ __forceinline__ f32x4_t mask_tile(f32x4_t d, const short* act4) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) d[e] = act4[e] > 0 ? d[e] : 0.f;
+  return d;
+}
+__device__ __forceinline__ void store_tile(short* sT, f32x4_t d, int unit0, int u) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) sT[(unit0 + 4 * q + e) * kTS + 16 * u + r] = bf16_bits(d[e]);
+}
+// An activation B operand (8 x bf16 of chunk c of column tile u) into sT[unit][sample].
+__device__ __forceinline__ void store_act(short* sT, bf16x8v_t a, int c, int u) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  const i16x2_t* v = reinterpret_cast<const i16x2_t*>(&a);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) sT[chunk_unit(c, q, e) * kTS + 16 * u + r] = v[e >> 1][e & 1];
+}
+
+// Weight and bias gradients of one layer over the wave's 32 NKC samples: dW[o][k] =
+// sum_s G[o][s] H[k][s] and db[o] = sum_s G[o][s] from the sample-major LDS copies sG
+// (`out` rows) and sH (`in` rows), as 16x16x32 MFMA over 32-sample steps; written to
+// the slab (dW row-major at offW, db at offB).
+template <int NKC>
+__device__ __forceinline__ void weight_grads(const short* sG, const short* sH, int out, int in, float* slab, int offW,
+                                             int offB) {
+  const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+  typedef short s16x8_t __attribute__((ext_vector_type(8)));
+  const s16x8_t ones_bits = {0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80};  // bf16 1.0
+  const bf16x8v_t ones = __builtin_bit_cast(bf16x8v_t, ones_bits);
+  for (int mt = 0; mt < out / 16; ++mt) {
+    bf16x8v_t ga[NKC];
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc)
+      ga[kc] = *reinterpret_cast<const bf16x8v_t*>(sG + (16 * mt + r) * kTS + 32 * kc + 8 * q);
     for (int nt = 0; nt < in / 16; ++nt) {
       f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      acc = tile_gemm(gOut, 64, 1, hIn, 1, 64, mt * 16, nt * 16, 64, acc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) sGradW[(mt * 16 + 4 * q + e) * in + nt * 16 + r] += acc[e];
-    }
-  if (!gIn) return;
-  for (int mt = 0; mt < in / 16; ++mt)
-    for (int nt = 0; nt < 4; ++nt) {
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      acc = tile_gemm(W, 1, in, gOut, 64, 1, mt * 16, nt * 16, out, acc);
+      for (int kc = 0; kc < NKC; ++kc)
+        acc = mfma32(ga[kc], *reinterpret_cast<const bf16x8v_t*>(sH + (16 * nt + r) * kTS + 32 * kc + 8 * q), acc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = mt * 16 + 4 * q + e, s = nt * 16 + r;
-        gIn[i * 64 + s] = hIn[i * 64 + s] > 0.f ? acc[e] : 0.f;
-      }
+      for (int e = 0; e < 4; ++e) slab[offW + (16 * mt + 4 * q + e) * in + 16 * nt + r] = acc[e];
     }
-}
-
-// X: [n][16] features, T: [n][2] targets.  grad (global, zeroed by the caller) +=
-// d/dparams of sum (y - t)^2 / (2 * norm_n); loss (global) += that sum.
-__global__ __launch_bounds__(64) void k_mlp_train(const float* __restrict__ prm, const float* __restrict__ X,
-                                                  const float* __restrict__ Tg, int n, int chunks, float norm_n,
-                                                  float* __restrict__ grad, float* __restrict__ loss) {
-  extern __shared__ float lds[];
-  float* sX = lds;               // 16 x 64
-  float* sH1 = sX + 16 * 64;     // 64 x 64
-  float* sH2 = sH1 + 64 * 64;
-  float* sH3 = sH2 + 64 * 64;
-  float* sY = sH3 + 64 * 64;     // 16 x 64
-  float* sG1 = sY + 16 * 64;     // 64 x 64
-  float* sG2 = sG1 + 64 * 64;    // 64 x 64
-  float* sGrad = sG2 + 64 * 64;  // kNParams
-  const int lane = threadIdx.x;
-  for (int i = lane; i < kNParams; i += 64) sGrad[i] = 0.f;
-  float lsum = 0.f;
-  const float inv_n = 1.0f / norm_n;
-  for (int c = 0; c < chunks; ++c) {
-    const int s0 = (blockIdx.x * chunks + c) * 64;
-    if (s0 >= n) break;
-    __syncthreads();
-    const int s = s0 + lane;
-    const bool valid = s < n;
-    for (int f = 0; f < kNFeat; ++f) sX[f * 64 + lane] = valid ? X[size_t(s) * kNFeat + f] : 0.f;
-    __syncthreads();
-    layer_fwd(prm + kOffW1, prm + kOffB1, kNFeat, kH, sX, sH1, true);
-    __syncthreads();
-    layer_fwd(prm + kOffW2, prm + kOffB2, kH, kH, sH1, sH2, true);
-    __syncthreads();
-    layer_fwd(prm + kOffW3, prm + kOffB3, kH, kH, sH2, sH3, true);
-    __syncthreads();
-    layer_fwd(prm + kOffW4, prm + kOffB4, kH, kOut, sH3, sY, false);
-    __syncthreads();
-    // dL/dy for the 2 used outputs of valid samples
-    for (int o = 0; o < kOut; ++o) {
-      float g = 0.f;
-      if (o < 2 && valid) {
-        const float e = sY[o * 64 + lane] - Tg[size_t(s) * 2 + o];
-        lsum += e * e;
-        g = e * inv_n;
-      }
-      sG1[o * 64 + lane] = g;
+    f32x4_t db = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) db = mfma32(ga[kc], ones, db);
+    if (r == 0) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) slab[offB + 16 * mt + 4 * q + e] = db[e];
     }
-    __syncthreads();
-    layer_bwd(prm + kOffW4, kH, kOut, sG1, sH3, sG2, sGrad + kOffW4, sGrad + kOffB4);
-    __syncthreads();
-    layer_bwd(prm + kOffW3, kH, kH, sG2, sH2, sG1, sGrad + kOffW3, sGrad + kOffB3);
-    __syncthreads();
-    layer_bwd(prm + kOffW2, kH, kH, sG1, sH1, sG2, sGrad + kOffW2, sGrad + kOffB2);
-    __syncthreads();
-    layer_bwd(prm + kOffW1, kNFeat, kH, sG2, sX, nullptr, sGrad + kOffW1, sGrad + kOffB1);
   }
-  __syncthreads();
-  for (int i = lane; i < kNParams; i += 64) atomicAdd(&grad[i], sGrad[i]);
-  // wave sum of the squared errors
-  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, 64);
-  if (lane == 0) atomicAdd(loss, lsum * 0.5f * inv_n);
 }
 
-// Adam (Kingma & Ba 2015) on the fp32 master parameters; clears the gradient.
-__global__ void k_adam(float* __restrict__ prm, float* __restrict__ grad, float* __restrict__ m, float* __restrict__ v,
-                       int n, float lr, float b1, float b2, float eps, float bc1, float bc2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float g = grad[i];
-  const float mi = b1 * m[i] + (1.f - b1) * g;
-  const float vi = b2 * v[i] + (1.f - b2) * g * g;
-  m[i] = mi;
-  v[i] = vi;
-  prm[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-  grad[i] = 0.f;
+// One optimiser step's gradients: workgroup = one wavefront = one chunk of 16 NU samples
+// (NU column tiles of 16; NU = 2 or 4).
+// X: [n][16], Tg: [n][2]; slab c (kSlab floats) receives d/dparams of
+// sum_s (y_s - t_s)^2 / (2 norm_n) over the chunk's samples, and that sum.  Forward and
+// activation gradients stay in registers (the renderer's MFMA register path, then W^T
+// tiles in the same K order); only the weight-gradient contractions over samples read
+// sample-major bf16 copies in LDS.
+template <int NU>
+__global__ __launch_bounds__(64) void k_mlp_train(const unsigned char* __restrict__ pack, const float* __restrict__ X,
+                                                  const float* __restrict__ Tg, int n, float norm_n,
+                                                  float* __restrict__ slabs) {
+  __shared__ __align__(16) short sWf[kPackShorts];
+  __shared__ __align__(16) float sB[kNBias];
+  __shared__ __align__(16) short sWb[kBwdShorts];
+  __shared__ __align__(16) float sX[kNFeat * 64 + 2 * 64];
+  __shared__ __align__(16) short sG[kH * kTS];
+  __shared__ __align__(16) short sH[kH * kTS];
+  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(pack);
+    uint4* d0 = reinterpret_cast<uint4*>(sWf);
+    for (int i = lane; i < kPackShorts / 8; i += 64) d0[i] = src[i];
+    const float* bsrc = reinterpret_cast<const float*>(pack + size_t(kPackShorts) * 2);
+    for (int i = lane; i < kNBias; i += 64) sB[i] = bsrc[i];
+    const uint4* wsrc = reinterpret_cast<const uint4*>(pack + size_t(kPackShorts) * 2 + size_t(kNBias) * 4);
+    uint4* d1 = reinterpret_cast<uint4*>(sWb);
+    for (int i = lane; i < kBwdShorts / 8; i += 64) d1[i] = wsrc[i];
+  }
+  static_assert(NU == 2 || NU == 4, "32- or 64-sample chunks");
+  constexpr int NKC = NU / 2;
+  const int s0 = blockIdx.x * (16 * NU);
+  const int s = s0 + lane;
+  const bool valid = lane < 16 * NU && s < n;
+  float f[kNFeat];
+#pragma unroll
+  for (int i = 0; i < kNFeat; ++i) f[i] = valid ? X[size_t(s) * kNFeat + i] : 0.f;
+  // inputs: bf16 rows for the forward operand (as mlp_wave) and the sample-major copy for dW1
+  typedef unsigned int u32x2v_t __attribute__((ext_vector_type(2)));
+  u32x2v_t* row = reinterpret_cast<u32x2v_t*>(sX + lane * kInRow);
+#pragma unroll
+  for (int i = 0; i < kNFeat / 4; ++i) {
+    const f32x2_t a = {f[4 * i], f[4 * i + 1]}, b = {f[4 * i + 2], f[4 * i + 3]};
+    row[i] = u32x2v_t{__builtin_bit_cast(unsigned int, __builtin_convertvector(a, bf16x2v_t)),
+                      __builtin_bit_cast(unsigned int, __builtin_convertvector(b, bf16x2v_t))};
+  }
+  __syncthreads();  // packs and inputs in LDS
+  const bf16x4_t* W1 = reinterpret_cast<const bf16x4_t*>(sWf);
+  const bf16x8v_t* W2 = reinterpret_cast<const bf16x8v_t*>(sWf + kTiles1 * 64 * 4);
+  const bf16x8v_t* W3 = W2 + kTiles2 * 64;
+  const bf16x8v_t* W4 = W3 + kTiles3 * 64;
+  const f32x4_t* B4 = reinterpret_cast<const f32x4_t*>(sB);
+  const bf16x4_t* V4 = reinterpret_cast<const bf16x4_t*>(sWb);
+  const bf16x8v_t* V3 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase3);
+  const bf16x8v_t* V2 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase2);
+  // ---- forward, activations kept as bf16 B operands
+  f32x4_t h[NU][4];
+  bf16x8v_t a1[NU][2], a2[NU][2], a3[NU][2];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const bf16x4_t xb = *reinterpret_cast<const bf16x4_t*>(sX + (u * 16 + r) * kInRow + 2 * q);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) h[u][mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
+    a1[u][0] = relu_pack(h[u][0], h[u][1]);
+    a1[u][1] = relu_pack(h[u][2], h[u][3]);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8v_t wa = W2[(mt * 2) * 64 + lane], wb = W2[(mt * 2 + 1) * 64 + lane];
+    const f32x4_t bias = B4[16 + mt * 4 + q];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) h[u][mt] = mfma32(wb, a1[u][1], mfma32(wa, a1[u][0], bias));
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    a2[u][0] = relu_pack(h[u][0], h[u][1]);
+    a2[u][1] = relu_pack(h[u][2], h[u][3]);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x8v_t wa = W3[(mt * 2) * 64 + lane], wb = W3[(mt * 2 + 1) * 64 + lane];
+    const f32x4_t bias = B4[32 + mt * 4 + q];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) h[u][mt] = mfma32(wb, a2[u][1], mfma32(wa, a2[u][0], bias));
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    a3[u][0] = relu_pack(h[u][0], h[u][1]);
+    a3[u][1] = relu_pack(h[u][2], h[u][3]);
+  }
+  // ---- output gradient g4 = (y - t) / norm_n on outputs 0, 1 (rows e < 2 of lanes q = 0)
+  float* slab = slabs + size_t(blockIdx.x) * kSlab;
+  const float inv_n = 1.0f / norm_n;
+  float lsum = 0.f;
+  f32x4_t g4[NU];
+  {
+    const bf16x8v_t w40 = W4[lane], w41 = W4[64 + lane];
+    const f32x4_t b4 = B4[48 + q];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const f32x4_t y = mfma32(w41, a3[u][1], mfma32(w40, a3[u][0], b4));
+      const int su = s0 + 16 * u + r;
+      f32x4_t g = {0.f, 0.f, 0.f, 0.f};
+      if (q == 0 && su < n) {
+        const float e0 = y[0] - Tg[size_t(su) * 2], e1 = y[1] - Tg[size_t(su) * 2 + 1];
+        lsum += e0 * e0 + e1 * e1;
+        g[0] = e0 * inv_n;
+        g[1] = e1 * inv_n;
+      }
+      g4[u] = g;
+    }
+  }
+  // ---- layer 4: dW4 = G4 H3^T, then G3 = (W4^T G4) masked by h3 > 0
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    store_tile(sG, g4[u], 0, u);
+    store_act(sH, a3[u][0], 0, u);
+    store_act(sH, a3[u][1], 1, u);
+  }
+  wave_sync();
+  weight_grads<NKC>(sG, sH, kOut, kH, slab, kOffW4, kOffB4);
+  f32x4_t g[NU][4];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const f32x2_t p0 = {g4[u][0], g4[u][1]}, p1 = {g4[u][2], g4[u][3]};
+    typedef unsigned int u32x2_t2 __attribute__((ext_vector_type(2)));
+    const u32x2_t2 gb = {__builtin_bit_cast(unsigned int, __builtin_convertvector(p0, bf16x2v_t)),
+                         __builtin_bit_cast(unsigned int, __builtin_convertvector(p1, bf16x2v_t))};
+    const bf16x4_t gop = __builtin_bit_cast(bf16x4_t, gb);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+      const short* act = reinterpret_cast<const short*>(&a3[u][mt >> 1]) + 4 * (mt & 1);
+      g[u][mt] = mask_tile(mfma16(V4[mt * 64 + lane], gop, z), act);
+    }
+  }
+  // ---- layers 3 and 2: dW = G H^T, then G_in = (W^T G) masked by the layer below
+#pragma unroll
+  for (int l = 3; l >= 2; --l) {
+    wave_sync();  // the previous layer's reads of sG / sH are done
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[u][mt], 16 * mt, u);
+      const bf16x8v_t* below = l == 3 ? a2[u] : a1[u];
+      store_act(sH, below[0], 0, u);
+      store_act(sH, below[1], 1, u);
+    }
+    wave_sync();
+    weight_grads<NKC>(sG, sH, kH, kH, slab, l == 3 ? kOffW3 : kOffW2, l == 3 ? kOffB3 : kOffB2);
+    const bf16x8v_t* V = l == 3 ? V3 : V2;
+    bf16x8v_t gb[NU][2];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      gb[u][0] = grad_pack(g[u][0], g[u][1]);
+      gb[u][1] = grad_pack(g[u][2], g[u][3]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const bf16x8v_t va = V[(mt * 2) * 64 + lane], vb = V[(mt * 2 + 1) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+        const bf16x8v_t* act = l == 3 ? a2[u] : a1[u];
+        g[u][mt] = mask_tile(mfma32(vb, gb[u][1], mfma32(va, gb[u][0], z)),
+                             reinterpret_cast<const short*>(&act[mt >> 1]) + 4 * (mt & 1));
+      }
+    }
+  }
+  // ---- layer 1: dW1 = G1 X^T
+  wave_sync();
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[u][mt], 16 * mt, u);
+#pragma unroll
+  for (int i = 0; i < kNFeat; ++i) sH[i * kTS + lane] = bf16_bits(f[i]);
+  // columns 16 NU..71 of X's rows are never read (the reads stop at column 16 NU - 1)
+  wave_sync();
+  weight_grads<NKC>(sG, sH, kH, kNFeat, slab, kOffW1, kOffB1);
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, 64);
+  if (lane == 0) slab[kNParams] = lsum * 0.5f * inv_n;
+}
+
+// Sum of the gradient slabs; mode 0 writes it (and the loss) to grad / loss, mode 1 also
+// takes an Adam step (Kingma & Ba 2015) on the fp32 master parameters, mode 2 only
+// packs; modes 1 and 2 write the bf16 training packs and the renderer's pack (forward
+// tiles + fp32 biases) of the updated parameters.  A workgroup takes 32 parameters
+// (one cache line of each slab) with 8 threads per parameter, each summing every 8th
+// slab with 8 loads in flight; the partial sums meet in LDS.
+constexpr int kUpdParams = 32, kUpdParts = 8;
+__global__ __launch_bounds__(256) void k_mlp_update(const float* __restrict__ slabs, int nslabs, int mode,
+                                                    float* __restrict__ prm, float* __restrict__ grad,
+                                                    float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                                    float b2, float eps, float bc1, float bc2,
+                                                    unsigned char* __restrict__ tpack, short* __restrict__ rpack,
+                                                    float* __restrict__ loss) {
+  __shared__ float part[kUpdParts][kUpdParams];
+  const int pi = threadIdx.x % kUpdParams, pt = threadIdx.x / kUpdParams;
+  const int i = blockIdx.x * kUpdParams + pi;  // parameter (kNParams: the loss)
+  float gs = 0.f;
+  if (mode != 2 && i <= kNParams) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int c = pt;
+    for (; c + 7 * kUpdParts < nslabs; c += 8 * kUpdParts) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += slabs[size_t(c + k * kUpdParts) * kSlab + i];
+    }
+    for (; c < nslabs; c += kUpdParts) acc[0] += slabs[size_t(c) * kSlab + i];
+    gs = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  }
+  part[pt][pi] = gs;
+  __syncthreads();
+  if (pt != 0 || i > kNParams) return;
+#pragma unroll
+  for (int k = 1; k < kUpdParts; ++k) gs += part[k][pi];
+  if (i == kNParams) {
+    if (mode != 2) *loss = gs;
+    return;
+  }
+  float w = prm[i];
+  if (mode != 2) {
+    if (mode == 0) {
+      grad[i] = gs;
+      return;
+    }
+    const float mi = b1 * m[i] + (1.f - b1) * gs;
+    const float vi = b2 * v[i] + (1.f - b2) * gs * gs;
+    m[i] = mi;
+    v[i] = vi;
+    w -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    prm[i] = w;
+  }
+  int fwd, bwd, bias;
+  pack_slots(i, fwd, bwd, bias);
+  short* tf = reinterpret_cast<short*>(tpack);
+  float* tb = reinterpret_cast<float*>(tpack + size_t(kPackShorts) * 2);
+  short* tw = reinterpret_cast<short*>(tpack + size_t(kPackShorts) * 2 + size_t(kNBias) * 4);
+  float* rb = reinterpret_cast<float*>(rpack + kPackShorts);
+  if (fwd >= 0) {
+    const short b = f2bf(w);
+    tf[fwd] = b;
+    rpack[fwd] = b;
+  }
+  if (bwd >= 0) tw[bwd] = f2bf(w);
+  if (bias >= 0) {
+    tb[bias] = w;
+    rb[bias] = w;
+  }
 }
 
 __global__ __launch_bounds__(64) void k_mlp_eval(const float* __restrict__ prm, const float* __restrict__ X, int n,
@@ -402,31 +677,45 @@ static void init_weights(std::vector<float>& w, uint64_t seed) {
   }
 }
 
-constexpr size_t kTrainLds = (16 * 64 * 2 + 64 * 64 * 5 + kNParams) * sizeof(float);
-
-// The training kernel keeps one 64-sample group's activations and the whole gradient
-// in LDS (~129 KiB of the 160 KiB gfx950 gives a workgroup).
-static int train_kernel_attr() {
-  static int done = 0;
-  if (!done) {
-    CVR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_train),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(kTrainLds)));
-    done = 1;
+// Device buffers of training: the bf16 training packs and the gradient slabs of n samples.
+static int train_buffers(cvr_ctx* c, int n) {
+  if (!c->d_nn_tpack && hipMalloc(&c->d_nn_tpack, kTrainPackBytes) != hipSuccess) {
+    c->d_nn_tpack = nullptr;
+    set_error("surrogate: out of device memory (training packs)");
+    return -5;
+  }
+  const size_t need = size_t((n + kTrainChunk - 1) / kTrainChunk) * kSlab;
+  if (need > c->nn_slabs_cap) {
+    if (c->d_nn_slabs) CVR_HIP_CHECK(hipFree(c->d_nn_slabs));
+    c->d_nn_slabs = nullptr;
+    c->nn_slabs_cap = 0;
+    if (hipMalloc(&c->d_nn_slabs, need * sizeof(float)) != hipSuccess) {
+      c->d_nn_slabs = nullptr;
+      set_error("surrogate: out of device memory (gradient slabs)");
+      return -5;
+    }
+    c->nn_slabs_cap = need;
   }
   return 0;
 }
 
+// The update kernel in one of its modes (0: gradient sum, 1: Adam step, 2: packs only).
+static void launch_update(cvr_ctx* c, int nslabs, int mode, float lr, float bc1, float bc2, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_update, dim3((kNParams + 1 + kUpdParams - 1) / kUpdParams), dim3(kUpdParams * kUpdParts), 0, st,
+                     c->d_nn_slabs, nslabs, mode,
+                     c->d_nn_w, c->d_nn_g, c->d_nn_m, c->d_nn_v, lr, 0.9f, 0.999f, 1e-8f, bc1, bc2,
+                     static_cast<unsigned char*>(c->d_nn_tpack), static_cast<short*>(c->d_nn_pack), c->d_nn_scalar);
+}
+
 // Gradient of L = sum (y - t)^2 / (2 n) over n samples at the current weights.
 int nn_gradient(cvr_ctx* c, const float* d_X, const float* d_T, int n, hipStream_t st) {
-  CVR_HIP_CHECK(hipMemsetAsync(c->d_nn_g, 0, kNParams * sizeof(float), st));
-  CVR_HIP_CHECK(hipMemsetAsync(c->d_nn_scalar, 0, sizeof(float), st));
-  const int chunks = 4;
-  const int blocks = (n + 64 * chunks - 1) / (64 * chunks);
-  const size_t shm = kTrainLds;
-  int rc = train_kernel_attr();
+  int rc = train_buffers(c, n);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(64), shm, st, c->d_nn_w, d_X, d_T, n, chunks, float(n),
-                     c->d_nn_g, c->d_nn_scalar);
+  const int chunks = (n + kTrainChunk - 1) / kTrainChunk;
+  launch_update(c, 0, 2, 0.f, 1.f, 1.f, st);  // the packs of the current weights
+  hipLaunchKernelGGL(k_mlp_train<kTrainChunk / 16>, dim3(chunks), dim3(64), 0, st,
+                     static_cast<const unsigned char*>(c->d_nn_tpack), d_X, d_T, n, float(n), c->d_nn_slabs);
+  launch_update(c, chunks, 0, 0.f, 1.f, 1.f, st);
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -520,25 +809,22 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   const int B = std::min(N.batch, n);
   const int steps_per_epoch = n / B;
   const int total = steps_per_epoch * N.epochs;
-  // 64-sample groups per workgroup: enough workgroups to fill the 256 CUs
-  const int chunks = std::max(1, std::min(4, B / (64 * 256)));
-  const size_t shm = kTrainLds;
-  rc = train_kernel_attr();
+  const int chunks = (B + kTrainChunk - 1) / kTrainChunk;  // one wavefront (and gradient slab) each
+  rc = train_buffers(c, B);
   if (rc) { cleanup(); return rc; }
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  launch_update(c, 0, 2, 0.f, 1.f, 1.f, st);  // packs of the initial weights
+  const double b1 = 0.9, b2 = 0.999;
   double bc1 = 1.0, bc2 = 1.0;
   for (int t = 0; t < total; ++t) {
     const int off = (t % steps_per_epoch) * B;
-    (void)hipMemsetAsync(c->d_nn_scalar, 0, sizeof(float), st);
-    hipLaunchKernelGGL(k_mlp_train, dim3((B + 64 * chunks - 1) / (64 * chunks)), dim3(64), shm, st, c->d_nn_w,
-                       d_X + size_t(off) * kNFeat, d_T + size_t(off) * 2, B, chunks, float(B), c->d_nn_g,
-                       c->d_nn_scalar);
+    hipLaunchKernelGGL(k_mlp_train<kTrainChunk / 16>, dim3(chunks), dim3(64), 0, st,
+                       static_cast<const unsigned char*>(c->d_nn_tpack), d_X + size_t(off) * kNFeat, d_T + size_t(off) * 2,
+                       B, float(B), c->d_nn_slabs);
     bc1 *= b1;
     bc2 *= b2;
     // cosine decay of the learning rate over the run
     const float lr = float(N.lr * 0.5 * (1.0 + std::cos(3.14159265358979 * double(t) / double(std::max(total, 1)))));
-    hipLaunchKernelGGL(k_adam, dim3((kNParams + 255) / 256), dim3(256), 0, st, c->d_nn_w, c->d_nn_g, c->d_nn_m,
-                       c->d_nn_v, kNParams, lr, b1, b2, eps, float(1.0 - bc1), float(1.0 - bc2));
+    launch_update(c, chunks, 1, lr, float(1.0 - bc1), float(1.0 - bc2), st);
   }
   if (hipGetLastError() != hipSuccess) { cleanup(); set_error("surrogate: training launch failed"); return -2; }
   float last_loss = 0.f;
