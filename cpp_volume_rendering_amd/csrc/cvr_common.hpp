@@ -161,6 +161,30 @@ __device__ __forceinline__ float sample_volume_oct(const VolGeom& g, f3 p) {
   return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
 }
 
+// The same read split in two, for marches that issue the loads of later samples early:
+// the corner block holding the taps of p, then the lerp of that block at p.
+template <typename T>
+__device__ __forceinline__ typename VolOct<T>::type volume_oct_load(const VolGeom& g, f3 p) {
+  typedef typename VolOct<T>::type RT;
+  int x0, y0, z0;
+  float fx, fy, fz;
+  quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
+  quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
+  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  return static_cast<const RT*>(g.oct)[brick_index(x0, y0, z0, g.nbx, g.nby)];
+}
+template <typename T>
+__device__ __forceinline__ float volume_oct_lerp(const VolGeom& g, typename VolOct<T>::type r, f3 p) {
+  int x0, y0, z0;
+  float fx, fy, fz;
+  quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
+  quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
+  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  float4 a, b;
+  VolOct<T>::f(r, a, b);
+  return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
+}
+
 // Trilinear sample of the raw volume at world-in-box position p (texture space
 // [0, G]); returns the normalised density v/255 or v/65535
 // (structuredgridvolume.cpp:121-151 + GL_LINEAR, utils.cpp:8-9).

@@ -14,12 +14,18 @@
 // workgroup a 16x16 tile; workgroups are re-ordered so each XCD marches one
 // contiguous band of tiles.  The 1D transfer function is staged in LDS.  The
 // sample position is recomputed from the sample index each step (no running
-// sum), so the step schedule does not drift in fp32.
+// sum), so the step schedule does not drift in fp32.  Positions do not depend on the
+// compositing, so the march is software-pipelined: the corner-block loads of the next
+// kGroup samples are in flight while the current kGroup are composited.
 #include "cvr_ctx.hpp"
+#include <type_traits>
 
 namespace cvr {
 
-template <typename T, bool TF_LDS>
+constexpr int kGroup = 4;  // samples whose volume loads are issued together
+template <class T> struct VoxTag { typedef T type; };
+
+template <typename T, bool TF_LDS, bool PH>
 __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const float4* __restrict__ tf, int ntf,
                                                float step, float et, PhongK ph, const float4* __restrict__ grad,
                                                float ka, float kd, float4* __restrict__ out,
@@ -47,31 +53,56 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
   if (ray_box(eye, dir, g.G, tn, tf_)) {
     const float D = fabsf(tf_ - tn);
     const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-    for (int j = 0; j < kMaxSamples; ++j) {
+    typedef typename VolOct<T>::type RT;
+    // sample j sits at the midpoint of [s, s + h], s = j step, h = min(step, D - s); for
+    // j past the end the position is still finite and its clamped block in bounds
+    auto pos = [&](int j) {
       const float s = float(j) * step;
-      if (!(s < D)) break;
-      const float h = fminf(step, D - s);
-      const f3 p = tp + dir * (s + 0.5f * h);
-      const float dens = sample_volume_oct<T>(g, p);
-      float4 src = sample_tf(tfp, ntf, dens);
-      ++nsamp;
-      if (src.w > 0.0f) {
-        ++nshade;
-        float dd, sp;
-        if (ph.on && phong_terms(grad, g, ph, p, eye, dd, sp)) {
-          const float m = ka + kd * dd, s = ph.ks * sp;
-          src.x = src.x * m + ph.spec[0] * s;
-          src.y = src.y * m + ph.spec[1] * s;
-          src.z = src.z * m + ph.spec[2] * s;
+      return tp + dir * (s + 0.5f * fminf(step, D - s));
+    };
+    // Samples in groups of kGroup; while one group is composited the loads of the next
+    // are in flight (two register sets, a and b, alternate).  Returns true when the ray
+    // ends inside the group.
+    auto group = [&](const RT* rec, int j0) {
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) {
+        const int j = j0 + k;
+        const float s = float(j) * step;
+        if (!(s < D) || j >= kMaxSamples) return true;
+        const float h = fminf(step, D - s);
+        const f3 p = tp + dir * (s + 0.5f * h);
+        float4 src = sample_tf(tfp, ntf, volume_oct_lerp<T>(g, rec[k], p));
+        ++nsamp;
+        if (src.w > 0.0f) {
+          ++nshade;
+          float dd, sp;
+          if (PH && phong_terms(grad, g, ph, p, eye, dd, sp)) {
+            const float m = ka + kd * dd, s2 = ph.ks * sp;
+            src.x = src.x * m + ph.spec[0] * s2;
+            src.y = src.y * m + ph.spec[1] * s2;
+            src.z = src.z * m + ph.spec[2] * s2;
+          }
+          const float a = 1.0f - __expf(-src.w * h);
+          const float wgt = (1.0f - dst.w) * a;
+          dst.x += wgt * src.x;
+          dst.y += wgt * src.y;
+          dst.z += wgt * src.z;
+          dst.w += wgt;
+          if (dst.w > et) return true;
         }
-        const float a = 1.0f - __expf(-src.w * h);
-        const float wgt = (1.0f - dst.w) * a;
-        dst.x += wgt * src.x;
-        dst.y += wgt * src.y;
-        dst.z += wgt * src.z;
-        dst.w += wgt;
-        if (dst.w > et) break;
       }
+      return false;
+    };
+    RT a[kGroup], b[kGroup];
+#pragma unroll
+    for (int k = 0; k < kGroup; ++k) a[k] = volume_oct_load<T>(g, pos(k));
+    for (int j0 = 0;; j0 += 2 * kGroup) {
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) b[k] = volume_oct_load<T>(g, pos(j0 + kGroup + k));
+      if (group(a, j0)) break;
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) a[k] = volume_oct_load<T>(g, pos(j0 + 2 * kGroup + k));
+      if (group(b, j0 + kGroup)) break;
     }
   }
   out[size_t(py) * rf.W + px] = dst;
@@ -116,21 +147,24 @@ int launch_rc1p(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* o
   size_t shm = lds ? size_t(c->tf_size) * sizeof(float4) : 0;
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   float4* o4 = reinterpret_cast<float4*>(out);
-  if (c->dtype == CVR_U8) {
-    if (lds)
-      hipLaunchKernelGGL((k_rc1p<uint8_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
-    else
-      hipLaunchKernelGGL((k_rc1p<uint8_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
-  } else {
-    if (lds)
-      hipLaunchKernelGGL((k_rc1p<uint16_t, true>), dim3(tiles), dim3(256), shm, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
-    else
-      hipLaunchKernelGGL((k_rc1p<uint16_t, false>), dim3(tiles), dim3(256), 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
-                         step, et, ph, gr, P.ka, P.kd, o4, cnt);
-  }
+  // the Blinn-Phong branch is a compile-time variant: without it the unrolled groups of
+  // the pipelined march stay small enough for twice the waves per SIMD
+  auto launch = [&](auto t, auto l, auto p) {
+    using T = typename decltype(t)::type;
+    constexpr bool L = decltype(l)::value, PHv = decltype(p)::value;
+    hipLaunchKernelGGL((k_rc1p<T, L, PHv>), dim3(tiles), dim3(256), L ? shm : 0, st, rf, g, c->d_tf_rgbt, c->tf_size,
+                       step, et, ph, gr, P.ka, P.kd, o4, cnt);
+  };
+  auto by_phong = [&](auto t, auto l) {
+    if (ph.on) launch(t, l, std::true_type{});
+    else launch(t, l, std::false_type{});
+  };
+  auto by_lds = [&](auto t) {
+    if (lds) by_phong(t, std::true_type{});
+    else by_phong(t, std::false_type{});
+  };
+  if (c->dtype == CVR_U8) by_lds(VoxTag<uint8_t>{});
+  else by_lds(VoxTag<uint16_t>{});
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }
