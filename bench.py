@@ -150,7 +150,8 @@ def read_footprint(tech, r, wl, bpv, light_cache):
     if light_cache:
         return total + 32 ** 3 * 8
     if tech == N.EBS:
-        total += (nx + 2) * (ny + 2) * (nz + 2) * 64
+        # the (hi, lo) SAT as the march reads it: linear 8-byte cells, or 64-byte corner blocks
+        total += (nx + 2) * (ny + 2) * (nz + 2) * (8 if r.params().sat_layout == 1 else 64)
     if tech in (N.DOS, N.CONE_TRACE, N.CONE_NN):
         p = r.params()
         prec = p.density_precision if tech == N.DOS else p.cone.density_precision

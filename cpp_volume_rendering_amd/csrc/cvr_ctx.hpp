@@ -18,6 +18,14 @@ struct PyrGeom {
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
 };
+// One box of the EBS directional-light cone chain (offsets from the sample, the
+// CUT_WHEN_AWAY_FROM_VOLUME coordinate apos + interval, 1 / box volume in voxels).
+struct ShadowRow {
+  float o1[3], o2[3];
+  float next;
+  float inv_vq;
+};
+static_assert(sizeof(ShadowRow) == 32, "two float4 per row");
 // Inputs a built pyramid depends on, compared field by field before reuse.
 struct PyrKey {
   uint64_t vol_v, tf_v;
@@ -71,6 +79,7 @@ struct cvr_ctx {
   void* d_sat_oct = nullptr;  // corner blocks of the SAT, 64 B per cell (k_sat.hip)
   float* d_ext_lut = nullptr;  // per raw value extinction (GetExtN), 256 or 65536 entries
   uint64_t sat_built_for = ~0ull;
+  uint64_t sat_oct_built_for = ~0ull;  // d_sat_oct holds the blocks of this SAT
 
   // Gaussian extinction pyramid (cone tracers, surrogate)
   float* d_pyr = nullptr;
@@ -109,6 +118,10 @@ struct cvr_ctx {
   float4* d_grad = nullptr;
   uint64_t grad_built_for = ~0ull;
 
+  // EBS directional shadow chain: host copy of the last table and its device copy
+  std::vector<cvr::ShadowRow> ebs_rows, ebs_rows_dev;
+  void* d_ebs_rows = nullptr;
+
   // object-space light cache (EBS, DOS): (occlusion, shadow) per cell, x fastest
   float2* d_lc = nullptr;
   size_t lc_cap = 0;
@@ -143,7 +156,8 @@ int launch_rc1p(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* o
 // Blinn-Phong constants of a render (builds the gradient when shading is on)
 int phong_setup(cvr_ctx* c, int on, int gradient_type, float ks, float shininess, const float spec[3],
                 const RayFrame& rf, const cvr_light* light, PhongK& ph, hipStream_t st);
-int build_sat(cvr_ctx* c, hipStream_t st);
+// the SAT of the current volume and TF; with_blocks: also its corner blocks
+int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks = false);
 // volume gradient of `type` for Blinn-Phong shading
 int build_gradient(cvr_ctx* c, int type, hipStream_t st);
 // light cache: validates the resolution and sizes the buffer; the march over it

@@ -14,6 +14,7 @@
 // cone in fp32 with the same formulas.
 #include "cvr_ctx.hpp"
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace cvr {
@@ -113,12 +114,7 @@ __host__ __device__ inline void cone_box(const ConeGeom<R>& g, R apos, R o1[3], 
   o1[sb] = b1; o2[sb] = b2;
 }
 
-// One box of the directional-light cone chain.
-struct ShadowRow {
-  float o1[3], o2[3];
-  float next;     // apos + interval (the CUT_WHEN_AWAY_FROM_VOLUME coordinate)
-  float inv_vq;   // 1 / box volume in voxels
-};
+constexpr int kMaxShadowRows = 4096;
 
 struct EbsK {
   VolGeom g;
@@ -165,8 +161,45 @@ __device__ __forceinline__ double sat_tap(const float4* __restrict__ sat, const 
   return c0 * (1 - fz) + c1 * fz;
 }
 
+// The same tap through the linear (hi, lo) SAT, 8 bytes per cell: each of the four rows
+// (y0 | y0+1, z0 | z0+1) gives its x-pair (x0, x0+1) in one 16-byte load.  x0 in
+// [0, n-2] with the weight clamped to [0, 1] is CLAMP_TO_EDGE at both ends (below the
+// first texel: cell 0; past the last: cell n-1); inside, the same taps and weights.
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ void pair_axis(double x, int n, int& i0, double& f) {
+  const double fl = floor(x);
+  i0 = min(max(int(fl), 0), n - 2);
+  f = fmin(fmax(x - double(i0), 0.0), 1.0);
+}
+__device__ __forceinline__ double sat_tap_lin(const float2* __restrict__ sat, const EbsK& k, double x, double y,
+                                              double z) {
+  int x0, y0, z0;
+  double fx, fy, fz;
+  pair_axis(x, k.w, x0, fx);
+  pair_axis(y, k.h, y0, fy);
+  pair_axis(z, k.d, z0, fz);
+  const size_t b = (size_t(z0) * k.h + y0) * k.w + x0, row = size_t(k.w), plane = size_t(k.w) * k.h;
+  const f4a8 q0 = *reinterpret_cast<const f4a8*>(sat + b);
+  const f4a8 q1 = *reinterpret_cast<const f4a8*>(sat + b + row);
+  const f4a8 q2 = *reinterpret_cast<const f4a8*>(sat + b + plane);
+  const f4a8 q3 = *reinterpret_cast<const f4a8*>(sat + b + plane + row);
+  const double c000 = hilo(q0.x, q0.y), c100 = hilo(q0.z, q0.w), c010 = hilo(q1.x, q1.y), c110 = hilo(q1.z, q1.w);
+  const double c001 = hilo(q2.x, q2.y), c101 = hilo(q2.z, q2.w), c011 = hilo(q3.x, q3.y), c111 = hilo(q3.z, q3.w);
+  const double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
+  const double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
+  const double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
+  return c0 * (1 - fz) + c1 * fz;
+}
+// A SAT tap in the layout L: 0 corner blocks (64 bytes per cell), 1 linear pairs.
+template <int L>
+__device__ __forceinline__ double tap(const void* __restrict__ sat, const EbsK& k, double x, double y, double z) {
+  if constexpr (L == 0) return sat_tap(static_cast<const float4*>(sat), k, x, y, z);
+  else return sat_tap_lin(static_cast<const float2*>(sat), k, x, y, z);
+}
+
 // EvaluateSAT3D (:87-100) after the border offset and clamp (:102-110 / :181-183).
-__device__ __forceinline__ double box_sum(const float4* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
+template <int L>
+__device__ __forceinline__ double box_sum(const void* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
   p1 = mk3(fminf(fmaxf(p1.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
            fminf(fmaxf(p1.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
            fminf(fmaxf(p1.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
@@ -176,24 +209,25 @@ __device__ __forceinline__ double box_sum(const float4* __restrict__ sat, const 
   const double x1 = double(p1.x) * k.sat_tex_mul[0] - 0.5, x2 = double(p2.x) * k.sat_tex_mul[0] - 0.5;
   const double y1 = double(p1.y) * k.sat_tex_mul[1] - 0.5, y2 = double(p2.y) * k.sat_tex_mul[1] - 0.5;
   const double z1 = double(p1.z) * k.sat_tex_mul[2] - 0.5, z2 = double(p2.z) * k.sat_tex_mul[2] - 0.5;
-  double V1 = sat_tap(sat, k, x2, y2, z2), V2 = sat_tap(sat, k, x1, y2, z2);
-  double V3 = sat_tap(sat, k, x2, y2, z1), V4 = sat_tap(sat, k, x1, y2, z1);
-  double V5 = sat_tap(sat, k, x2, y1, z2), V6 = sat_tap(sat, k, x1, y1, z2);
-  double V7 = sat_tap(sat, k, x2, y1, z1), V8 = sat_tap(sat, k, x1, y1, z1);
+  double V1 = tap<L>(sat, k, x2, y2, z2), V2 = tap<L>(sat, k, x1, y2, z2);
+  double V3 = tap<L>(sat, k, x2, y2, z1), V4 = tap<L>(sat, k, x1, y2, z1);
+  double V5 = tap<L>(sat, k, x2, y1, z2), V6 = tap<L>(sat, k, x1, y1, z2);
+  double V7 = tap<L>(sat, k, x2, y1, z1), V8 = tap<L>(sat, k, x1, y1, z1);
   return V1 - V2 - V3 + V4 - V5 + V6 + V7 - V8;
 }
 
 // ExtinctionAmbientOcclusion (:112-146).
-__device__ float ebs_ao(const float4* __restrict__ sat, const EbsK& k, f3 tx) {
+template <int L>
+__device__ float ebs_ao(const void* __restrict__ sat, const EbsK& k, f3 tx) {
   const f3 sc = ld3(k.scale);
   const double rad = k.ao_radius;
-  double S0 = box_sum(sat, k, tx - sc * k.ao_radius, tx + sc * k.ao_radius);
+  double S0 = box_sum<L>(sat, k, tx - sc * k.ao_radius, tx + sc * k.ao_radius);
   double t = S0 * (1.0 / (rad * rad));
   double Si = S0;
   for (int i = 1; i < k.shells; ++i) {
     const float r1f = k.ao_radius * float(i + 1);
     const double r1 = rad * double(i + 1);
-    double S1 = box_sum(sat, k, tx - sc * r1f, tx + sc * r1f);
+    double S1 = box_sum<L>(sat, k, tx - sc * r1f, tx + sc * r1f);
     t = t + (S1 - Si) * (1.0 / (r1 * r1));
     Si = S1;
   }
@@ -204,7 +238,8 @@ __device__ float ebs_ao(const float4* __restrict__ sat, const EbsK& k, f3 tx) {
 __device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
 // EvaluateShadowSAT3D (:148-188) summed along the cone; directional light from the table.
-__device__ float ebs_shadow_table(const float4* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
+template <int L>
+__device__ float ebs_shadow_table(const void* __restrict__ sat, const EbsK& k, const ShadowRow* rows, int nrows,
                                   f3 tx, unsigned int& nbox) {
   const float pa = comp(tx, k.axis), lo = k.minVol[k.axis], hi = k.maxVol[k.axis];
   double st = 0.0;
@@ -213,14 +248,15 @@ __device__ float ebs_shadow_table(const float4* __restrict__ sat, const EbsK& k,
     const float nxt = pa + R.next;
     if (!(nxt > lo && nxt < hi)) break;
     f3 p1 = tx + ld3(R.o1), p2 = tx + ld3(R.o2);
-    st += box_sum(sat, k, p1, p2) * double(R.inv_vq) * double(k.sdw_weight);
+    st += box_sum<L>(sat, k, p1, p2) * double(R.inv_vq) * double(k.sdw_weight);
     ++nbox;
   }
   return __expf(-float(st));
 }
 
 // Point light: the cone axis depends on the sample (:437-438).
-__device__ float ebs_shadow_point(const float4* __restrict__ sat, const EbsK& k, f3 tx, unsigned int& nbox) {
+template <int L>
+__device__ float ebs_shadow_point(const void* __restrict__ sat, const EbsK& k, f3 tx, unsigned int& nbox) {
   f3 real = tx - mk3(0.5f * k.g.G[0], 0.5f * k.g.G[1], 0.5f * k.g.G[2]);
   f3 cvv = norm3(ld3(k.light_pos) - real);
   float cv[3] = {cvv.x, cvv.y, cvv.z};
@@ -238,65 +274,39 @@ __device__ float ebs_shadow_point(const float4* __restrict__ sat, const EbsK& k,
     f3 p1 = tx + ld3(o1), p2 = tx + ld3(o2);
     const float vq = (fabsf(p1.x - p2.x) / k.scale[0]) * (fabsf(p1.y - p2.y) / k.scale[1]) *
                      (fabsf(p1.z - p2.z) / k.scale[2]);
-    st += (box_sum(sat, k, p1, p2) / double(vq)) * double(k.sdw_weight);
+    st += (box_sum<L>(sat, k, p1, p2) / double(vq)) * double(k.sdw_weight);
     ++nbox;
     apos = apos + cg.interval;
   }
   return __expf(-float(st));
 }
 
-// Directional light: the cone chain is the same for every sample.  The workgroup builds
-// it in FP64 (the decisions in ceil() and the distance test then match a double
-// evaluation of :190-430 exactly) into LDS; row r is the r-th box.  *s_nrows enters as
-// k.nrows (set and synchronised by the caller) and leaves as the chain length.
-__device__ void ebs_build_rows(const EbsK& k, ShadowRow* s_rows, int* s_nrows) {
-  if (k.apply_sdw && k.shadow_type == 1) {
-    double cv[3] = {double(k.light_fwd[0]), double(k.light_fwd[1]), double(k.light_fwd[2])};
-    nrm(cv);
-    const double scd[3] = {double(k.scale[0]), double(k.scale[1]), double(k.scale[2])};
-    ConeGeom<double> cg;
-    cone_setup<double>(cv, double(k.sdw_angle), double(k.sdw_interval), double(k.sdw_initial), scd, cg);
-    for (int r = threadIdx.x; r < k.nrows; r += blockDim.x) {
-      double apos = cg.apos0;
-      for (int i = 0; i < r; ++i) apos = apos + cg.interval;
-      if (!((apos / cv[cg.axis]) < double(k.sdw_max_dist))) {
-        atomicMin(s_nrows, r);
-        continue;
-      }
-      double o1[3], o2[3];
-      cone_box<double>(cg, apos, o1, o2);
-      ShadowRow row;
-      double vq = 1.0;
-      for (int i = 0; i < 3; ++i) {
-        row.o1[i] = float(o1[i]);
-        row.o2[i] = float(o2[i]);
-        vq *= fabs(o1[i] - o2[i]) / scd[i];
-      }
-      row.next = float(apos + cg.interval);
-      row.inv_vq = float(1.0 / vq);
-      s_rows[r] = row;
-    }
-    __syncthreads();
-  }
+// Directional light: the cone chain is the same for every sample.  The host builds it
+// once per render in FP64 (ebs_rows: the decisions in ceil() and the distance test then
+// match a double evaluation of :190-430 exactly); each workgroup copies the rows into
+// LDS.  Row r is the r-th box.
+__device__ __forceinline__ void ebs_load_rows(const ShadowRow* __restrict__ rows, int nrows, ShadowRow* s_rows) {
+  const float4* src = reinterpret_cast<const float4*>(rows);
+  float4* dst = reinterpret_cast<float4*>(s_rows);
+  for (int i = threadIdx.x; i < nrows * 2; i += blockDim.x) dst[i] = src[i];
+  __syncthreads();
 }
 
-template <typename T>
+template <typename T, int L>
 // Two waves per SIMD: left to itself the compiler takes 256 VGPRs + 7 AGPRs, one wave
 // more than the register file holds twice, and the latency-bound SAT reads then run at
 // one wave per SIMD (256^3 @ 1024^2 with shadows: 82.6 -> 75.7 ms, the same frame).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
-                                              const float4* __restrict__ sat, float step, float et, PhongK ph,
+                                              const void* __restrict__ sat, const ShadowRow* __restrict__ rows,
+                                              float step, float et, PhongK ph,
                                               const float4* __restrict__ grad, float4* __restrict__ out,
                                               unsigned long long* __restrict__ counters) {
   extern __shared__ float4 smem[];
-  __shared__ int s_nrows;
   float4* s_tf = smem;
   ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem + ntf);
   for (int i = threadIdx.x; i < ntf; i += blockDim.x) s_tf[i] = tf[i];
-  if (threadIdx.x == 0) s_nrows = k.nrows;
-  __syncthreads();
-  ebs_build_rows(k, s_rows, &s_nrows);
-  const int nrows = s_nrows;
+  ebs_load_rows(rows, k.nrows, s_rows);
+  const int nrows = k.nrows;
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tiles_x = (rf.W + 15) >> 4;
   int tx_, ty_;
@@ -328,11 +338,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         ++nshade;
         float Io = 0.f, Is = 0.f;
         if (k.apply_occ) {
-          Io = ebs_ao(sat, k, p);
+          Io = ebs_ao<L>(sat, k, p);
           nbox += unsigned(k.shells);
         }
         if (k.apply_sdw)
-          Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, p, nbox) : ebs_shadow_point(sat, k, p, nbox);
+          Is = k.shadow_type == 1 ? ebs_shadow_table<L>(sat, k, s_rows, nrows, p, nbox)
+                                  : ebs_shadow_point<L>(sat, k, p, nbox);
         if (ph.on) {
           // Phong branch (:523-543): a sample whose gradient vanishes keeps its colour
           float dd, sp;
@@ -369,15 +380,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // Object-space light cache (rc1pextbsd/lightcachecomputation.comp:440-468): cell i holds
 // (AO, shadow) at texture-space position (i + 1/2) * cell, 1 for a term that is off,
 // from the renderer's own AO and shadow evaluations.  One lane per cell.
-__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const float4* __restrict__ sat, int rx, int ry, int rz, f3 cell,
-                                                float2* __restrict__ lc) {
+template <int L>
+__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const void* __restrict__ sat, const ShadowRow* __restrict__ rows,
+                                                int rx, int ry, int rz, f3 cell, float2* __restrict__ lc) {
   extern __shared__ float4 smem[];
-  __shared__ int s_nrows;
   ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem);
-  if (threadIdx.x == 0) s_nrows = k.nrows;
-  __syncthreads();
-  ebs_build_rows(k, s_rows, &s_nrows);
-  const int nrows = s_nrows;
+  ebs_load_rows(rows, k.nrows, s_rows);
+  const int nrows = k.nrows;
   const size_t n = size_t(rx) * ry * rz;
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -385,9 +394,10 @@ __global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const float4* __restrict
   const f3 tp = mk3((float(x) + 0.5f) * cell.x, (float(y) + 0.5f) * cell.y, (float(z) + 0.5f) * cell.z);
   unsigned int nbox = 0;
   float Io = 1.f, Is = 1.f;
-  if (k.apply_occ) Io = ebs_ao(sat, k, tp);
+  if (k.apply_occ) Io = ebs_ao<L>(sat, k, tp);
   if (k.apply_sdw)
-    Is = k.shadow_type == 1 ? ebs_shadow_table(sat, k, s_rows, nrows, tp, nbox) : ebs_shadow_point(sat, k, tp, nbox);
+    Is = k.shadow_type == 1 ? ebs_shadow_table<L>(sat, k, s_rows, nrows, tp, nbox)
+                            : ebs_shadow_point<L>(sat, k, tp, nbox);
   lc[i] = make_float2(Io, Is);
 }
 
@@ -401,7 +411,7 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     set_error("cvr_render: EBS ao_shells >= 1, ao_radius > 0 and sdw_interval > 0 required");
     return -1;
   }
-  int rc = build_sat(c, st);
+  int rc = build_sat(c, st, P.sat_layout == 0);
   if (rc) return rc;
   EbsK k{};
   k.g = geom_of(c);
@@ -432,8 +442,8 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
   } else {
     for (int i = 0; i < 3; ++i) { k.light_pos[i] = rf.eye[i]; k.light_fwd[i] = -rf.f[i]; }
   }
-  // directional light: LDS rows for the cone chain; |apos| grows by interval*scale per
-  // row and the chain stops once |apos| / |cv_axis| >= max distance, so this bounds it
+  // directional light: the cone chain in FP64 on the host; |apos| grows by interval*scale
+  // per row and the chain stops once |apos| / |cv_axis| >= max distance, which bounds it
   k.nrows = 0;
   if (P.apply_shadow && P.shadow_type == 1) {
     double cv[3] = {k.light_fwd[0], k.light_fwd[1], k.light_fwd[2]};
@@ -442,11 +452,44 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     k.axis = (ac[2] > ac[0] && ac[2] > ac[1]) ? 2 : (ac[1] > ac[0] ? 1 : 0);
     const double per_row = double(k.sdw_interval) * double(k.scale[k.axis]);
     const double n = double(k.sdw_max_dist) / per_row + std::fabs(double(k.sdw_initial)) / double(k.sdw_interval) + 2;
-    if (!(n < 4096)) {
+    if (!(n < kMaxShadowRows)) {
       set_error("cvr_render: EBS shadow chain longer than 4096 boxes");
       return -4;
     }
-    k.nrows = int(n);
+    const double scd[3] = {double(k.scale[0]), double(k.scale[1]), double(k.scale[2])};
+    ConeGeom<double> cg;
+    cone_setup<double>(cv, double(k.sdw_angle), double(k.sdw_interval), double(k.sdw_initial), scd, cg);
+    std::vector<ShadowRow>& rows = c->ebs_rows;
+    rows.clear();
+    double apos = cg.apos0;
+    for (int r = 0; r < int(n); ++r, apos = apos + cg.interval) {
+      if (!((apos / cv[cg.axis]) < double(k.sdw_max_dist))) break;
+      double o1[3], o2[3];
+      cone_box<double>(cg, apos, o1, o2);
+      ShadowRow row;
+      double vq = 1.0;
+      for (int i = 0; i < 3; ++i) {
+        row.o1[i] = float(o1[i]);
+        row.o2[i] = float(o2[i]);
+        vq *= std::fabs(o1[i] - o2[i]) / scd[i];
+      }
+      row.next = float(apos + cg.interval);
+      row.inv_vq = float(1.0 / vq);
+      rows.push_back(row);
+    }
+    k.nrows = int(rows.size());
+    if (!c->d_ebs_rows && hipMalloc(&c->d_ebs_rows, kMaxShadowRows * sizeof(ShadowRow)) != hipSuccess) {
+      c->d_ebs_rows = nullptr;
+      set_error("cvr_render: out of device memory (shadow rows)");
+      return -5;
+    }
+    // the table is rewritten only when it changes: a render still in flight may read it
+    if (k.nrows > 0 && (c->ebs_rows_dev.size() != rows.size() ||
+                        std::memcmp(c->ebs_rows_dev.data(), rows.data(), rows.size() * sizeof(ShadowRow)) != 0)) {
+      CVR_HIP_CHECK(hipStreamSynchronize(st));
+      CVR_HIP_CHECK(hipMemcpy(c->d_ebs_rows, rows.data(), rows.size() * sizeof(ShadowRow), hipMemcpyHostToDevice));
+      c->ebs_rows_dev = rows;
+    }
   }
   const float step = P.step_size > 0 ? P.step_size : default_step(c);
   PhongK ph;
@@ -462,8 +505,17 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     const size_t n = size_t(rx) * ry * rz;
     // one wave per workgroup: a 32^3 cache is 512 waves, which then spread over every CU
     // instead of half of them (light-cache frame 0.927 -> 0.874 ms, the same cache)
-    hipLaunchKernelGGL(k_lc_ebs, dim3(unsigned((n + 63) / 64)), dim3(64), size_t(k.nrows) * sizeof(ShadowRow), st, k,
-                       static_cast<const float4*>(c->d_sat_oct), rx, ry, rz, cell, c->d_lc);
+    if (size_t(k.nrows) * sizeof(ShadowRow) > 64 * 1024) {
+      set_error("cvr_render: EBS light cache: shadow chain of more than 2048 boxes does not fit LDS");
+      return -4;
+    }
+    const ShadowRow* lrows = static_cast<const ShadowRow*>(c->d_ebs_rows);
+    if (P.sat_layout == 1)
+      hipLaunchKernelGGL(k_lc_ebs<1>, dim3(unsigned((n + 63) / 64)), dim3(64), size_t(k.nrows) * sizeof(ShadowRow), st,
+                         k, static_cast<const void*>(c->d_sat), lrows, rx, ry, rz, cell, c->d_lc);
+    else
+      hipLaunchKernelGGL(k_lc_ebs<0>, dim3(unsigned((n + 63) / 64)), dim3(64), size_t(k.nrows) * sizeof(ShadowRow), st,
+                         k, static_cast<const void*>(c->d_sat_oct), lrows, rx, ry, rz, cell, c->d_lc);
     CVR_HIP_CHECK(hipGetLastError());
     return launch_lc_render(c, rf, P.apply_occlusion, P.apply_shadow, P.ka, P.kd, step, P.et_threshold, ph, out, st);
   }
@@ -474,12 +526,24 @@ int launch_ebs(cvr_ctx* c, const RayFrame& rf, const cvr_light* light, float* ou
     return -4;
   }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_ebs<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, static_cast<const float4*>(c->d_sat_oct),
-                       step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
-  else
-    hipLaunchKernelGGL(k_ebs<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, static_cast<const float4*>(c->d_sat_oct),
-                       step, P.et_threshold, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
+  const void* sat = P.sat_layout == 1 ? static_cast<const void*>(c->d_sat) : c->d_sat_oct;
+  const ShadowRow* rows = static_cast<const ShadowRow*>(c->d_ebs_rows);
+  float4* o4 = reinterpret_cast<float4*>(out);
+  if (c->dtype == CVR_U8) {
+    if (P.sat_layout == 1)
+      hipLaunchKernelGGL((k_ebs<uint8_t, 1>), dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, sat, rows, step,
+                         P.et_threshold, ph, c->d_grad, o4, cnt);
+    else
+      hipLaunchKernelGGL((k_ebs<uint8_t, 0>), dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, sat, rows, step,
+                         P.et_threshold, ph, c->d_grad, o4, cnt);
+  } else {
+    if (P.sat_layout == 1)
+      hipLaunchKernelGGL((k_ebs<uint16_t, 1>), dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, sat,
+                         rows, step, P.et_threshold, ph, c->d_grad, o4, cnt);
+    else
+      hipLaunchKernelGGL((k_ebs<uint16_t, 0>), dim3(tiles), dim3(256), shm, st, rf, k, c->d_tf_rgbt, c->tf_size, sat,
+                         rows, step, P.et_threshold, ph, c->d_grad, o4, cnt);
+  }
   CVR_HIP_CHECK(hipGetLastError());
   return 0;
 }

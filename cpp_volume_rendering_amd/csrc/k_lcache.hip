@@ -143,6 +143,10 @@ int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sd
   L.ks = apply_sdw ? ph.ks : 0.f;
   const int tiles = ((rf.W + 15) / 16) * ((rf.H + 15) / 16);
   const size_t shm = size_t(c->tf_size) * sizeof(float4);
+  if (shm > 64 * 1024) {
+    set_error("cvr_render: light-cache march: transfer function larger than 4096 entries (staged in LDS)");
+    return -4;
+  }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
   if (c->dtype == CVR_U8)
     hipLaunchKernelGGL(k_lc_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,

@@ -137,9 +137,34 @@ __global__ __launch_bounds__(256) void k_sat_oct(const float2* __restrict__ sat,
   o[3] = make_float4(i.x, i.y, j.x, j.y);
 }
 
-int build_sat(cvr_ctx* c, hipStream_t st) {
+// Corner blocks of a built SAT (64 bytes per cell), for the marches that read them.
+static int build_sat_oct(cvr_ctx* c, hipStream_t st) {
+  const int w = c->nx + 2, h = c->ny + 2, d = c->nz + 2;
+  const size_t cells = size_t(w) * h * d;
+  if (!c->d_sat_oct) {
+    if (hipMalloc(&c->d_sat_oct, cells * 4 * sizeof(float4)) != hipSuccess) {
+      c->d_sat_oct = nullptr;
+      set_error("cvr_prepare: out of device memory for the summed-area table's corner blocks");
+      return -5;
+    }
+  }
+  if (d > 65535) { set_error("summed-area table: depth above 65535"); return -4; }
+  hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned((h + 3) / 4), unsigned(d)), dim3(256), 0, st,
+                     c->d_sat, static_cast<float4*>(c->d_sat_oct), w, h, d);
+  CVR_HIP_CHECK(hipGetLastError());
+  c->sat_oct_built_for = c->sat_built_for;
+  return 0;
+}
+
+int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks) {
   const uint64_t key = (c->vol_version << 32) ^ c->tf_version;
-  if (c->d_sat && c->sat_built_for == key) return 0;
+  if (c->d_sat && c->sat_built_for == key) {
+    if (with_blocks && !(c->d_sat_oct && c->sat_oct_built_for == key)) {
+      if (const int rc = build_sat_oct(c, st)) return rc;
+      CVR_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    return 0;
+  }
   const int w = c->nx + 2, h = c->ny + 2, d = c->nz + 2;
   const size_t cells = size_t(w) * h * d;
   if (!c->d_sat) {
@@ -178,18 +203,11 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
                      size_t(w) * h);
   CVR_HIP_CHECK(hipGetLastError());
   if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[1], st));
-  // corner blocks for the render kernels' taps
-  if (!c->d_sat_oct) {
-    if (hipMalloc(&c->d_sat_oct, cells * 4 * sizeof(float4)) != hipSuccess) {
-      c->d_sat_oct = nullptr;
-      set_error("cvr_prepare: out of device memory for the summed-area table's corner blocks");
-      return -5;
-    }
-  }
-  if (d > 65535) { set_error("summed-area table: depth above 65535"); return -4; }
-  hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned((h + 3) / 4), unsigned(d)), dim3(256), 0, st,
-                     c->d_sat, static_cast<float4*>(c->d_sat_oct), w, h, d);
-  CVR_HIP_CHECK(hipGetLastError());
+  c->sat_built_for = key;
+  c->sat_oct_built_for = ~0ull;
+  // corner blocks for the marches that read them
+  if (with_blocks)
+    if (const int rc = build_sat_oct(c, st)) return rc;
   if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[2], st));
   // the LUT is read by pass 1 only; keep it until the stream has consumed it
   CVR_HIP_CHECK(hipStreamSynchronize(st));
@@ -198,9 +216,8 @@ int build_sat(cvr_ctx* c, hipStream_t st) {
     CVR_HIP_CHECK(hipEventElapsedTime(&a, c->ev_sat[0], c->ev_sat[1]));
     CVR_HIP_CHECK(hipEventElapsedTime(&b, c->ev_sat[1], c->ev_sat[2]));
     c->stats.sat_scan_ms = a;
-    c->stats.sat_blocks_ms = b;
+    c->stats.sat_blocks_ms = with_blocks ? b : 0.0;
   }
-  c->sat_built_for = key;
   return 0;
 }
 

@@ -55,7 +55,7 @@ class EbsParams(C.Structure):
                 ("sdw_weight", C.c_float), ("sdw_max_dist", C.c_float), ("shadow_type", C.c_int),
                 ("ka", C.c_float), ("kd", C.c_float), ("light_cache", C.c_int), ("light_cache_res", C.c_int * 3),
                 ("apply_phong", C.c_int), ("gradient_type", C.c_int), ("ks", C.c_float), ("shininess", C.c_float),
-                ("specular", C.c_float * 3)]
+                ("specular", C.c_float * 3), ("sat_layout", C.c_int)]
 
 
 class GtParams(C.Structure):

@@ -77,9 +77,11 @@ def assert_parity(img, ref, m, stride=1, max_over_frac=0.005):
 
 
 @pytest.mark.parametrize("shadow_type", [1, 0])
-def test_ebs_small(shadow_type):
+@pytest.mark.parametrize("sat_layout", [1, 0])
+def test_ebs_small(shadow_type, sat_layout):
+    """Both SAT layouts the march can read (linear (hi, lo) pairs, corner blocks)."""
     wl = W.banded_noise(n=48, width=64, height=64, seed=12)
-    img, ref, m = ebs_pair(wl, 64, 64, shadow_type=shadow_type)
+    img, ref, m = ebs_pair(wl, 64, 64, shadow_type=shadow_type, sat_layout=sat_layout)
     assert_parity(img, ref, m)
     assert ref[..., :3].max() > 0.05
 
@@ -111,8 +113,11 @@ def test_ebs_full_volume_sparse_pixels():
 
 def test_ebs_full_config_1024():
     """The EBS config at its own frame size: 256^3 uint16 banded noise, directional light,
-    1024^2; the oracle renders every 32nd pixel (x and y) on its FP64 SAT."""
+    1024^2; the oracle renders every 32nd pixel (x and y) on its FP64 SAT.  The two SAT
+    layouts give the same frame to fp32 rounding."""
     wl = W.banded_noise(n=256, width=1024, height=1024)
     img, ref, m = ebs_pair(wl, 1024, 1024, stride=32, shadow_type=1)
     assert ref[..., 3].max() > 0.2
     assert_parity(img, ref, m, stride=32)
+    img0, _, _ = ebs_pair(wl, 1024, 1024, stride=1024, shadow_type=1, sat_layout=0)
+    assert np.abs(img0 - img).max() <= 1e-6
