@@ -38,6 +38,9 @@ WORKLOAD_METRIC = {
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_SAMPLE = 2 * (16 * 64 + 64 * 64 + 64 * 64 + 64 * 16)  # 20480
+# image RMSE (RGB over covered pixels) the batch's step search may add against the
+# default step: half of the 0.01 the surrogate is held to against the exact tracer
+STEP_RMSE_GATE = 0.005
 
 
 def log(*a):
@@ -312,9 +315,10 @@ def bench_batch(args, N, W, torch, stream):
     b.set_jobs(jobs)
     out = torch.empty((len(jobs), 512, 512, 4), dtype=torch.float32, device="cuda")
     t1 = time.perf_counter()
-    tune = b.autotune(out, stride=8, stream=stream.cuda_stream)
-    log(f"autotune {time.perf_counter() - t1:.1f}s: tile_w {tune.tile_w} shade_batch {tune.shade_batch} "
-        f"queues {tune.queues} wg/cu {tune.workgroups_per_cu} ({tune.ms:.2f} ms on every 8th job)")
+    tune = b.autotune(out, stride=8, stream=stream.cuda_stream, max_step_rmse=STEP_RMSE_GATE)
+    log(f"autotune {time.perf_counter() - t1:.1f}s: step_scale {tune.step_scale} (image RMSE {tune.step_rmse:.5f} "
+        f"against the default step, gate {STEP_RMSE_GATE}) tile_w {tune.tile_w} shade_batch {tune.shade_batch} "
+        f"queues {tune.queues} wg/cu {tune.workgroups_per_cu} span {tune.span} ({tune.ms:.2f} ms on every 8th job)")
     for _ in range(args.warmup):
         b.render(out, stream.cuda_stream)
     stream.synchronize()
@@ -338,7 +342,10 @@ def bench_batch(args, N, W, torch, stream):
         "config": {"workload": "batch_4vol_16view_4light_512", "images_per_step": len(jobs),
                    "volumes": [f"{wl.name} {wl.volume.shape[::-1]}" for wl in wls],
                    "tuning": {"tile_w": tune.tile_w, "shade_batch": round(tune.shade_batch, 3),
-                              "queues": tune.queues, "workgroups_per_cu": tune.workgroups_per_cu},
+                              "queues": tune.queues, "workgroups_per_cu": tune.workgroups_per_cu, "span": tune.span,
+                              "step_scale": round(tune.step_scale, 3),
+                              "step_rmse_vs_default_step": round(tune.step_rmse, 6),
+                              "step_rmse_gate": STEP_RMSE_GATE},
                    "samples_per_step": int(st.samples), "shaded_samples_per_step": int(st.shaded_samples),
                    "fallback_samples_per_step": int(st.fallback_samples)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -499,10 +499,14 @@ int cvr_set_params(cvr_handle h, const void* params, size_t bytes) {
   }
   if (h->technique == CVR_CONE_NN) {
     // training inputs changed: the network is retrained on the next prepare/render
-    // (the gate, the range margin and force_exact apply at render time only)
+    // (the march step and early-termination threshold, the gate, the range margin and
+    // force_exact apply at render time only)
     const cvr_nn_params* n = static_cast<const cvr_nn_params*>(params);
     const cvr_nn_params& o = h->nn;
-    if (std::memcmp(&n->cone, &o.cone, sizeof(cvr_cone_params)) != 0 || n->n_train != o.n_train ||
+    cvr_cone_params nc = n->cone, oc = o.cone;
+    nc.step_size = oc.step_size = 0.f;
+    nc.et_threshold = oc.et_threshold = 0.f;
+    if (std::memcmp(&nc, &oc, sizeof(cvr_cone_params)) != 0 || n->n_train != o.n_train ||
         n->n_val != o.n_val || n->epochs != o.epochs || n->batch != o.batch || n->lr != o.lr || n->seed != o.seed)
       h->nn_trained = false;
   }

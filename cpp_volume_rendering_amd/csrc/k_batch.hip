@@ -11,6 +11,7 @@
 // device code as the single-frame surrogate renderer.
 #include "cvr_nn.hpp"
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -104,6 +105,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
 }
 
+// Sum of squared RGB differences over the pixels either image covers (alpha > 0) and
+// the count of those values: the image error of a step schedule against the default one.
+__global__ __launch_bounds__(256) void k_rgb_sse(const float4* __restrict__ a, const float4* __restrict__ b, size_t n,
+                                                 double* __restrict__ acc) {
+  double s = 0.0, c = 0.0;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += size_t(gridDim.x) * 256) {
+    const float4 x = a[i], y = b[i];
+    if (x.w > 0.f || y.w > 0.f) {
+      const double dr = double(x.x) - double(y.x), dg = double(x.y) - double(y.y), db = double(x.z) - double(y.z);
+      s += dr * dr + dg * dg + db * db;
+      c += 3.0;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_down(s, o);
+    c += __shfl_down(c, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&acc[0], s);
+    atomicAdd(&acc[1], c);
+  }
+}
+
 }  // namespace cvr
 
 using namespace cvr;
@@ -130,7 +154,7 @@ struct cvr_batch {
   std::vector<cvr_handle> vols;
   std::vector<MemberSig> sigs;  // per member, as of the last table build
   std::vector<cvr_job> jobs;
-  cvr_batch_tuning tune{8, 0.5f, kQueues, 2, 1, 0.0};
+  cvr_batch_tuning tune{8, 0.5f, kQueues, 2, 1, 0.0, 1.0f, 0.0f, 0.0};
   // device side
   VolDesc* d_vols = nullptr;
   JobDesc* d_jobs = nullptr;
@@ -244,7 +268,7 @@ int build_tables(cvr_batch* b) {
     d.bpack = reinterpret_cast<const float*>(static_cast<const short*>(c->d_nn_pack) + kPackShorts);
     d.tf = c->d_tf_rgbt;
     d.ntf = c->tf_size;
-    d.step = N.cone.step_size > 0 ? N.cone.step_size : default_step(c);
+    d.step = (N.cone.step_size > 0 ? N.cone.step_size : default_step(c)) * b->tune.step_scale;
     d.et = N.cone.et_threshold;
     b->max_ntf = std::max(b->max_ntf, c->tf_size);
     cells[v] = size_t(c->nx) * c->ny * c->nz;
@@ -336,10 +360,61 @@ int build_graph(cvr_batch* b, float* out) {
 int check_tuning(const cvr_batch_tuning& t) {
   if ((t.tile_w != 4 && t.tile_w != 8 && t.tile_w != 16 && t.tile_w != 32) || !(t.shade_batch >= 0.f && t.shade_batch <= 1.f) ||
       (t.queues != 1 && t.queues != kQueues) || t.workgroups_per_cu < 1 || t.workgroups_per_cu > 4 ||
-      (t.span != 1 && t.span != 2 && t.span != 4 && t.span != 8)) {
+      (t.span != 1 && t.span != 2 && t.span != 4 && t.span != 8) || !(t.step_scale >= 0.25f && t.step_scale <= 4.f) ||
+      !(t.max_step_rmse >= 0.f && t.max_step_rmse <= 1.f)) {
     set_error("batch tuning: tile_w in {4,8,16,32}, shade_batch in [0,1], queues 1 or 8, workgroups_per_cu in [1,4], "
-              "span 1, 2, 4 or 8");
+              "span 1, 2, 4 or 8, step_scale in [0.25, 4], max_step_rmse in [0, 1]");
     return CVR_ERR_INVALID;
+  }
+  return CVR_OK;
+}
+
+// Step search of cvr_batch_autotune on the jobs currently set: renders them at
+// step_scale 1 as the reference, then at each larger step_scale in turn up to the first
+// whose images differ from the reference by more than max_step_rmse (the error grows
+// with the step), and keeps the fastest admitted step.
+template <class TimeOf>
+int tune_step(cvr_batch* b, float* out, hipStream_t st, TimeOf& time_of, cvr_batch_tuning& cur, double& cur_ms) {
+  const size_t px = b->jobs.size() * size_t(b->W) * b->H;
+  float4* ref = nullptr;
+  double* acc = nullptr;
+  if (hipMalloc(&ref, px * sizeof(float4)) != hipSuccess || hipMalloc(&acc, 2 * sizeof(double)) != hipSuccess) {
+    if (ref) (void)hipFree(ref);
+    set_error("cvr_batch_autotune: out of device memory for the step search");
+    return CVR_ERR_NOMEM;
+  }
+  struct Free {
+    void* p[2];
+    ~Free() {
+      for (void* q : p) (void)hipFree(q);
+    }
+  } fr{{ref, acc}};
+  const float scales[] = {1.f, 1.25f, 1.5f, 2.f, 2.5f, 3.f};
+  cvr_batch_tuning t = cur;
+  t.step_scale = scales[0];
+  t.step_rmse = 0.0;
+  int rc = time_of(t, &cur_ms);
+  if (rc) return rc;
+  cur = t;
+  CVR_HIP_CHECK(hipMemcpyAsync(ref, out, px * sizeof(float4), hipMemcpyDeviceToDevice, st));
+  const int grid = int(std::min<size_t>((px + 255) / 256, 4096));
+  for (int i = 1; i < int(sizeof(scales) / sizeof(scales[0])); ++i) {
+    t.step_scale = scales[i];
+    double ms = 0;
+    if ((rc = time_of(t, &ms))) return rc;
+    CVR_HIP_CHECK(hipMemsetAsync(acc, 0, 2 * sizeof(double), st));
+    hipLaunchKernelGGL(k_rgb_sse, dim3(grid), dim3(256), 0, st, reinterpret_cast<const float4*>(out), ref, px, acc);
+    CVR_HIP_CHECK(hipGetLastError());
+    double h[2] = {0.0, 0.0};
+    CVR_HIP_CHECK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, st));
+    CVR_HIP_CHECK(hipStreamSynchronize(st));
+    const double rmse = h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0;
+    if (rmse > double(cur.max_step_rmse)) break;
+    if (ms < cur_ms) {
+      cur = t;
+      cur.step_rmse = rmse;
+      cur_ms = ms;
+    }
   }
   return CVR_OK;
 }
@@ -428,7 +503,8 @@ int cvr_batch_set_tuning(cvr_batch_handle b, const cvr_batch_tuning* t) {
   if (!b || !t) { set_error("cvr_batch_set_tuning: null argument"); return CVR_ERR_INVALID; }
   int rc = check_tuning(*t);
   if (rc) return rc;
-  const bool items_change = t->tile_w != b->tune.tile_w || t->queues != b->tune.queues || t->span != b->tune.span;
+  const bool items_change = t->tile_w != b->tune.tile_w || t->queues != b->tune.queues || t->span != b->tune.span ||
+                            t->step_scale != b->tune.step_scale;
   b->tune = *t;
   if (items_change) b->tables_dirty = true;
   b->dirty = true;
@@ -467,6 +543,7 @@ int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_str
   if (b->jobs.empty()) { set_error("cvr_batch_autotune: no jobs"); return CVR_ERR_STATE; }
   Guard gd(b->device);
   hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  const cvr_batch_tuning start = b->tune;
   const std::vector<cvr_job> all = b->jobs;
   std::vector<cvr_job> sub;
   for (size_t i = 0; i < all.size(); i += size_t(stride)) sub.push_back(all[i]);
@@ -488,10 +565,15 @@ int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_str
     *ms = f;
     return CVR_OK;
   };
-  // coordinate descent over tile width, shading batch, queues, workgroups per CU
+  // the step schedule first (its images against the step_scale 1 images of the same
+  // jobs), then coordinate descent over tile width, shading batch, queues, workgroups
+  // per CU and span at the chosen step
   cvr_batch_tuning cur = b->tune;
+  cur.step_rmse = 0.0;
   double cur_ms = 0;
-  int rc = time_of(cur, &cur_ms);
+  int rc = CVR_OK;
+  if (cur.max_step_rmse > 0.f) rc = tune_step(b, out, st, time_of, cur, cur_ms);
+  else rc = time_of(cur, &cur_ms);
   const int tws[] = {4, 8, 16, 32};
   const float sbs[] = {0.f, 0.25f, 0.5f, 0.75f, 1.f};
   const int qs[] = {1, kQueues};
@@ -521,7 +603,10 @@ int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_str
   (void)hipEventDestroy(e1);
   b->jobs = all;
   b->tables_dirty = b->dirty = true;
-  if (rc) return rc;
+  if (rc) {
+    (void)cvr_batch_set_tuning(b, &start);
+    return rc;
+  }
   cur.ms = cur_ms;
   b->tune = cur;
   if (best) *best = cur;

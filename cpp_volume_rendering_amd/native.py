@@ -109,7 +109,8 @@ class Job(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_w", C.c_int), ("shade_batch", C.c_float), ("queues", C.c_int),
-                ("workgroups_per_cu", C.c_int), ("span", C.c_int), ("ms", C.c_double)]
+                ("workgroups_per_cu", C.c_int), ("span", C.c_int), ("ms", C.c_double), ("step_scale", C.c_float),
+                ("max_step_rmse", C.c_float), ("step_rmse", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -538,7 +539,10 @@ class Batch:
 
     def set_tuning(self, **kw):
         t = self.tuning()
+        names = {f[0] for f in BatchTuning._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise ValueError(f"unknown batch tuning field {k!r}")
             setattr(t, k, v)
         check(lib().cvr_batch_set_tuning(self.h, C.byref(t)))
         return t
@@ -551,7 +555,11 @@ class Batch:
             raise CvrError(ERR_INVALID, "batch output must be a GPU tensor")
         return ptr
 
-    def autotune(self, out, stride=1, stream=None) -> BatchTuning:
+    def autotune(self, out, stride=1, stream=None, max_step_rmse=None) -> BatchTuning:
+        """Tunes the work shape (and, with max_step_rmse > 0, the step schedule first) on
+        every stride-th job; see cvr_batch_autotune."""
+        if max_step_rmse is not None:
+            self.set_tuning(max_step_rmse=max_step_rmse)
         t = BatchTuning()
         check(lib().cvr_batch_autotune(self.h, self._dev_out(out), stride, stream, C.byref(t)))
         return t

@@ -459,7 +459,9 @@ typedef struct cvr_job {
   cvr_light light;
 } cvr_job;
 
-/* Work shape of the persistent launch (pixels do not depend on it). */
+/* Work shape of the persistent launch and step schedule.  The pixels do not depend on
+ * the work shape (tile_w .. span); step_scale changes the march step and with it the
+ * pixels, within the image error the autotuner accepts for it. */
 typedef struct cvr_batch_tuning {
   int tile_w;            /* wavefront tile width: 4, 8, 16 or 32 pixels (height 64 / width) */
   float shade_batch;     /* fraction of marching lanes that wait before a shading pass, [0, 1] */
@@ -467,6 +469,13 @@ typedef struct cvr_batch_tuning {
   int workgroups_per_cu; /* persistent workgroups per compute unit, 1..4 */
   int span;              /* wave tiles per wavefront, handed to lanes as their rays end: 1, 2, 4 or 8 */
   double ms;             /* autotune: the measured time of the chosen shape */
+  float step_scale;      /* multiplier of every member's march step (its cone.step_size, or the
+                            default half-voxel step), [0.25, 4]; 1 = each member's own step.  The
+                            reference sweeps this step by hand (rc1prenderer.cpp:225-229). */
+  float max_step_rmse;   /* autotune input: the largest image RMSE (RGB of the pixels either image
+                            covers) a step_scale above 1 may add against step_scale 1 on the tuned
+                            jobs; 0 (default) leaves step_scale unchanged */
+  double step_rmse;      /* autotune output: that RMSE for the chosen step_scale (0 when not tuned) */
 } cvr_batch_tuning;
 
 int cvr_batch_create(int device, int width, int height, cvr_batch_handle* out);
@@ -476,7 +485,11 @@ int cvr_batch_set_jobs(cvr_batch_handle b, int n, const cvr_job* jobs);
 int cvr_batch_set_tuning(cvr_batch_handle b, const cvr_batch_tuning* t);
 int cvr_batch_get_tuning(cvr_batch_handle b, cvr_batch_tuning* t);
 /* Times candidate shapes on every `stride`-th job (stride >= 1) rendered into `out`,
- * keeps the fastest and reports it. */
+ * keeps the fastest and reports it.  With max_step_rmse > 0 in the current tuning it
+ * first searches the step schedule: step_scale 1, 1.25, 1.5, 2, 2.5, 3 in turn, each
+ * rendered and compared on the device with the step_scale 1 images, up to the first that
+ * adds more than max_step_rmse; the fastest admitted step is kept, then the shape is
+ * tuned at that step.  Allocates one temporary image buffer for the tuned jobs. */
 int cvr_batch_autotune(cvr_batch_handle b, float* out, int stride, void* hip_stream, cvr_batch_tuning* best);
 int cvr_batch_render(cvr_batch_handle b, float* out, void* hip_stream);
 /* Counters of the last batch render (waits for the device). */

@@ -119,6 +119,76 @@ def test_autotune_then_render(scene):
     assert np.array_equal(got, batch_reference(b, len(jobs)))
 
 
+def default_step(scale=(1.0, 1.0, 1.0)):
+    """The default march step as the library forms it (rc1prenderer.cpp:62-63):
+    (0.5f / sqrtf(3)) * |voxel scale|, evaluated in double and stored as float."""
+    sx, sy, sz = scale
+    return np.float32(float(np.float32(0.5) / np.sqrt(np.float32(3.0))) * np.sqrt(sx * sx + sy * sy + sz * sz))
+
+
+def frames_at_step(rs, jobs, scale):
+    """Single-frame renders with every member's step multiplied by `scale` in float32
+    (what a batch at step_scale `scale` computes; the scene's volumes have unit voxels);
+    the members' steps are restored."""
+    saved = [r.params().cone.step_size for r in rs]
+    try:
+        for r in rs:
+            p = r.params()
+            p.cone.step_size = float(default_step() * np.float32(scale))
+            r.set_params(p)
+        return np.stack([rs[v].render(cam, WD, HT, light=l) for v, cam, l in jobs])
+    finally:
+        for r, st in zip(rs, saved):
+            p = r.params()
+            p.cone.step_size = st
+            r.set_params(p)
+
+
+def test_step_scale_equals_single_frames_at_the_scaled_step(scene):
+    """step_scale multiplies every member's march step: the batch's images equal
+    single-frame renders at the multiplied step, and changing the step does not retrain
+    the members' networks."""
+    rs, jobs, ref = scene
+    before = [r.nn_report().steps for r in rs]
+    b = make_batch(rs, jobs)
+    b.set_tuning(step_scale=2.0)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    b.render(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = frames_at_step(rs, jobs, 2.0)
+    err = float(np.abs(got - want).max())
+    print("step_scale 2: max-abs vs single frames at twice the step", err)
+    assert err <= EAGER_TOL
+    assert float(np.abs(got - ref).max()) > 10 * EAGER_TOL  # the step did change the pixels
+    assert [r.nn_report().steps for r in rs] == before
+
+
+def test_autotune_searches_the_step_schedule(scene):
+    """With max_step_rmse set the tuner searches the step first: the step_scale it keeps
+    adds at most that image RMSE (RGB over covered pixels) against step_scale 1, as the
+    tuner reports it and as measured here over every job, and a render afterwards gives
+    the single-frame images at that step."""
+    from surrogate_oracle import image_rmse
+    rs, jobs, ref = scene
+    b = make_batch(rs, jobs)
+    out = torch.zeros((len(jobs), HT, WD, 4), device="cuda")
+    gate = 0.01
+    t = b.autotune(out, stride=1, max_step_rmse=gate)
+    print("tuned step_scale", t.step_scale, "rmse", t.step_rmse, "ms", t.ms)
+    assert t.step_scale in (1.0, 1.25, 1.5, 2.0, 2.5, 3.0) and 0.0 <= t.step_rmse <= gate
+    assert b.tuning().step_scale == t.step_scale
+    out.zero_()
+    b.render(out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    _, rgb, ncov = image_rmse(got, ref)
+    assert ncov > 0 and rgb <= gate + 1e-4
+    assert abs(rgb - t.step_rmse) <= 1e-4
+    want = frames_at_step(rs, jobs, t.step_scale)
+    assert float(np.abs(got - want).max()) <= EAGER_TOL
+
+
 def test_batch_errors(scene):
     rs, jobs, _ = scene
     b = N.Batch(WD, HT)
@@ -130,6 +200,12 @@ def test_batch_errors(scene):
         b.set_jobs([(1, jobs[0][1], jobs[0][2])])
     with pytest.raises(N.CvrError):
         b.set_tuning(tile_w=12)
+    with pytest.raises(N.CvrError):
+        b.set_tuning(step_scale=8.0)
+    with pytest.raises(N.CvrError):
+        b.set_tuning(max_step_rmse=-0.5)
+    with pytest.raises(ValueError):
+        b.set_tuning(step=2.0)
     out = torch.zeros((1, HT, WD, 4), device="cuda")
     with pytest.raises(N.CvrError):
         b.render(out)  # no jobs yet
