@@ -10,8 +10,13 @@ timed the same way beside it and judges the surrogate's image RMSE.  Other
 workloads: rc1p (128^3 plain caster at 512^2), ebs (256^3 extinction-based shading
 at 1024^2), cone (256^3 exact cone tracer at 1024^2), dos (256^3 section-table
 directional occlusion + point-light shadow at 1024^2), batch (4 uneven volumes x 16
-views x 4 lights, 512^2, one graph-captured persistent launch).  Prints one JSON line.
-Only one GPU is used; --gpus above 1 is refused.
+views x 4 lights, 512^2, one graph-captured persistent launch, work shape and step
+schedule autotuned), sat (the EBS config's summed-area-table build alone), ebs-lc and
+dos-lc (the 256^3 configs through a 32^3 object-space light cache).  Prints one JSON
+line; the headline exits with status 3 (after printing) when the surrogate's image
+RMSE against the exact tracer exceeds 0.01.  Each gather-bound line is priced at the
+gather peak of the cache level its read footprint fits, measured by a microbenchmark
+first.  Only one GPU is used; --gpus above 1 is refused.
 """
 from __future__ import annotations
 
