@@ -113,9 +113,10 @@ __device__ __forceinline__ void lin_axis(float x, int n, int& i0, int& i1, float
 // texel centre both taps clamp to cell 0, so the weight is zeroed there.
 __device__ __forceinline__ void quad_axis(float x, int n, int& i, float& f) {
   const float xc = fmaxf(x, 0.f);  // below the first texel centre: cell 0 with weight 0
-  const float fl = floorf(xc);
-  f = xc - fl;
-  i = min(int(fl), n - 1);
+  // xc >= 0: its fractional part is exact (v_fract_f32 = xc - floor(xc)) and truncation
+  // is the floor
+  f = __builtin_amdgcn_fractf(xc);
+  i = min(int(xc), n - 1);
 }
 
 // Trilinear from two quad loads (planes z0 and z1): c = (v00, v10, v01, v11) over (x, y).

@@ -84,6 +84,32 @@ __device__ __forceinline__ Oct8 pyr_decode(typename PyrOct<F>::type r) {
   }
 }
 
+// Trilinear value of one corner block at weights (fx, fy, fz): quad_lerp of the decoded
+// record.  For fp16 records the four x lerps take the two halves of each dword (the x
+// pair at one y and z) straight from the record with mixed-precision FMAs
+// (v_fma_mix_f32: f16 operands, fp32 arithmetic): d = hi * 1 - lo is the fp32
+// difference rounded once, and lo + fx * d the fused lerp, the same values and roundings
+// as converting the eight texels first, in 8 instructions instead of 16.  `one` is 1.0f
+// from a kernel argument (PyrGeom::one), opaque to the compiler, so that hi * 1 - lo
+// stays one fused instruction.
+typedef _Float16 half2v_t __attribute__((ext_vector_type(2)));
+template <int F>
+__device__ __forceinline__ float pyr_lerp(typename PyrOct<F>::type r, float fx, float fy, float fz, float one) {
+  if constexpr (F == 1) {
+    auto xl = [&](unsigned int w) {
+      const half2v_t h = __builtin_bit_cast(half2v_t, w);
+      const float lo = float(h[0]), hi = float(h[1]);
+      return __builtin_fmaf(fx, __builtin_fmaf(hi, one, -lo), lo);
+    };
+    const float a0 = xl(r.x), a1 = xl(r.y), b0 = xl(r.z), b1 = xl(r.w);
+    const float c0 = a0 + fy * (a1 - a0), c1 = b0 + fy * (b1 - b0);
+    return c0 + fz * (c1 - c0);
+  } else {
+    const Oct8 o = pyr_decode<F>(r);
+    return quad_lerp(o.a, o.b, fx, fy, fz);
+  }
+}
+
 // GL_LINEAR lookup of pyramid level `l` at texture-space point tp: the texel coordinate
 // tp * dims / G - 1/2 is one fma per axis, the read one corner block.
 template <int F>
@@ -94,8 +120,8 @@ __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const Pyr
   quad_axis(fmaf(tp.y, pg.ts[l][1], -0.5f), pg.dims[l][1], y0, fy);
   quad_axis(fmaf(tp.z, pg.ts[l][2], -0.5f), pg.dims[l][2], z0, fz);
   typedef typename PyrOct<F>::type B;
-  const Oct8 o = pyr_decode<F>(static_cast<const B*>(pyr)[pg.boff[l] + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])]);
-  return quad_lerp(o.a, o.b, fx, fy, fz);
+  return pyr_lerp<F>(static_cast<const B*>(pyr)[pg.boff[l] + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])], fx, fy,
+                     fz, pg.one);
 }
 
 // Border falloff of GetGaussianExtinction (rc1pdosct/ray_bbox_marching.comp:96-109):
@@ -214,8 +240,7 @@ __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const Pyr
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < kFeatGroup; ++j) {
-      const Oct8 o = pyr_decode<F>(b[j]);
-      f[g0 + j] = 1.f - __expf(-(quad_lerp(o.a, o.b, fx[j], fy[j], fz[j]) * w[j]));
+      f[g0 + j] = 1.f - __expf(-(pyr_lerp<F>(b[j], fx[j], fy[j], fz[j], pg.one) * w[j]));
     }
   }
 }

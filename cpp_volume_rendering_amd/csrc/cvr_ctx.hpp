@@ -17,6 +17,7 @@ struct PyrGeom {
   int nb[kMaxLevels][2];    // bricks along x and y per level
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
+  float one;  // 1.0f, read as a kernel argument: a multiplier the compiler cannot fold
 };
 // One box of the EBS directional-light cone chain (offsets from the sample, the
 // CUT_WHEN_AWAY_FROM_VOLUME coordinate apos + interval, 1 / box volume in voxels).

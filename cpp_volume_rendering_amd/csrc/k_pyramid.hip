@@ -255,6 +255,7 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
   }
   VolGeom g = geom_of(c);
   for (int k = 0; k < 3; ++k) pg.G[k] = g.G[k];
+  pg.one = 1.0f;
   if (c->d_pyr) CVR_HIP_CHECK(hipFree(c->d_pyr));
   c->d_pyr = nullptr;
   float* d_op = nullptr;
