@@ -23,8 +23,9 @@ constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7, kNStencil = 2 * kNOcc + kNSdw, 
 
 // One surrogate input's trilinear read with its pyramid level resolved on the host:
 // the stencil point (distance along its axis, segment length) and the geometry of the
-// level it reads (texel coordinate per world unit, size, bricks, block offset, border
-// falloff 1 / (2 (2^l)^2)).  64 bytes; the table is read through scalar loads.
+// level it reads (texel coordinate per world unit, size, bricks, byte offset of the
+// level's corner blocks, border falloff 1 / (2 (2^l)^2)).  64 bytes; the table is read
+// through scalar loads.
 struct FeatTap {
   float dist, seg;
   float ts[3];
@@ -210,10 +211,24 @@ typedef const __attribute__((address_space(4))) FeatTap* ConstTaps;
 template <int F>
 __host__ __device__ constexpr int feat_group() { return CVR_FEAT_GROUP ? CVR_FEAT_GROUP : (F == 0 ? 8 : 4); }
 
-template <int F>
-__device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const FeatTap* __restrict__ taps,
-                                         f3 p, f3 k, f3 L, float* f) {
+// Byte offset of corner block (x0, y0, z0) in a level of `nbx` x `nby` bricks, records of
+// 2^kShift bytes: brick number, then the 6 bits of in-brick position (brick_index).  The
+// 32-bit form needs the level's records to span less than 4 GiB (PyrGeom::wide = 0); its
+// products are 24-bit (v_mad_u32_u24).
+template <int kShift>
+__device__ __forceinline__ uint32_t oct_offset32(int x0, int y0, int z0, int nbx, int nby) {
+  const uint32_t b = __umul24(__umul24(uint32_t(z0) >> 2, uint32_t(nby)) + (uint32_t(y0) >> 2), uint32_t(nbx)) +
+                     (uint32_t(x0) >> 2);
+  const uint32_t local = ((uint32_t(z0) & 3u) << 4) | ((uint32_t(y0) & 3u) << 2) | (uint32_t(x0) & 3u);
+  return ((b << 6) | local) << kShift;
+}
+
+template <int F, bool kWide>
+__device__ __forceinline__ void nn_feats_at(const void* __restrict__ pyr, const PyrGeom& pg,
+                                            const FeatTap* __restrict__ taps, f3 p, f3 k, f3 L, float* f) {
   typedef typename PyrOct<F>::type B;
+  constexpr int kShift = sizeof(B) == 32 ? 5 : 4;
+  static_assert(sizeof(B) == (1u << kShift), "records of 16 or 32 bytes");
   constexpr int kFeatGroup = feat_group<F>();
   static_assert(kNFeat % kFeatGroup == 0, "the load groups must tile the inputs");
   // the table is wave-uniform and read-only: constant address space -> scalar loads
@@ -231,8 +246,12 @@ __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const Pyr
       quad_axis(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dims[0], x0, fx[j]);
       quad_axis(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dims[1], y0, fy[j]);
       quad_axis(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dims[2], z0, fz[j]);
-      const B* lv = static_cast<const B*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
-      b[j] = lv[brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1])];
+      // the level's first byte (a wave-uniform address), then the record's byte offset
+      const char* lv = static_cast<const char*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
+      if constexpr (kWide)
+        b[j] = *reinterpret_cast<const B*>(lv + (brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]) << kShift));
+      else
+        b[j] = *reinterpret_cast<const B*>(lv + oct_offset32<kShift>(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]));
       w[j] = border_falloff(pg, tp, ft[i].inv2sg2);
       if (i != kNFeat - 1) w[j] *= ft[i].seg;
     }
@@ -243,6 +262,14 @@ __device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const Pyr
       f[g0 + j] = 1.f - __expf(-(pyr_lerp<F>(b[j], fx[j], fy[j], fz[j], pg.one) * w[j]));
     }
   }
+}
+
+// Dispatch on the level sizes: 32-bit record offsets unless a level spans 4 GiB or more.
+template <int F>
+__device__ __forceinline__ void nn_feats(const void* __restrict__ pyr, const PyrGeom& pg, const FeatTap* __restrict__ taps,
+                                         f3 p, f3 k, f3 L, float* f) {
+  if (pg.wide) nn_feats_at<F, true>(pyr, pg, taps, p, k, L, f);
+  else nn_feats_at<F, false>(pyr, pg, taps, p, k, L, f);
 }
 
 // Network-free estimate from the same inputs (the network predicts the residual):

@@ -18,6 +18,7 @@ struct PyrGeom {
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
   float G[3];
   float one;  // 1.0f, read as a kernel argument: a multiplier the compiler cannot fold
+  int wide;   // some level's corner blocks span 4 GiB or more: 64-bit record offsets
 };
 // One box of the EBS directional-light cone chain (offsets from the sample, the
 // CUT_WHEN_AWAY_FROM_VOLUME coordinate apos + interval, 1 / box volume in voxels).

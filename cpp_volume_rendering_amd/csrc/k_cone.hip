@@ -155,8 +155,9 @@ static FeatTap feat_tap(const PyrGeom& pg, const StencilF& s) {
   std::memcpy(&t.inv2sg2, &bits, 4);
   t.nb[0] = pg.nb[l][0];
   t.nb[1] = pg.nb[l][1];
-  t.boff_lo = unsigned(pg.boff[l] & 0xFFFFFFFFu);
-  t.boff_hi = unsigned(pg.boff[l] >> 32);
+  const size_t bytes = pg.boff[l] * (pg.fmt == 0 ? 32u : 16u);  // records of 32 (fp32) or 16 bytes
+  t.boff_lo = unsigned(bytes & 0xFFFFFFFFu);
+  t.boff_hi = unsigned(bytes >> 32);
   t.level = l;
   return t;
 }

@@ -251,6 +251,7 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
     pg.nb[i][0] = bricks(pg.dims[i][0]);
     pg.nb[i][1] = bricks(pg.dims[i][1]);
     btotal += brick_cells(pg.dims[i][0], pg.dims[i][1], pg.dims[i][2]);
+    if (brick_cells(pg.dims[i][0], pg.dims[i][1], pg.dims[i][2]) * (fmt == 0 ? 32u : 16u) > 0xFFFFFFFFull) pg.wide = 1;
     total += size_t(pg.dims[i][0]) * pg.dims[i][1] * pg.dims[i][2];
   }
   VolGeom g = geom_of(c);

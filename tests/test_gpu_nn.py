@@ -288,6 +288,28 @@ def test_cfg4_surrogate_frame_matches_fp64_oracle(cfg4):
     assert_frame_parity(img, ref, m, 2e-3, 32, 0.005)
 
 
+def test_cfg4_fp32_pyramid_wide_offsets_match_fp64_oracle(cfg4):
+    """fp32 pyramid storage at 512^3: level 0's corner blocks span 2^27 cells x 32 bytes
+    = 4 GiB, so the feature reads take the 64-bit record offsets (the fp16 default takes
+    the 32-bit ones); the frame against the FP64 surrogate frame on every 64th pixel."""
+    wl, _ = cfg4
+    assert 512 ** 3 * 32 > 0xFFFFFFFF
+    r = N.Renderer(N.CONE_NN)
+    r.set_volume(wl.volume)
+    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+    p = r.params()
+    p.cone.density_precision = 0
+    p.n_train, p.epochs = 1 << 20, 2
+    p.rmse_gate = 1.0  # the network renders whatever its validation error
+    r.set_params(p)
+    r.prepare()
+    cam, light = wl.camera(), wl.light()
+    img = r.render(cam, 1024, 1024, light=light)
+    ref, m, _ = SO.nn_oracle_frame(wl, r, cam, light, 1024, 1024, stride=64)
+    assert ref[..., 3].max() > 0.2
+    assert_frame_parity(img, ref, m, 2e-3, 64, 0.005)
+
+
 def test_cfg4_image_error_against_exact_tracer(cfg4):
     """The acceptance figure of the surrogate: image RMSE <= 0.01 against the exact cone
     tracer at 512^3 and 1024^2, over the whole frame and over RGB of covered pixels; and
