@@ -26,6 +26,7 @@ VolGeom geom_of(const cvr_ctx* c) {
   g.oct = c->d_vol_oct;
   g.nbx = bricks(c->nx);
   g.nby = bricks(c->ny);
+  for (int i = 0; i < 3; ++i) g.vmax[i] = float(n[i] - 1);
   return g;
 }
 

@@ -55,6 +55,7 @@ struct VolGeom {
   const void* vox;   // raw voxels, x fastest
   const void* oct;   // per voxel its 2 x 2 x 2 corner block, edge-clamped (8 values), brick order
   int nbx, nby;      // bricks along x and y
+  float vmax[3];     // n - 1 per axis: the last texel index as a float
 };
 
 // --------------------------------------------------------------------------- device helpers
@@ -119,6 +120,16 @@ __device__ __forceinline__ void quad_axis(float x, int n, int& i, float& f) {
   i = min(int(xc), n - 1);
 }
 
+// quad_axis with the last texel index nmax = n - 1 as a float: one median-of-three clamps
+// x to [0, n - 1].  Past the last texel centre the block's +1 neighbour along the axis is
+// the edge texel itself, so a zero weight there gives the same value as quad_axis's
+// fractional one (a + f * 0 = a), and the cell is the same.
+__device__ __forceinline__ void quad_axis_f(float x, float nmax, int& i, float& f) {
+  const float xc = __builtin_amdgcn_fmed3f(x, 0.f, nmax);
+  f = __builtin_amdgcn_fractf(xc);
+  i = int(xc);
+}
+
 // Trilinear from two quad loads (planes z0 and z1): c = (v00, v10, v01, v11) over (x, y).
 __device__ __forceinline__ float quad_lerp(float4 a, float4 b, float fx, float fy, float fz) {
   const float a0 = a.x + fx * (a.y - a.x), a1 = a.z + fx * (a.w - a.z);
@@ -154,9 +165,9 @@ __device__ __forceinline__ float sample_volume_oct(const VolGeom& g, f3 p) {
   const RT* __restrict__ q = static_cast<const RT*>(g.oct);
   int x0, y0, z0;
   float fx, fy, fz;
-  quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
-  quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
-  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  quad_axis_f(p.x * g.tex_mul[0] - 0.5f, g.vmax[0], x0, fx);
+  quad_axis_f(p.y * g.tex_mul[1] - 0.5f, g.vmax[1], y0, fy);
+  quad_axis_f(p.z * g.tex_mul[2] - 0.5f, g.vmax[2], z0, fz);
   float4 a, b;
   VolOct<T>::f(q[brick_index(x0, y0, z0, g.nbx, g.nby)], a, b);
   return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));
@@ -169,18 +180,18 @@ __device__ __forceinline__ typename VolOct<T>::type volume_oct_load(const VolGeo
   typedef typename VolOct<T>::type RT;
   int x0, y0, z0;
   float fx, fy, fz;
-  quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
-  quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
-  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  quad_axis_f(p.x * g.tex_mul[0] - 0.5f, g.vmax[0], x0, fx);
+  quad_axis_f(p.y * g.tex_mul[1] - 0.5f, g.vmax[1], y0, fy);
+  quad_axis_f(p.z * g.tex_mul[2] - 0.5f, g.vmax[2], z0, fz);
   return static_cast<const RT*>(g.oct)[brick_index(x0, y0, z0, g.nbx, g.nby)];
 }
 template <typename T>
 __device__ __forceinline__ float volume_oct_lerp(const VolGeom& g, typename VolOct<T>::type r, f3 p) {
   int x0, y0, z0;
   float fx, fy, fz;
-  quad_axis(p.x * g.tex_mul[0] - 0.5f, g.nx, x0, fx);
-  quad_axis(p.y * g.tex_mul[1] - 0.5f, g.ny, y0, fy);
-  quad_axis(p.z * g.tex_mul[2] - 0.5f, g.nz, z0, fz);
+  quad_axis_f(p.x * g.tex_mul[0] - 0.5f, g.vmax[0], x0, fx);
+  quad_axis_f(p.y * g.tex_mul[1] - 0.5f, g.vmax[1], y0, fy);
+  quad_axis_f(p.z * g.tex_mul[2] - 0.5f, g.vmax[2], z0, fz);
   float4 a, b;
   VolOct<T>::f(r, a, b);
   return quad_lerp(a, b, fx, fy, fz) * (sizeof(T) == 1 ? (1.0f / 255.0f) : (1.0f / 65535.0f));

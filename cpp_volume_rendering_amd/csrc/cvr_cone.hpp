@@ -22,15 +22,17 @@ struct StencilF {
 constexpr int kNFeat = 16, kNOcc = 4, kNSdw = 7, kNStencil = 2 * kNOcc + kNSdw, kMaxSteps = 256;
 
 // One surrogate input's trilinear read with its pyramid level resolved on the host:
-// the stencil point (distance along its axis, segment length) and the geometry of the
-// level it reads (texel coordinate per world unit, size, bricks, byte offset of the
-// level's corner blocks, border falloff 1 / (2 (2^l)^2)).  64 bytes; the table is read
+// the stencil point (distance along its axis; segment length x log2(e), 1 x log2(e) for
+// the local opacity) and the geometry of the level it reads (texel coordinate per world
+// unit, last texel index per axis as a float, bricks, byte offset of the level's corner
+// blocks, border falloff log2(e) / (2 (2^l)^2)), with log2(e) folded in so that both
+// exponentials of a read are single exp2 instructions.  64 bytes; the table is read
 // through scalar loads.
 struct FeatTap {
   float dist, seg;
   float ts[3];
-  float inv2sg2;
-  int dims[3];
+  float fall_k;
+  float dmax[3];
   int nb[2];
   unsigned int boff_lo, boff_hi;
   int level, pad[2];
@@ -117,9 +119,9 @@ template <int F>
 __device__ __forceinline__ float pyr_tri(const void* __restrict__ pyr, const PyrGeom& pg, int l, f3 tp) {
   int x0, y0, z0;
   float fx, fy, fz;
-  quad_axis(fmaf(tp.x, pg.ts[l][0], -0.5f), pg.dims[l][0], x0, fx);
-  quad_axis(fmaf(tp.y, pg.ts[l][1], -0.5f), pg.dims[l][1], y0, fy);
-  quad_axis(fmaf(tp.z, pg.ts[l][2], -0.5f), pg.dims[l][2], z0, fz);
+  quad_axis_f(fmaf(tp.x, pg.ts[l][0], -0.5f), pg.dmax[l][0], x0, fx);
+  quad_axis_f(fmaf(tp.y, pg.ts[l][1], -0.5f), pg.dmax[l][1], y0, fy);
+  quad_axis_f(fmaf(tp.z, pg.ts[l][2], -0.5f), pg.dmax[l][2], z0, fz);
   typedef typename PyrOct<F>::type B;
   return pyr_lerp<F>(static_cast<const B*>(pyr)[pg.boff[l] + brick_index(x0, y0, z0, pg.nb[l][0], pg.nb[l][1])], fx, fy,
                      fz, pg.one);
@@ -243,23 +245,26 @@ __device__ __forceinline__ void nn_feats_at(const void* __restrict__ pyr, const 
       const f3 a = i < 2 * kNOcc ? k : L;
       const f3 tp = i == kNFeat - 1 ? p : p + a * ft[i].dist;
       int x0, y0, z0;
-      quad_axis(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dims[0], x0, fx[j]);
-      quad_axis(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dims[1], y0, fy[j]);
-      quad_axis(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dims[2], z0, fz[j]);
+      quad_axis_f(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dmax[0], x0, fx[j]);
+      quad_axis_f(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dmax[1], y0, fy[j]);
+      quad_axis_f(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dmax[2], z0, fz[j]);
       // the level's first byte (a wave-uniform address), then the record's byte offset
       const char* lv = static_cast<const char*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
       if constexpr (kWide)
         b[j] = *reinterpret_cast<const B*>(lv + (brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]) << kShift));
       else
         b[j] = *reinterpret_cast<const B*>(lv + oct_offset32<kShift>(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]));
-      w[j] = border_falloff(pg, tp, ft[i].inv2sg2);
-      if (i != kNFeat - 1) w[j] *= ft[i].seg;
+      // border falloff x segment length, both scaled by log2(e) on the host
+      const float cx = __builtin_amdgcn_fmed3f(tp.x, 0.f, pg.G[0]) - tp.x;
+      const float cy = __builtin_amdgcn_fmed3f(tp.y, 0.f, pg.G[1]) - tp.y;
+      const float cz = __builtin_amdgcn_fmed3f(tp.z, 0.f, pg.G[2]) - tp.z;
+      w[j] = __builtin_amdgcn_exp2f(-(cx * cx + cy * cy + cz * cz) * ft[i].fall_k) * ft[i].seg;
     }
     // keep the machine scheduler from pulling the lerps up between the loads
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int j = 0; j < kFeatGroup; ++j) {
-      f[g0 + j] = 1.f - __expf(-(pyr_lerp<F>(b[j], fx[j], fy[j], fz[j], pg.one) * w[j]));
+      f[g0 + j] = 1.f - __builtin_amdgcn_exp2f(-(pyr_lerp<F>(b[j], fx[j], fy[j], fz[j], pg.one) * w[j]));
     }
   }
 }

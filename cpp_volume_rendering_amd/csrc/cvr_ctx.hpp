@@ -16,6 +16,7 @@ struct PyrGeom {
   size_t boff[kMaxLevels];  // level offsets (in records) in the brick-ordered corner blocks
   int nb[kMaxLevels][2];    // bricks along x and y per level
   float ts[kMaxLevels][3];  // texel coordinate per world unit (dims / G) per level
+  float dmax[kMaxLevels][3];  // dims - 1 per level: the last texel index as a float
   float G[3];
   float one;  // 1.0f, read as a kernel argument: a multiplier the compiler cannot fold
   int wide;   // some level's corner blocks span 4 GiB or more: 64-bit record offsets

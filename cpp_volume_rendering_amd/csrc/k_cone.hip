@@ -145,14 +145,14 @@ static void stencil(const cvr_cone_params& P, double occ_max, double sdw_max, in
 static FeatTap feat_tap(const PyrGeom& pg, const StencilF& s) {
   FeatTap t{};
   const int l = int(s.lod);
+  const double log2e = 1.4426950408889634;
   t.dist = s.dist;
-  t.seg = s.seg;
+  t.seg = float(double(s.seg) * log2e);
   for (int a = 0; a < 3; ++a) {
     t.ts[a] = pg.ts[l][a];
-    t.dims[a] = pg.dims[l][a];
+    t.dmax[a] = pg.dmax[l][a];
   }
-  uint32_t bits = (126u - 2u * unsigned(l)) << 23;  // 1 / (2 (2^l)^2), exactly
-  std::memcpy(&t.inv2sg2, &bits, 4);
+  t.fall_k = float(log2e / (2.0 * std::ldexp(1.0, 2 * l)));  // log2(e) / (2 (2^l)^2)
   t.nb[0] = pg.nb[l][0];
   t.nb[1] = pg.nb[l][1];
   const size_t bytes = pg.boff[l] * (pg.fmt == 0 ? 32u : 16u);  // records of 32 (fp32) or 16 bytes

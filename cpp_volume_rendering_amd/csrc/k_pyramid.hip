@@ -247,7 +247,10 @@ int build_pyramid(cvr_ctx* c, int base_x, int base_y, int base_z, float sigma0, 
     pg.dims[i][2] = std::max(1, base_z >> i);
     pg.off[i] = total;
     pg.boff[i] = btotal;
-    for (int k = 0; k < 3; ++k) pg.ts[i][k] = float(double(pg.dims[i][k]) / double(geom_of(c).G[k]));
+    for (int k = 0; k < 3; ++k) {
+      pg.ts[i][k] = float(double(pg.dims[i][k]) / double(geom_of(c).G[k]));
+      pg.dmax[i][k] = float(pg.dims[i][k] - 1);
+    }
     pg.nb[i][0] = bricks(pg.dims[i][0]);
     pg.nb[i][1] = bricks(pg.dims[i][1]);
     btotal += brick_cells(pg.dims[i][0], pg.dims[i][1], pg.dims[i][2]);
