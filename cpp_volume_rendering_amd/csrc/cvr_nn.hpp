@@ -197,6 +197,16 @@ __device__ __forceinline__ bool nn_tile(const RayFrame& rf, const VolGeom& g, co
   float D = 0.f, h = 0.f;
   f3 dir = mk3(0.f, 0.f, 1.f), tp = eye, p = eye;  // the occlusion axis is -dir
   float4 dst = make_float4(0.f, 0.f, 0.f, 0.f), src = dst;
+  // The corner block of the lane's next sample, loaded as soon as the current sample is
+  // known, so that its read is in flight while the wave gathers and shades (sample j sits
+  // at the midpoint of [s, s + h], s = j step, h = min(step, D - s); past the end of the
+  // ray the position is still finite and its clamped block in bounds).
+  typedef typename VolOct<T>::type RT;
+  auto pos = [&](int jj) {
+    const float s = float(jj) * step;
+    return tp + dir * (s + 0.5f * fminf(step, D - s));
+  };
+  RT rec = {};
   bool parked = false, ood_seen = false;
   int next = 0;  // wave-uniform: next pixel of the span to hand out
   for (;;) {
@@ -218,6 +228,7 @@ __device__ __forceinline__ bool nn_tile(const RayFrame& rf, const VolGeom& g, co
             tp = eye + dir * tn + half;
             j = 0;
             dst = make_float4(0.f, 0.f, 0.f, 0.f);
+            rec = volume_oct_load<T>(g, pos(0));
           } else {
             out[size_t(py) * rf.W + px] = make_float4(0.f, 0.f, 0.f, 0.f);  // the ray misses the box
           }
@@ -240,7 +251,8 @@ __device__ __forceinline__ bool nn_tile(const RayFrame& rf, const VolGeom& g, co
       } else {
         h = fminf(step, D - s);
         p = tp + dir * (s + 0.5f * h);
-        src = sample_tf(s_tf, ntf, sample_volume_oct<T>(g, p));
+        src = sample_tf(s_tf, ntf, volume_oct_lerp<T>(g, rec, p));
+        rec = volume_oct_load<T>(g, pos(j + 1));
         ++cnt[0];
         if (src.w > 0.f) parked = true;
         else ++j;
