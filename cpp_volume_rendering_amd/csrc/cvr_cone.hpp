@@ -198,20 +198,21 @@ __device__ __forceinline__ float ct_occlusion(const void* __restrict__ pyr, cons
 // The surrogate's 16 inputs (see the CPU oracle's nn_features): for occlusion segment i
 // the opacity read on the axial cone (2i) and on the ring (2i+1), 7 shadow segment
 // opacities along the light (8..14), the local opacity (15).  The corner-block loads of
-// a group of feat_group<F>() inputs are issued before the first of them is used (texel
-// weights and falloff x segment length kept in registers), so one memory round trip
-// serves the group; each read is gauss_ext_level's, with the segment length folded into
-// the falloff factor.  Groups of 8 suit the 32-byte fp32 records and groups of 4 the
-// 16-byte ones (512^3 surrogate frame at 1024^2: fp32 6.98 -> 6.84 ms with 8, fp16
-// 6.57 -> 6.82 ms with 8; 16 spills).  The group only orders the loads: every input is
-// computed the same way whatever its size.
+// a group of feat_group() inputs are issued together (texel weights and falloff x segment
+// length kept in registers), so one memory round trip serves the group, and the groups
+// are software-pipelined: the next group's loads are in flight while the current group
+// is interpolated (512^3 surrogate frame at 1024^2, fp16: 5.80 -> 5.57 ms).  Each read is
+// gauss_ext_level's, with the segment length folded into the falloff factor.  Groups of 4
+// keep two groups of 32-byte fp32 records within the registers of two waves per SIMD.
+// The groups only order the loads: every input is computed the same way whatever their
+// size.
 typedef const __attribute__((address_space(4))) FeatTap* ConstTaps;
 
 #ifndef CVR_FEAT_GROUP
 #define CVR_FEAT_GROUP 0
 #endif
 template <int F>
-__host__ __device__ constexpr int feat_group() { return CVR_FEAT_GROUP ? CVR_FEAT_GROUP : (F == 0 ? 8 : 4); }
+__host__ __device__ constexpr int feat_group() { return CVR_FEAT_GROUP ? CVR_FEAT_GROUP : 4; }
 
 // Byte offset of corner block (x0, y0, z0) in a level of `nbx` x `nby` bricks, records of
 // 2^kShift bytes: brick number, then the 6 bits of in-brick position (brick_index).  The
@@ -235,37 +236,52 @@ __device__ __forceinline__ void nn_feats_at(const void* __restrict__ pyr, const 
   static_assert(kNFeat % kFeatGroup == 0, "the load groups must tile the inputs");
   // the table is wave-uniform and read-only: constant address space -> scalar loads
   const ConstTaps ft = (ConstTaps)(uintptr_t)taps;
-#pragma unroll
-  for (int g0 = 0; g0 < kNFeat; g0 += kFeatGroup) {
+  // one group's records, texel weights and falloff x segment length
+  struct Grp {
     B b[kFeatGroup];
     float fx[kFeatGroup], fy[kFeatGroup], fz[kFeatGroup], w[kFeatGroup];
+  };
+  auto issue = [&](int g0, Grp& G) {
 #pragma unroll
     for (int j = 0; j < kFeatGroup; ++j) {
       const int i = g0 + j;
       const f3 a = i < 2 * kNOcc ? k : L;
       const f3 tp = i == kNFeat - 1 ? p : p + a * ft[i].dist;
       int x0, y0, z0;
-      quad_axis_f(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dmax[0], x0, fx[j]);
-      quad_axis_f(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dmax[1], y0, fy[j]);
-      quad_axis_f(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dmax[2], z0, fz[j]);
+      quad_axis_f(fmaf(tp.x, ft[i].ts[0], -0.5f), ft[i].dmax[0], x0, G.fx[j]);
+      quad_axis_f(fmaf(tp.y, ft[i].ts[1], -0.5f), ft[i].dmax[1], y0, G.fy[j]);
+      quad_axis_f(fmaf(tp.z, ft[i].ts[2], -0.5f), ft[i].dmax[2], z0, G.fz[j]);
       // the level's first byte (a wave-uniform address), then the record's byte offset
       const char* lv = static_cast<const char*>(pyr) + ((size_t(ft[i].boff_hi) << 32) | ft[i].boff_lo);
       if constexpr (kWide)
-        b[j] = *reinterpret_cast<const B*>(lv + (brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]) << kShift));
+        G.b[j] = *reinterpret_cast<const B*>(lv + (brick_index(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]) << kShift));
       else
-        b[j] = *reinterpret_cast<const B*>(lv + oct_offset32<kShift>(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]));
+        G.b[j] = *reinterpret_cast<const B*>(lv + oct_offset32<kShift>(x0, y0, z0, ft[i].nb[0], ft[i].nb[1]));
       // border falloff x segment length, both scaled by log2(e) on the host
       const float cx = __builtin_amdgcn_fmed3f(tp.x, 0.f, pg.G[0]) - tp.x;
       const float cy = __builtin_amdgcn_fmed3f(tp.y, 0.f, pg.G[1]) - tp.y;
       const float cz = __builtin_amdgcn_fmed3f(tp.z, 0.f, pg.G[2]) - tp.z;
-      w[j] = __builtin_amdgcn_exp2f(-(cx * cx + cy * cy + cz * cz) * ft[i].fall_k) * ft[i].seg;
+      G.w[j] = __builtin_amdgcn_exp2f(-(cx * cx + cy * cy + cz * cz) * ft[i].fall_k) * ft[i].seg;
     }
+  };
+  auto finish = [&](int g0, const Grp& G) {
+#pragma unroll
+    for (int j = 0; j < kFeatGroup; ++j)
+      f[g0 + j] = 1.f - __builtin_amdgcn_exp2f(-(pyr_lerp<F>(G.b[j], G.fx[j], G.fy[j], G.fz[j], pg.one) * G.w[j]));
+  };
+  // software pipeline over the groups: group g + 1's loads go out before group g is
+  // computed, so one group is always in flight while the other is interpolated
+  constexpr int kGroups = kNFeat / kFeatGroup;
+  Grp ga, gb;
+  issue(0, ga);
+#pragma unroll
+  for (int g = 0; g < kGroups; ++g) {
+    Grp& cur = (g & 1) ? gb : ga;
+    Grp& nxt = (g & 1) ? ga : gb;
+    if (g + 1 < kGroups) issue((g + 1) * kFeatGroup, nxt);
     // keep the machine scheduler from pulling the lerps up between the loads
     __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int j = 0; j < kFeatGroup; ++j) {
-      f[g0 + j] = 1.f - __builtin_amdgcn_exp2f(-(pyr_lerp<F>(b[j], fx[j], fy[j], fz[j], pg.one) * w[j]));
-    }
+    finish(g * kFeatGroup, cur);
   }
 }
 
