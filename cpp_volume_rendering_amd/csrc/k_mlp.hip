@@ -11,11 +11,13 @@
 // exactly the B operand of the next layer: inference keeps all activations in
 // registers; only the input features pass through LDS once.
 //
-// Training (k_mlp_train, k_mlp_update): one wavefront per 64-sample chunk runs the
-// forward on the register path, the activation gradients through W^T tiles in
-// registers, and the weight gradients as MFMA over samples from sample-major bf16
-// copies in LDS; it writes its chunk's gradient slab (no atomics).  k_mlp_update sums
-// the slabs, takes the Adam step and writes the bf16 packs of the new weights.
+// Training (k_mlp_train, k_mlp_update): a workgroup of four wavefronts takes a 64-sample
+// chunk, 16 samples per wavefront.  Each wavefront runs the forward on the register path
+// and the activation gradients through W^T tiles in registers; the weight gradients are
+// MFMA contractions over the chunk's 64 samples from sample-major bf16 copies in LDS that
+// all four wavefronts fill, each wavefront computing a quarter of the output tiles, so the
+// workgroup writes one gradient slab (no atomics).  k_mlp_update sums the slabs, takes the
+// Adam step and writes the bf16 packs of the new weights.
 #include "cvr_nn.hpp"
 #include <cmath>
 #include <cstring>
@@ -76,10 +78,11 @@ constexpr int kBwdShorts = kBTiles4 * 64 * 4 + (kBTiles3 + kBTiles2) * 64 * 8;
 constexpr int kBwdBase3 = kBTiles4 * 64 * 4, kBwdBase2 = kBwdBase3 + kBTiles3 * 64 * 8;
 constexpr size_t kTrainPackBytes = size_t(kPackShorts) * 2 + size_t(kNBias) * 4 + size_t(kBwdShorts) * 2;
 // Per-chunk gradient slab: the kNParams gradients of a chunk's samples, then their
-// squared-error sum.  A chunk is one training wavefront's samples: 32 gives every CU two
-// wavefronts on a 16384-sample step.
+// squared-error sum.  A chunk is one training workgroup's samples (four wavefronts of 16):
+// a 16384-sample step is 256 workgroups, one per CU, one wavefront per SIMD.
 constexpr int kSlab = kNParams + 16;
-constexpr int kTrainChunk = 32;
+constexpr int kTrainChunk = 64;
+constexpr int kTrainWaves = 4;
 // LDS row stride (bf16) of the sample-major copies of activations and gradients: 64 samples
 // + 8 of padding, so that 16 rows read at one column fall on distinct banks.
 constexpr int kTS = 72;
@@ -154,33 +157,34 @@ __device__ __forceinline__ void store_act(short* sT, bf16x8v_t a, int c, int u) 
   for (int e = 0; e < 8; ++e) sT[chunk_unit(c, q, e) * kTS + 16 * u + r] = v[e >> 1][e & 1];
 }
 
-// Weight and bias gradients of one layer over the wave's 32 NKC samples: dW[o][k] =
-// sum_s G[o][s] H[k][s] and db[o] = sum_s G[o][s] from the sample-major LDS copies sG
-// (`out` rows) and sH (`in` rows), as 16x16x32 MFMA over 32-sample steps; written to
-// the slab (dW row-major at offW, db at offB).
-template <int NKC>
+// Weight and bias gradients of one layer over a chunk's 64 samples: dW[o][k] = sum_s
+// G[o][s] H[k][s] and db[o] = sum_s G[o][s] from the sample-major LDS copies sG (`out`
+// rows) and sH (`in` rows), as 16x16x32 MFMA over two 32-sample steps; written to the
+// slab (dW row-major at offW, db at offB).  Wavefront w takes row tile w of a 64-row
+// layer (every column tile), or column tile w of the 16-row output layer.
 __device__ __forceinline__ void weight_grads(const short* sG, const short* sH, int out, int in, float* slab, int offW,
-                                             int offB) {
+                                             int offB, int w) {
   const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
   typedef short s16x8_t __attribute__((ext_vector_type(8)));
   const s16x8_t ones_bits = {0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80, 0x3F80};  // bf16 1.0
   const bf16x8v_t ones = __builtin_bit_cast(bf16x8v_t, ones_bits);
-  for (int mt = 0; mt < out / 16; ++mt) {
-    bf16x8v_t ga[NKC];
+  const int mt = out == kH ? w : 0;
+  const int nt0 = out == kH ? 0 : w, nt1 = out == kH ? in / 16 : w + 1;
+  bf16x8v_t ga[2];
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc)
-      ga[kc] = *reinterpret_cast<const bf16x8v_t*>(sG + (16 * mt + r) * kTS + 32 * kc + 8 * q);
-    for (int nt = 0; nt < in / 16; ++nt) {
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < 2; ++kc) ga[kc] = *reinterpret_cast<const bf16x8v_t*>(sG + (16 * mt + r) * kTS + 32 * kc + 8 * q);
+  for (int nt = nt0; nt < nt1; ++nt) {
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kc = 0; kc < NKC; ++kc)
-        acc = mfma32(ga[kc], *reinterpret_cast<const bf16x8v_t*>(sH + (16 * nt + r) * kTS + 32 * kc + 8 * q), acc);
+    for (int kc = 0; kc < 2; ++kc)
+      acc = mfma32(ga[kc], *reinterpret_cast<const bf16x8v_t*>(sH + (16 * nt + r) * kTS + 32 * kc + 8 * q), acc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) slab[offW + (16 * mt + 4 * q + e) * in + 16 * nt + r] = acc[e];
-    }
+    for (int e = 0; e < 4; ++e) slab[offW + (16 * mt + 4 * q + e) * in + 16 * nt + r] = acc[e];
+  }
+  if (out == kH || w == 0) {
     f32x4_t db = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) db = mfma32(ga[kc], ones, db);
+    for (int kc = 0; kc < 2; ++kc) db = mfma32(ga[kc], ones, db);
     if (r == 0) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) slab[offB + 16 * mt + 4 * q + e] = db[e];
@@ -188,50 +192,52 @@ __device__ __forceinline__ void weight_grads(const short* sG, const short* sH, i
   }
 }
 
-// One optimiser step's gradients: workgroup = one wavefront = one chunk of 16 NU samples
-// (NU column tiles of 16; NU = 2 or 4).
+// One optimiser step's gradients: a workgroup of four wavefronts takes a chunk of 64
+// samples, 16 per wavefront (one MFMA column tile each).
 // X: [n][16], Tg: [n][2]; slab c (kSlab floats) receives d/dparams of
 // sum_s (y_s - t_s)^2 / (2 norm_n) over the chunk's samples, and that sum.  Forward and
 // activation gradients stay in registers (the renderer's MFMA register path, then W^T
-// tiles in the same K order); only the weight-gradient contractions over samples read
-// sample-major bf16 copies in LDS.
-template <int NU>
-__global__ __launch_bounds__(64) void k_mlp_train(const unsigned char* __restrict__ pack, const float* __restrict__ X,
-                                                  const float* __restrict__ Tg, int n, float norm_n,
-                                                  float* __restrict__ slabs) {
+// tiles in the same K order); the weight-gradient contractions over the chunk read
+// sample-major bf16 copies in LDS (wavefront w's samples in columns 16w..16w+15).
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned char* __restrict__ pack,
+                                                                const float* __restrict__ X,
+                                                                const float* __restrict__ Tg, int n, float norm_n,
+                                                                float* __restrict__ slabs) {
   __shared__ __align__(16) short sWf[kPackShorts];
   __shared__ __align__(16) float sB[kNBias];
   __shared__ __align__(16) short sWb[kBwdShorts];
-  __shared__ __align__(16) float sX[kNFeat * 64 + 2 * 64];
+  __shared__ __align__(16) float sXall[kTrainWaves][16 * kInRow];
   __shared__ __align__(16) short sG[kH * kTS];
   __shared__ __align__(16) short sH[kH * kTS];
-  const int lane = threadIdx.x, r = lane & 15, q = lane >> 4;
+  __shared__ float sLoss[kTrainWaves];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
   {
     const uint4* src = reinterpret_cast<const uint4*>(pack);
     uint4* d0 = reinterpret_cast<uint4*>(sWf);
-    for (int i = lane; i < kPackShorts / 8; i += 64) d0[i] = src[i];
+    for (int i = tid; i < kPackShorts / 8; i += 64 * kTrainWaves) d0[i] = src[i];
     const float* bsrc = reinterpret_cast<const float*>(pack + size_t(kPackShorts) * 2);
-    for (int i = lane; i < kNBias; i += 64) sB[i] = bsrc[i];
+    for (int i = tid; i < kNBias; i += 64 * kTrainWaves) sB[i] = bsrc[i];
     const uint4* wsrc = reinterpret_cast<const uint4*>(pack + size_t(kPackShorts) * 2 + size_t(kNBias) * 4);
     uint4* d1 = reinterpret_cast<uint4*>(sWb);
-    for (int i = lane; i < kBwdShorts / 8; i += 64) d1[i] = wsrc[i];
+    for (int i = tid; i < kBwdShorts / 8; i += 64 * kTrainWaves) d1[i] = wsrc[i];
   }
-  static_assert(NU == 2 || NU == 4, "32- or 64-sample chunks");
-  constexpr int NKC = NU / 2;
-  const int s0 = blockIdx.x * (16 * NU);
+  const int s0 = blockIdx.x * kTrainChunk + 16 * w;  // this wavefront's first sample
   const int s = s0 + lane;
-  const bool valid = lane < 16 * NU && s < n;
+  const bool valid = lane < 16 && s < n;
   float f[kNFeat];
 #pragma unroll
   for (int i = 0; i < kNFeat; ++i) f[i] = valid ? X[size_t(s) * kNFeat + i] : 0.f;
-  // inputs: bf16 rows for the forward operand (as mlp_wave) and the sample-major copy for dW1
+  // inputs: bf16 rows for the forward operand (as mlp_wave)
+  float* sX = sXall[w];
   typedef unsigned int u32x2v_t __attribute__((ext_vector_type(2)));
-  u32x2v_t* row = reinterpret_cast<u32x2v_t*>(sX + lane * kInRow);
+  if (lane < 16) {
+    u32x2v_t* row = reinterpret_cast<u32x2v_t*>(sX + lane * kInRow);
 #pragma unroll
-  for (int i = 0; i < kNFeat / 4; ++i) {
-    const f32x2_t a = {f[4 * i], f[4 * i + 1]}, b = {f[4 * i + 2], f[4 * i + 3]};
-    row[i] = u32x2v_t{__builtin_bit_cast(unsigned int, __builtin_convertvector(a, bf16x2v_t)),
-                      __builtin_bit_cast(unsigned int, __builtin_convertvector(b, bf16x2v_t))};
+    for (int i = 0; i < kNFeat / 4; ++i) {
+      const f32x2_t a = {f[4 * i], f[4 * i + 1]}, b = {f[4 * i + 2], f[4 * i + 3]};
+      row[i] = u32x2v_t{__builtin_bit_cast(unsigned int, __builtin_convertvector(a, bf16x2v_t)),
+                        __builtin_bit_cast(unsigned int, __builtin_convertvector(b, bf16x2v_t))};
+    }
   }
   __syncthreads();  // packs and inputs in LDS
   const bf16x4_t* W1 = reinterpret_cast<const bf16x4_t*>(sWf);
@@ -243,75 +249,49 @@ __global__ __launch_bounds__(64) void k_mlp_train(const unsigned char* __restric
   const bf16x8v_t* V3 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase3);
   const bf16x8v_t* V2 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase2);
   // ---- forward, activations kept as bf16 B operands
-  f32x4_t h[NU][4];
-  bf16x8v_t a1[NU][2], a2[NU][2], a3[NU][2];
+  f32x4_t h[4];
+  bf16x8v_t a1[2], a2[2], a3[2];
+  {
+    const bf16x4_t xb = *reinterpret_cast<const bf16x4_t*>(sX + r * kInRow + 2 * q);
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const bf16x4_t xb = *reinterpret_cast<const bf16x4_t*>(sX + (u * 16 + r) * kInRow + 2 * q);
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) h[u][mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
-    a1[u][0] = relu_pack(h[u][0], h[u][1]);
-    a1[u][1] = relu_pack(h[u][2], h[u][3]);
+    for (int mt = 0; mt < 4; ++mt) h[mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
+    a1[0] = relu_pack(h[0], h[1]);
+    a1[1] = relu_pack(h[2], h[3]);
   }
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8v_t wa = W2[(mt * 2) * 64 + lane], wb = W2[(mt * 2 + 1) * 64 + lane];
-    const f32x4_t bias = B4[16 + mt * 4 + q];
+  for (int mt = 0; mt < 4; ++mt)
+    h[mt] = mfma32(W2[(mt * 2 + 1) * 64 + lane], a1[1], mfma32(W2[(mt * 2) * 64 + lane], a1[0], B4[16 + mt * 4 + q]));
+  a2[0] = relu_pack(h[0], h[1]);
+  a2[1] = relu_pack(h[2], h[3]);
 #pragma unroll
-    for (int u = 0; u < NU; ++u) h[u][mt] = mfma32(wb, a1[u][1], mfma32(wa, a1[u][0], bias));
-  }
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    a2[u][0] = relu_pack(h[u][0], h[u][1]);
-    a2[u][1] = relu_pack(h[u][2], h[u][3]);
-  }
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    const bf16x8v_t wa = W3[(mt * 2) * 64 + lane], wb = W3[(mt * 2 + 1) * 64 + lane];
-    const f32x4_t bias = B4[32 + mt * 4 + q];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) h[u][mt] = mfma32(wb, a2[u][1], mfma32(wa, a2[u][0], bias));
-  }
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    a3[u][0] = relu_pack(h[u][0], h[u][1]);
-    a3[u][1] = relu_pack(h[u][2], h[u][3]);
-  }
+  for (int mt = 0; mt < 4; ++mt)
+    h[mt] = mfma32(W3[(mt * 2 + 1) * 64 + lane], a2[1], mfma32(W3[(mt * 2) * 64 + lane], a2[0], B4[32 + mt * 4 + q]));
+  a3[0] = relu_pack(h[0], h[1]);
+  a3[1] = relu_pack(h[2], h[3]);
   // ---- output gradient g4 = (y - t) / norm_n on outputs 0, 1 (rows e < 2 of lanes q = 0)
   float* slab = slabs + size_t(blockIdx.x) * kSlab;
   const float inv_n = 1.0f / norm_n;
   float lsum = 0.f;
-  f32x4_t g4[NU];
+  f32x4_t g4 = {0.f, 0.f, 0.f, 0.f};
   {
-    const bf16x8v_t w40 = W4[lane], w41 = W4[64 + lane];
-    const f32x4_t b4 = B4[48 + q];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      const f32x4_t y = mfma32(w41, a3[u][1], mfma32(w40, a3[u][0], b4));
-      const int su = s0 + 16 * u + r;
-      f32x4_t g = {0.f, 0.f, 0.f, 0.f};
-      if (q == 0 && su < n) {
-        const float e0 = y[0] - Tg[size_t(su) * 2], e1 = y[1] - Tg[size_t(su) * 2 + 1];
-        lsum += e0 * e0 + e1 * e1;
-        g[0] = e0 * inv_n;
-        g[1] = e1 * inv_n;
-      }
-      g4[u] = g;
+    const f32x4_t y = mfma32(W4[64 + lane], a3[1], mfma32(W4[lane], a3[0], B4[48 + q]));
+    const int su = s0 + r;
+    if (q == 0 && su < n) {
+      const float e0 = y[0] - Tg[size_t(su) * 2], e1 = y[1] - Tg[size_t(su) * 2 + 1];
+      lsum = e0 * e0 + e1 * e1;
+      g4[0] = e0 * inv_n;
+      g4[1] = e1 * inv_n;
     }
   }
   // ---- layer 4: dW4 = G4 H3^T, then G3 = (W4^T G4) masked by h3 > 0
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    store_tile(sG, g4[u], 0, u);
-    store_act(sH, a3[u][0], 0, u);
-    store_act(sH, a3[u][1], 1, u);
-  }
-  wave_sync();
-  weight_grads<NKC>(sG, sH, kOut, kH, slab, kOffW4, kOffB4);
-  f32x4_t g[NU][4];
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const f32x2_t p0 = {g4[u][0], g4[u][1]}, p1 = {g4[u][2], g4[u][3]};
+  store_tile(sG, g4, 0, w);
+  store_act(sH, a3[0], 0, w);
+  store_act(sH, a3[1], 1, w);
+  __syncthreads();
+  weight_grads(sG, sH, kOut, kH, slab, kOffW4, kOffB4, w);
+  f32x4_t g[4];
+  {
+    const f32x2_t p0 = {g4[0], g4[1]}, p1 = {g4[2], g4[3]};
     typedef unsigned int u32x2_t2 __attribute__((ext_vector_type(2)));
     const u32x2_t2 gb = {__builtin_bit_cast(unsigned int, __builtin_convertvector(p0, bf16x2v_t)),
                          __builtin_bit_cast(unsigned int, __builtin_convertvector(p1, bf16x2v_t))};
@@ -319,56 +299,48 @@ __global__ __launch_bounds__(64) void k_mlp_train(const unsigned char* __restric
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-      const short* act = reinterpret_cast<const short*>(&a3[u][mt >> 1]) + 4 * (mt & 1);
-      g[u][mt] = mask_tile(mfma16(V4[mt * 64 + lane], gop, z), act);
+      const short* act = reinterpret_cast<const short*>(&a3[mt >> 1]) + 4 * (mt & 1);
+      g[mt] = mask_tile(mfma16(V4[mt * 64 + lane], gop, z), act);
     }
   }
   // ---- layers 3 and 2: dW = G H^T, then G_in = (W^T G) masked by the layer below
 #pragma unroll
   for (int l = 3; l >= 2; --l) {
-    wave_sync();  // the previous layer's reads of sG / sH are done
+    __syncthreads();  // every wavefront's reads of sG / sH for the layer above are done
 #pragma unroll
-    for (int u = 0; u < NU; ++u) {
-#pragma unroll
-      for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[u][mt], 16 * mt, u);
-      const bf16x8v_t* below = l == 3 ? a2[u] : a1[u];
-      store_act(sH, below[0], 0, u);
-      store_act(sH, below[1], 1, u);
-    }
-    wave_sync();
-    weight_grads<NKC>(sG, sH, kH, kH, slab, l == 3 ? kOffW3 : kOffW2, l == 3 ? kOffB3 : kOffB2);
+    for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[mt], 16 * mt, w);
+    const bf16x8v_t* below = l == 3 ? a2 : a1;
+    store_act(sH, below[0], 0, w);
+    store_act(sH, below[1], 1, w);
+    __syncthreads();
+    weight_grads(sG, sH, kH, kH, slab, l == 3 ? kOffW3 : kOffW2, l == 3 ? kOffB3 : kOffB2, w);
     const bf16x8v_t* V = l == 3 ? V3 : V2;
-    bf16x8v_t gb[NU][2];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      gb[u][0] = grad_pack(g[u][0], g[u][1]);
-      gb[u][1] = grad_pack(g[u][2], g[u][3]);
-    }
+    const bf16x8v_t gb0 = grad_pack(g[0], g[1]), gb1 = grad_pack(g[2], g[3]);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-      const bf16x8v_t va = V[(mt * 2) * 64 + lane], vb = V[(mt * 2 + 1) * 64 + lane];
-#pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-        const bf16x8v_t* act = l == 3 ? a2[u] : a1[u];
-        g[u][mt] = mask_tile(mfma32(vb, gb[u][1], mfma32(va, gb[u][0], z)),
-                             reinterpret_cast<const short*>(&act[mt >> 1]) + 4 * (mt & 1));
-      }
+      const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+      g[mt] = mask_tile(mfma32(V[(mt * 2 + 1) * 64 + lane], gb1, mfma32(V[(mt * 2) * 64 + lane], gb0, z)),
+                        reinterpret_cast<const short*>(&below[mt >> 1]) + 4 * (mt & 1));
     }
   }
   // ---- layer 1: dW1 = G1 X^T
-  wave_sync();
+  __syncthreads();
 #pragma unroll
-  for (int u = 0; u < NU; ++u)
+  for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[mt], 16 * mt, w);
+  if (lane < 16) {
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) store_tile(sG, g[u][mt], 16 * mt, u);
-#pragma unroll
-  for (int i = 0; i < kNFeat; ++i) sH[i * kTS + lane] = bf16_bits(f[i]);
-  // columns 16 NU..71 of X's rows are never read (the reads stop at column 16 NU - 1)
-  wave_sync();
-  weight_grads<NKC>(sG, sH, kH, kNFeat, slab, kOffW1, kOffB1);
+    for (int i = 0; i < kNFeat; ++i) sH[i * kTS + 16 * w + lane] = bf16_bits(f[i]);
+  }
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off, 64);
-  if (lane == 0) slab[kNParams] = lsum * 0.5f * inv_n;
+  if (lane == 0) sLoss[w] = lsum;
+  __syncthreads();
+  weight_grads(sG, sH, kH, kNFeat, slab, kOffW1, kOffB1, w);
+  if (tid == 0) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < kTrainWaves; ++k) t += sLoss[k];
+    slab[kNParams] = t * 0.5f * inv_n;
+  }
 }
 
 // Sum of the gradient slabs; mode 0 writes it (and the loss) to grad / loss, mode 1 also
@@ -713,7 +685,7 @@ int nn_gradient(cvr_ctx* c, const float* d_X, const float* d_T, int n, hipStream
   if (rc) return rc;
   const int chunks = (n + kTrainChunk - 1) / kTrainChunk;
   launch_update(c, 0, 2, 0.f, 1.f, 1.f, st);  // the packs of the current weights
-  hipLaunchKernelGGL(k_mlp_train<kTrainChunk / 16>, dim3(chunks), dim3(64), 0, st,
+  hipLaunchKernelGGL(k_mlp_train, dim3(chunks), dim3(64 * kTrainWaves), 0, st,
                      static_cast<const unsigned char*>(c->d_nn_tpack), d_X, d_T, n, float(n), c->d_nn_slabs);
   launch_update(c, chunks, 0, 0.f, 1.f, 1.f, st);
   CVR_HIP_CHECK(hipGetLastError());
@@ -809,7 +781,7 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   const int B = std::min(N.batch, n);
   const int steps_per_epoch = n / B;
   const int total = steps_per_epoch * N.epochs;
-  const int chunks = (B + kTrainChunk - 1) / kTrainChunk;  // one wavefront (and gradient slab) each
+  const int chunks = (B + kTrainChunk - 1) / kTrainChunk;  // one workgroup (and gradient slab) each
   rc = train_buffers(c, B);
   if (rc) { cleanup(); return rc; }
   launch_update(c, 0, 2, 0.f, 1.f, 1.f, st);  // packs of the initial weights
@@ -817,7 +789,7 @@ int nn_train(cvr_ctx* c, hipStream_t st) {
   double bc1 = 1.0, bc2 = 1.0;
   for (int t = 0; t < total; ++t) {
     const int off = (t % steps_per_epoch) * B;
-    hipLaunchKernelGGL(k_mlp_train<kTrainChunk / 16>, dim3(chunks), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_mlp_train, dim3(chunks), dim3(64 * kTrainWaves), 0, st,
                        static_cast<const unsigned char*>(c->d_nn_tpack), d_X + size_t(off) * kNFeat, d_T + size_t(off) * 2,
                        B, float(B), c->d_nn_slabs);
     bc1 *= b1;
