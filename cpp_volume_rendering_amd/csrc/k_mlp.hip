@@ -203,24 +203,44 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
                                                                 const float* __restrict__ X,
                                                                 const float* __restrict__ Tg, int n, float norm_n,
                                                                 float* __restrict__ slabs) {
-  __shared__ __align__(16) short sWf[kPackShorts];
-  __shared__ __align__(16) float sB[kNBias];
-  __shared__ __align__(16) short sWb[kBwdShorts];
   __shared__ __align__(16) float sXall[kTrainWaves][16 * kInRow];
   __shared__ __align__(16) short sG[kH * kTS];
   __shared__ __align__(16) short sH[kH * kTS];
   __shared__ float sLoss[kTrainWaves];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, r = lane & 15, q = lane >> 4;
-  {
-    const uint4* src = reinterpret_cast<const uint4*>(pack);
-    uint4* d0 = reinterpret_cast<uint4*>(sWf);
-    for (int i = tid; i < kPackShorts / 8; i += 64 * kTrainWaves) d0[i] = src[i];
-    const float* bsrc = reinterpret_cast<const float*>(pack + size_t(kPackShorts) * 2);
-    for (int i = tid; i < kNBias; i += 64 * kTrainWaves) sB[i] = bsrc[i];
-    const uint4* wsrc = reinterpret_cast<const uint4*>(pack + size_t(kPackShorts) * 2 + size_t(kNBias) * 4);
-    uint4* d1 = reinterpret_cast<uint4*>(sWb);
-    for (int i = tid; i < kBwdShorts / 8; i += 64 * kTrainWaves) d1[i] = wsrc[i];
+  // Every lane's weight operands (forward tiles, W^T tiles, its biases) straight from the
+  // packs into registers, all loads in flight at once: with one wavefront per SIMD the
+  // registers are there, and no LDS staging or barrier precedes the forward.
+  const bf16x4_t* W1g = reinterpret_cast<const bf16x4_t*>(pack);
+  const bf16x8v_t* W2g = reinterpret_cast<const bf16x8v_t*>(pack + size_t(kTiles1) * 64 * 4 * 2);
+  const bf16x8v_t* W3g = W2g + kTiles2 * 64;
+  const bf16x8v_t* W4g = W3g + kTiles3 * 64;
+  const f32x4_t* B4g = reinterpret_cast<const f32x4_t*>(pack + size_t(kPackShorts) * 2);
+  const unsigned char* bwd = pack + size_t(kPackShorts) * 2 + size_t(kNBias) * 4;
+  const bf16x4_t* V4g = reinterpret_cast<const bf16x4_t*>(bwd);
+  const bf16x8v_t* V3g = reinterpret_cast<const bf16x8v_t*>(bwd + size_t(kBwdBase3) * 2);
+  const bf16x8v_t* V2g = reinterpret_cast<const bf16x8v_t*>(bwd + size_t(kBwdBase2) * 2);
+  bf16x4_t w1[4], v4[4];
+  bf16x8v_t w2[8], w3[8], w4[2], v3[8], v2[8];
+  f32x4_t b1[4], b2[4], b3[4], b4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    w1[t] = W1g[t * 64 + lane];
+    v4[t] = V4g[t * 64 + lane];
+    b1[t] = B4g[t * 4 + q];
+    b2[t] = B4g[16 + t * 4 + q];
+    b3[t] = B4g[32 + t * 4 + q];
   }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    w2[t] = W2g[t * 64 + lane];
+    w3[t] = W3g[t * 64 + lane];
+    v3[t] = V3g[t * 64 + lane];
+    v2[t] = V2g[t * 64 + lane];
+  }
+  w4[0] = W4g[lane];
+  w4[1] = W4g[64 + lane];
+  b4 = B4g[48 + q];
   const int s0 = blockIdx.x * kTrainChunk + 16 * w;  // this wavefront's first sample
   const int s = s0 + lane;
   const bool valid = lane < 16 && s < n;
@@ -239,33 +259,25 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
                         __builtin_bit_cast(unsigned int, __builtin_convertvector(b, bf16x2v_t))};
     }
   }
-  __syncthreads();  // packs and inputs in LDS
-  const bf16x4_t* W1 = reinterpret_cast<const bf16x4_t*>(sWf);
-  const bf16x8v_t* W2 = reinterpret_cast<const bf16x8v_t*>(sWf + kTiles1 * 64 * 4);
-  const bf16x8v_t* W3 = W2 + kTiles2 * 64;
-  const bf16x8v_t* W4 = W3 + kTiles3 * 64;
-  const f32x4_t* B4 = reinterpret_cast<const f32x4_t*>(sB);
-  const bf16x4_t* V4 = reinterpret_cast<const bf16x4_t*>(sWb);
-  const bf16x8v_t* V3 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase3);
-  const bf16x8v_t* V2 = reinterpret_cast<const bf16x8v_t*>(sWb + kBwdBase2);
+  wave_sync();  // this wavefront's input rows in LDS
   // ---- forward, activations kept as bf16 B operands
   f32x4_t h[4];
   bf16x8v_t a1[2], a2[2], a3[2];
   {
     const bf16x4_t xb = *reinterpret_cast<const bf16x4_t*>(sX + r * kInRow + 2 * q);
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) h[mt] = mfma16(W1[mt * 64 + lane], xb, B4[mt * 4 + q]);
+    for (int mt = 0; mt < 4; ++mt) h[mt] = mfma16(w1[mt], xb, b1[mt]);
     a1[0] = relu_pack(h[0], h[1]);
     a1[1] = relu_pack(h[2], h[3]);
   }
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
-    h[mt] = mfma32(W2[(mt * 2 + 1) * 64 + lane], a1[1], mfma32(W2[(mt * 2) * 64 + lane], a1[0], B4[16 + mt * 4 + q]));
+    h[mt] = mfma32(w2[mt * 2 + 1], a1[1], mfma32(w2[mt * 2], a1[0], b2[mt]));
   a2[0] = relu_pack(h[0], h[1]);
   a2[1] = relu_pack(h[2], h[3]);
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt)
-    h[mt] = mfma32(W3[(mt * 2 + 1) * 64 + lane], a2[1], mfma32(W3[(mt * 2) * 64 + lane], a2[0], B4[32 + mt * 4 + q]));
+    h[mt] = mfma32(w3[mt * 2 + 1], a2[1], mfma32(w3[mt * 2], a2[0], b3[mt]));
   a3[0] = relu_pack(h[0], h[1]);
   a3[1] = relu_pack(h[2], h[3]);
   // ---- output gradient g4 = (y - t) / norm_n on outputs 0, 1 (rows e < 2 of lanes q = 0)
@@ -274,7 +286,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
   float lsum = 0.f;
   f32x4_t g4 = {0.f, 0.f, 0.f, 0.f};
   {
-    const f32x4_t y = mfma32(W4[64 + lane], a3[1], mfma32(W4[lane], a3[0], B4[48 + q]));
+    const f32x4_t y = mfma32(w4[1], a3[1], mfma32(w4[0], a3[0], b4));
     const int su = s0 + r;
     if (q == 0 && su < n) {
       const float e0 = y[0] - Tg[size_t(su) * 2], e1 = y[1] - Tg[size_t(su) * 2 + 1];
@@ -300,7 +312,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
     for (int mt = 0; mt < 4; ++mt) {
       const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
       const short* act = reinterpret_cast<const short*>(&a3[mt >> 1]) + 4 * (mt & 1);
-      g[mt] = mask_tile(mfma16(V4[mt * 64 + lane], gop, z), act);
+      g[mt] = mask_tile(mfma16(v4[mt], gop, z), act);
     }
   }
   // ---- layers 3 and 2: dW = G H^T, then G_in = (W^T G) masked by the layer below
@@ -314,12 +326,12 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
     store_act(sH, below[1], 1, w);
     __syncthreads();
     weight_grads(sG, sH, kH, kH, slab, l == 3 ? kOffW3 : kOffW2, l == 3 ? kOffB3 : kOffB2, w);
-    const bf16x8v_t* V = l == 3 ? V3 : V2;
+    const bf16x8v_t* V = l == 3 ? v3 : v2;
     const bf16x8v_t gb0 = grad_pack(g[0], g[1]), gb1 = grad_pack(g[2], g[3]);
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       const f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-      g[mt] = mask_tile(mfma32(V[(mt * 2 + 1) * 64 + lane], gb1, mfma32(V[(mt * 2) * 64 + lane], gb0, z)),
+      g[mt] = mask_tile(mfma32(V[mt * 2 + 1], gb1, mfma32(V[mt * 2], gb0, z)),
                         reinterpret_cast<const short*>(&below[mt >> 1]) + 4 * (mt & 1));
     }
   }
