@@ -139,7 +139,7 @@ static void defaults(int technique, void* p) {
     e->ks = 0.8f;  // renderingparameters.cpp:25-26
     e->shininess = 30.0f;
     e->specular[0] = e->specular[1] = e->specular[2] = 1.0f;  // lightsourcelist.cpp:24
-    e->sat_layout = 0;  // corner blocks: 76.5 ms against 79.4 ms (median of 12) at 256^3 @1024^2
+    e->sat_layout = 0;  // corner blocks: 53.2 ms against 61.0 ms at 256^3 @1024^2
   } else if (technique == CVR_CONE_GT) {
     // crtgtrenderer.cpp:32-51 (both lighting terms on here: with both off the shading
     // divides by zero)

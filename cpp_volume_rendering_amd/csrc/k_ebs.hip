@@ -137,83 +137,157 @@ __device__ __forceinline__ void sat_axis(double x, int n, int& i0, double& f, bo
   below = fl < 0.0;
   i0 = min(max(int(fl), 0), n - 1);
 }
-// GL_LINEAR tap of the SAT at texel coordinate (x, y, z), :74-83, in FP64, through the
-// corner blocks (one 64-byte record).
-__device__ __forceinline__ double sat_tap(const float4* __restrict__ sat, const EbsK& k, double x, double y,
-                                          double z) {
-  int x0, y0, z0;
-  double fx, fy, fz;
-  bool bx, by, bz;
-  sat_axis(x, k.w, x0, fx, bx);
-  sat_axis(y, k.h, y0, fy, by);
-  sat_axis(z, k.d, z0, fz, bz);
-  const float4* r = sat + 4 * ((size_t(z0) * k.h + y0) * k.w + x0);
-  const float4 q0 = r[0], q1 = r[1], q2 = r[2], q3 = r[3];
-  // (x, y) plane z0: q0 = (x0, y0), (x1, y0); q1 = (x0, y1), (x1, y1); plane z1: q2, q3
-  double c000 = hilo(q0.x, q0.y), c100 = hilo(q0.z, q0.w), c010 = hilo(q1.x, q1.y), c110 = hilo(q1.z, q1.w);
-  double c001 = hilo(q2.x, q2.y), c101 = hilo(q2.z, q2.w), c011 = hilo(q3.x, q3.y), c111 = hilo(q3.z, q3.w);
-  if (bz) { c001 = c000; c101 = c100; c011 = c010; c111 = c110; }
-  if (by) { c010 = c000; c110 = c100; c011 = c001; c111 = c101; }
-  if (bx) { c100 = c000; c110 = c010; c101 = c001; c111 = c011; }
-  double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
-  double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
-  double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
-  return c0 * (1 - fz) + c1 * fz;
-}
-
-// The same tap through the linear (hi, lo) SAT, 8 bytes per cell: each of the four rows
-// (y0 | y0+1, z0 | z0+1) gives its x-pair (x0, x0+1) in one 16-byte load.  x0 in
-// [0, n-2] with the weight clamped to [0, 1] is CLAMP_TO_EDGE at both ends (below the
-// first texel: cell 0; past the last: cell n-1); inside, the same taps and weights.
+// GL_LINEAR taps of the SAT (:74-83) are evaluated in FP64 from one of two layouts:
+// layout 0, the corner blocks (one 64-byte record per tap, sat_axis); layout 1, the linear
+// (hi, lo) SAT, 8 bytes per cell, where each of the four rows (y0 | y0+1, z0 | z0+1) gives
+// its x-pair (x0, x0+1) in one 16-byte load.  In layout 1, x0 in [0, n-2] with the weight
+// clamped to [0, 1] is CLAMP_TO_EDGE at both ends (below the first texel: cell 0; past
+// the last: cell n-1); inside, the same taps and weights.
 typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
 __device__ __forceinline__ void pair_axis(double x, int n, int& i0, double& f) {
   const double fl = floor(x);
   i0 = min(max(int(fl), 0), n - 2);
   f = fmin(fmax(x - double(i0), 0.0), 1.0);
 }
-__device__ __forceinline__ double sat_tap_lin(const float2* __restrict__ sat, const EbsK& k, double x, double y,
-                                              double z) {
-  int x0, y0, z0;
-  double fx, fy, fz;
-  pair_axis(x, k.w, x0, fx);
-  pair_axis(y, k.h, y0, fy);
-  pair_axis(z, k.d, z0, fz);
-  const size_t b = (size_t(z0) * k.h + y0) * k.w + x0, row = size_t(k.w), plane = size_t(k.w) * k.h;
-  const f4a8 q0 = *reinterpret_cast<const f4a8*>(sat + b);
-  const f4a8 q1 = *reinterpret_cast<const f4a8*>(sat + b + row);
-  const f4a8 q2 = *reinterpret_cast<const f4a8*>(sat + b + plane);
-  const f4a8 q3 = *reinterpret_cast<const f4a8*>(sat + b + plane + row);
-  const double c000 = hilo(q0.x, q0.y), c100 = hilo(q0.z, q0.w), c010 = hilo(q1.x, q1.y), c110 = hilo(q1.z, q1.w);
-  const double c001 = hilo(q2.x, q2.y), c101 = hilo(q2.z, q2.w), c011 = hilo(q3.x, q3.y), c111 = hilo(q3.z, q3.w);
-  const double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
-  const double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
-  const double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
-  return c0 * (1 - fz) + c1 * fz;
-}
-// A SAT tap in the layout L: 0 corner blocks (64 bytes per cell), 1 linear pairs.
-template <int L>
-__device__ __forceinline__ double tap(const void* __restrict__ sat, const EbsK& k, double x, double y, double z) {
-  if constexpr (L == 0) return sat_tap(static_cast<const float4*>(sat), k, x, y, z);
-  else return sat_tap_lin(static_cast<const float2*>(sat), k, x, y, z);
-}
-
-// EvaluateSAT3D (:87-100) after the border offset and clamp (:102-110 / :181-183).
-template <int L>
-__device__ __forceinline__ double box_sum(const void* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
+// The texel coordinates of a box sum's corners (EvaluateSAT3D :87-100, after the border
+// offset and clamp :102-110 / :181-183).
+struct BoxC {
+  double x1, x2, y1, y2, z1, z2;
+};
+__device__ __forceinline__ BoxC box_coords(const EbsK& k, f3 p1, f3 p2) {
   p1 = mk3(fminf(fmaxf(p1.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
            fminf(fmaxf(p1.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
            fminf(fmaxf(p1.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
   p2 = mk3(fminf(fmaxf(p2.x + k.scale[0], k.minSAT[0]), k.maxSAT[0]),
            fminf(fmaxf(p2.y + k.scale[1], k.minSAT[1]), k.maxSAT[1]),
            fminf(fmaxf(p2.z + k.scale[2], k.minSAT[2]), k.maxSAT[2]));
-  const double x1 = double(p1.x) * k.sat_tex_mul[0] - 0.5, x2 = double(p2.x) * k.sat_tex_mul[0] - 0.5;
-  const double y1 = double(p1.y) * k.sat_tex_mul[1] - 0.5, y2 = double(p2.y) * k.sat_tex_mul[1] - 0.5;
-  const double z1 = double(p1.z) * k.sat_tex_mul[2] - 0.5, z2 = double(p2.z) * k.sat_tex_mul[2] - 0.5;
-  double V1 = tap<L>(sat, k, x2, y2, z2), V2 = tap<L>(sat, k, x1, y2, z2);
-  double V3 = tap<L>(sat, k, x2, y2, z1), V4 = tap<L>(sat, k, x1, y2, z1);
-  double V5 = tap<L>(sat, k, x2, y1, z2), V6 = tap<L>(sat, k, x1, y1, z2);
-  double V7 = tap<L>(sat, k, x2, y1, z1), V8 = tap<L>(sat, k, x1, y1, z1);
-  return V1 - V2 - V3 + V4 - V5 + V6 + V7 - V8;
+  return BoxC{double(p1.x) * k.sat_tex_mul[0] - 0.5, double(p2.x) * k.sat_tex_mul[0] - 0.5,
+              double(p1.y) * k.sat_tex_mul[1] - 0.5, double(p2.y) * k.sat_tex_mul[1] - 0.5,
+              double(p1.z) * k.sat_tex_mul[2] - 0.5, double(p2.z) * k.sat_tex_mul[2] - 0.5};
+}
+
+// One corner tap of a box sum, loaded: its record (layout 0: the corner block, four
+// float4; layout 1: its four x pairs) and FP64 weights.  tap_issue starts the loads and
+// tap_eval interpolates, so that later taps' loads are in flight while earlier ones are
+// interpolated.
+typedef float fv4 __attribute__((ext_vector_type(4)));
+template <int L> struct TapSlot {
+  fv4 q[4];
+  double fx, fy, fz;
+  unsigned int below;  // layout 0: bit 0 (x), 1 (y), 2 (z) below the first texel
+};
+template <int L>
+__device__ __forceinline__ void tap_issue(const void* __restrict__ sat, const EbsK& k, double x, double y, double z,
+                                          TapSlot<L>& S) {
+  int x0, y0, z0;
+  if constexpr (L == 0) {
+    bool bx, by, bz;
+    sat_axis(x, k.w, x0, S.fx, bx);
+    sat_axis(y, k.h, y0, S.fy, by);
+    sat_axis(z, k.d, z0, S.fz, bz);
+    S.below = unsigned(bx) | (unsigned(by) << 1) | (unsigned(bz) << 2);
+    const fv4* r = static_cast<const fv4*>(sat) + 4 * ((size_t(z0) * k.h + y0) * k.w + x0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) S.q[i] = r[i];
+  } else {
+    pair_axis(x, k.w, x0, S.fx);
+    pair_axis(y, k.h, y0, S.fy);
+    pair_axis(z, k.d, z0, S.fz);
+    S.below = 0u;
+    const float2* sp = static_cast<const float2*>(sat);
+    const size_t b = (size_t(z0) * k.h + y0) * k.w + x0, row = size_t(k.w), plane = size_t(k.w) * k.h;
+    S.q[0] = *reinterpret_cast<const f4a8*>(sp + b);
+    S.q[1] = *reinterpret_cast<const f4a8*>(sp + b + row);
+    S.q[2] = *reinterpret_cast<const f4a8*>(sp + b + plane);
+    S.q[3] = *reinterpret_cast<const f4a8*>(sp + b + plane + row);
+  }
+}
+template <int L>
+__device__ __forceinline__ double tap_eval(const TapSlot<L>& S) {
+  const fv4 q0 = S.q[0], q1 = S.q[1], q2 = S.q[2], q3 = S.q[3];
+  // (x, y) plane z0: q0 = (x0, y0), (x1, y0); q1 = (x0, y1), (x1, y1); plane z1: q2, q3
+  double c000 = hilo(q0.x, q0.y), c100 = hilo(q0.z, q0.w), c010 = hilo(q1.x, q1.y), c110 = hilo(q1.z, q1.w);
+  double c001 = hilo(q2.x, q2.y), c101 = hilo(q2.z, q2.w), c011 = hilo(q3.x, q3.y), c111 = hilo(q3.z, q3.w);
+  if constexpr (L == 0) {
+    // below the first texel of an axis its second cell is replaced by the first
+    if (S.below & 4u) { c001 = c000; c101 = c100; c011 = c010; c111 = c110; }
+    if (S.below & 2u) { c010 = c000; c110 = c100; c011 = c001; c111 = c101; }
+    if (S.below & 1u) { c100 = c000; c110 = c010; c101 = c001; c111 = c011; }
+  }
+  const double fx = S.fx, fy = S.fy, fz = S.fz;
+  const double c00 = c000 * (1 - fx) + c100 * fx, c10 = c010 * (1 - fx) + c110 * fx;
+  const double c01 = c001 * (1 - fx) + c101 * fx, c11 = c011 * (1 - fx) + c111 * fx;
+  const double c0 = c00 * (1 - fy) + c10 * fy, c1 = c01 * (1 - fy) + c11 * fy;
+  return c0 * (1 - fz) + c1 * fz;
+}
+// Corner v of a box: V1..V8 of EvaluateSAT3D = (x2, y2, z2), (x1, y2, z2), (x2, y2, z1),
+// (x1, y2, z1), then the same with y1; the box sum is V1 - V2 - V3 + V4 - V5 + V6 + V7 - V8.
+template <int L>
+__device__ __forceinline__ void corner_issue(const void* __restrict__ sat, const EbsK& k, const BoxC& c, int v,
+                                             TapSlot<L>& S) {
+  tap_issue<L>(sat, k, (v & 1) ? c.x1 : c.x2, (v & 4) ? c.y1 : c.y2, (v & 2) ? c.z1 : c.z2, S);
+}
+
+// A stream of box sums with four corner taps always in flight: slot j holds corner j or
+// j + 4 of the current box; as each tap is interpolated its slot takes the tap four
+// corners later in the stream (the next box's first four corners once the current box's
+// last four are issued).  next(c, wq) gives the next box's corners and a weight and
+// returns false at the end of the stream; fold(S, wq) takes the box sums in stream order,
+// each summed left to right as V1 - V2 - V3 + V4 - V5 + V6 + V7 - V8.  Box geometry never
+// depends on loaded values, so the next box is known before the current one is summed.
+template <int L, class Next, class Fold>
+__device__ __forceinline__ void box_stream(const void* __restrict__ sat, const EbsK& k, Next&& next, Fold&& fold) {
+  BoxC c;
+  float wq = 0.f;
+  if (!next(c, wq)) return;
+  TapSlot<L> s0, s1, s2, s3;
+  corner_issue<L>(sat, k, c, 0, s0);
+  corner_issue<L>(sat, k, c, 1, s1);
+  corner_issue<L>(sat, k, c, 2, s2);
+  corner_issue<L>(sat, k, c, 3, s3);
+  for (;;) {
+    double acc = tap_eval<L>(s0);
+    corner_issue<L>(sat, k, c, 4, s0);
+    acc = acc - tap_eval<L>(s1);
+    corner_issue<L>(sat, k, c, 5, s1);
+    acc = acc - tap_eval<L>(s2);
+    corner_issue<L>(sat, k, c, 6, s2);
+    acc = acc + tap_eval<L>(s3);
+    corner_issue<L>(sat, k, c, 7, s3);
+    float wn = 0.f;
+    // the current box's corners are all issued; at the end of the stream next() leaves c
+    // as it is, and the slots take the last box's corners again (in bounds, never summed),
+    // so that every lane issues the same loads and no slot keeps two values
+    const bool more = next(c, wn);
+    acc = acc - tap_eval<L>(s0);
+    corner_issue<L>(sat, k, c, 0, s0);
+    acc = acc + tap_eval<L>(s1);
+    corner_issue<L>(sat, k, c, 1, s1);
+    acc = acc + tap_eval<L>(s2);
+    corner_issue<L>(sat, k, c, 2, s2);
+    acc = acc - tap_eval<L>(s3);
+    corner_issue<L>(sat, k, c, 3, s3);
+    fold(acc, wq);
+    if (!more) break;
+    wq = wn;
+  }
+}
+
+// EvaluateSAT3D (:87-100) of one box.
+template <int L>
+__device__ __forceinline__ double box_sum(const void* __restrict__ sat, const EbsK& k, f3 p1, f3 p2) {
+  bool first = true;
+  double S = 0.0;
+  box_stream<L>(
+      sat, k,
+      [&](BoxC& c, float&) {
+        if (!first) return false;
+        c = box_coords(k, p1, p2);
+        first = false;
+        return true;
+      },
+      [&](double v, float) { S = v; });
+  return S;
 }
 
 // ExtinctionAmbientOcclusion (:112-146).
@@ -221,16 +295,27 @@ template <int L>
 __device__ float ebs_ao(const void* __restrict__ sat, const EbsK& k, f3 tx) {
   const f3 sc = ld3(k.scale);
   const double rad = k.ao_radius;
-  double S0 = box_sum<L>(sat, k, tx - sc * k.ao_radius, tx + sc * k.ao_radius);
-  double t = S0 * (1.0 / (rad * rad));
-  double Si = S0;
-  for (int i = 1; i < k.shells; ++i) {
-    const float r1f = k.ao_radius * float(i + 1);
-    const double r1 = rad * double(i + 1);
-    double S1 = box_sum<L>(sat, k, tx - sc * r1f, tx + sc * r1f);
-    t = t + (S1 - Si) * (1.0 / (r1 * r1));
-    Si = S1;
-  }
+  int gi = 0, fi = 0;
+  double t = 0.0, Si = 0.0;
+  box_stream<L>(
+      sat, k,
+      [&](BoxC& c, float&) {
+        if (gi >= k.shells) return false;
+        const float r1f = k.ao_radius * float(gi + 1);
+        c = box_coords(k, tx - sc * r1f, tx + sc * r1f);
+        ++gi;
+        return true;
+      },
+      [&](double S1, float) {
+        if (fi == 0) {
+          t = S1 * (1.0 / (rad * rad));
+        } else {
+          const double r1 = rad * double(fi + 1);
+          t = t + (S1 - Si) * (1.0 / (r1 * r1));
+        }
+        Si = S1;
+        ++fi;
+      });
   const double rn = rad * double(k.shells);
   return __expf(-float((1.0 / (rn * rn)) * t));
 }
@@ -243,14 +328,23 @@ __device__ float ebs_shadow_table(const void* __restrict__ sat, const EbsK& k, c
                                   f3 tx, unsigned int& nbox) {
   const float pa = comp(tx, k.axis), lo = k.minVol[k.axis], hi = k.maxVol[k.axis];
   double st = 0.0;
-  for (int r = 0; r < nrows; ++r) {
-    const ShadowRow& R = rows[r];
-    const float nxt = pa + R.next;
-    if (!(nxt > lo && nxt < hi)) break;
-    f3 p1 = tx + ld3(R.o1), p2 = tx + ld3(R.o2);
-    st += box_sum<L>(sat, k, p1, p2) * double(R.inv_vq) * double(k.sdw_weight);
-    ++nbox;
-  }
+  int r = 0;
+  box_stream<L>(
+      sat, k,
+      [&](BoxC& c, float& wq) {
+        if (r >= nrows) return false;
+        const ShadowRow& R = rows[r];
+        const float nxt = pa + R.next;
+        if (!(nxt > lo && nxt < hi)) return false;
+        c = box_coords(k, tx + ld3(R.o1), tx + ld3(R.o2));
+        wq = R.inv_vq;
+        ++r;
+        return true;
+      },
+      [&](double S, float inv_vq) {
+        st += S * double(inv_vq) * double(k.sdw_weight);
+        ++nbox;
+      });
   return __expf(-float(st));
 }
 
@@ -266,18 +360,24 @@ __device__ float ebs_shadow_point(const void* __restrict__ sat, const EbsK& k, f
   const float pa = comp(tx, a), lo = k.minVol[a], hi = k.maxVol[a];
   float apos = cg.apos0;
   double st = 0.0;
-  while ((apos / cv[a]) < k.sdw_max_dist) {
-    const float nxt = pa + (apos + cg.interval);
-    if (!(nxt > lo && nxt < hi)) break;
-    float o1[3], o2[3];
-    cone_box<float>(cg, apos, o1, o2);
-    f3 p1 = tx + ld3(o1), p2 = tx + ld3(o2);
-    const float vq = (fabsf(p1.x - p2.x) / k.scale[0]) * (fabsf(p1.y - p2.y) / k.scale[1]) *
-                     (fabsf(p1.z - p2.z) / k.scale[2]);
-    st += (box_sum<L>(sat, k, p1, p2) / double(vq)) * double(k.sdw_weight);
-    ++nbox;
-    apos = apos + cg.interval;
-  }
+  box_stream<L>(
+      sat, k,
+      [&](BoxC& c, float& vq) {
+        if (!((apos / cv[a]) < k.sdw_max_dist)) return false;
+        const float nxt = pa + (apos + cg.interval);
+        if (!(nxt > lo && nxt < hi)) return false;
+        float o1[3], o2[3];
+        cone_box<float>(cg, apos, o1, o2);
+        const f3 p1 = tx + ld3(o1), p2 = tx + ld3(o2);
+        vq = (fabsf(p1.x - p2.x) / k.scale[0]) * (fabsf(p1.y - p2.y) / k.scale[1]) * (fabsf(p1.z - p2.z) / k.scale[2]);
+        c = box_coords(k, p1, p2);
+        apos = apos + cg.interval;
+        return true;
+      },
+      [&](double S, float vq) {
+        st += (S / double(vq)) * double(k.sdw_weight);
+        ++nbox;
+      });
   return __expf(-float(st));
 }
 
@@ -293,10 +393,12 @@ __device__ __forceinline__ void ebs_load_rows(const ShadowRow* __restrict__ rows
 }
 
 template <typename T, int L>
-// Two waves per SIMD: left to itself the compiler takes 256 VGPRs + 7 AGPRs, one wave
-// more than the register file holds twice, and the latency-bound SAT reads then run at
-// one wave per SIMD (256^3 @ 1024^2 with shadows: 82.6 -> 75.7 ms, the same frame).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
+// One wave per SIMD: the box streams keep four corner records in flight per lane, which
+// with the FP64 interpolation takes more than the 256 architectural VGPRs; at one wave
+// per SIMD the excess lives in AGPRs instead of scratch memory (256^3 @ 1024^2 with
+// shadows: 62.0 ms at two waves per SIMD with scratch spills, 53.2 ms at one; 76.2 ms
+// before the streams, one box at a time at two waves per SIMD).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_ebs(RayFrame rf, EbsK k, const float4* __restrict__ tf, int ntf,
                                               const void* __restrict__ sat, const ShadowRow* __restrict__ rows,
                                               float step, float et, PhongK ph,
                                               const float4* __restrict__ grad, float4* __restrict__ out,
@@ -381,7 +483,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // (AO, shadow) at texture-space position (i + 1/2) * cell, 1 for a term that is off,
 // from the renderer's own AO and shadow evaluations.  One lane per cell.
 template <int L>
-__global__ __launch_bounds__(256) void k_lc_ebs(EbsK k, const void* __restrict__ sat, const ShadowRow* __restrict__ rows,
+// 64-lane workgroups, at most one wave per SIMD (a 32^3 cache is 512 waves): the box
+// streams' registers beyond the 256 architectural VGPRs live in AGPRs.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_lc_ebs(EbsK k, const void* __restrict__ sat, const ShadowRow* __restrict__ rows,
                                                 int rx, int ry, int rz, f3 cell, float2* __restrict__ lc) {
   extern __shared__ float4 smem[];
   ShadowRow* s_rows = reinterpret_cast<ShadowRow*>(smem);

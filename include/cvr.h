@@ -127,8 +127,8 @@ typedef struct cvr_ebs_params {
   /* Speed and memory only (pixels agree to the last bits): the SAT layout the march
    * reads, 0 corner blocks (64 bytes per cell beside the table, one record per GL_LINEAR
    * tap), 1 the linear (hi, lo) table itself (8 bytes per cell, four 16-byte x-pairs per
-   * tap).  0 (76.5 ms against 79.4 ms per 256^3 frame at 1024^2, median of 12 on
-   * MI355X); 1 needs 1/8 of the SAT memory. */
+   * tap).  0 (53.2 ms against 61.0 ms per 256^3 frame at 1024^2 on MI355X); 1 needs
+   * 1/8 of the SAT memory. */
   int sat_layout;
 } cvr_ebs_params;
 
