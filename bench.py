@@ -162,7 +162,8 @@ def read_footprint(tech, r, wl, bpv, light_cache):
         total += (nx + 2) * (ny + 2) * (nz + 2) * (8 if r.params().sat_layout == 1 else 64)
     if tech in (N.DOS, N.CONE_TRACE, N.CONE_NN):
         p = r.params()
-        prec = p.density_precision if tech == N.DOS else p.cone.density_precision
+        # DOS and the cone tracer carry density_precision themselves, the surrogate in its cone params
+        prec = p.cone.density_precision if tech == N.CONE_NN else p.density_precision
         cells = sum(int(a.size) for a in r.pyramid())
         total += cells * 8 * (4 if prec == 0 else 2)
     return total
