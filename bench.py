@@ -390,7 +390,7 @@ def bench_sat(args, N, W, stream):
         "vs_baseline": None, "dtype": "f32 (two-float)", "data": "synthetic (seeded)",
         "config": {"workload": wl.name, "volume": f"{nx}x{ny}x{nz} u16", "sat": f"{nx + 2}x{ny + 2}x{nz + 2} float2",
                    "algorithmic_bytes": algo, "corner_block_copy_ms": round(ms_b, 5),
-                   "kernels": "k_sat_x, k_sat_strided (y), k_sat_strided (z)"},
+                   "kernels": "k_sat_x, k_sat_seg (y), k_sat_seg (z)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
     }

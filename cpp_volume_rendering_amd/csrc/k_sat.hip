@@ -8,10 +8,15 @@
 // cell is stored as float2 (hi, lo).
 //
 // Pass 1 (x): one wavefront per row, a 64-lane shuffle scan per 64-cell chunk
-//   with the chunk carry broadcast from lane 63; reads the raw voxels through the
-//   per-value extinction LUT, writes float2.
-// Pass 2 (y) / pass 3 (z): one lane per column, lanes along x (coalesced), each
-//   lane walks its column with 8 loads in flight.
+//   with the chunk carry broadcast from lane 63; the voxels (and their extinction
+//   through the per-value LUT) of up to kXChunks chunks are loaded before the chunks
+//   are scanned in turn; writes float2.
+// Pass 2 (y) / pass 3 (z): segmented column scans.  A workgroup takes 64 columns,
+//   lanes along x (coalesced); each of its wavefronts holds one segment of every
+//   column in registers (all of its loads in flight at once), scans it, and adds the
+//   totals of the segments before it, exchanged through LDS: every cell is read and
+//   written once.  Columns longer than 16 segments of kSegCells fall back to a lane
+//   walking its whole column.
 #include "cvr_ctx.hpp"
 
 namespace cvr {
@@ -33,6 +38,9 @@ __device__ __forceinline__ df df_add(df a, df b) {
   return df{hi, lo};
 }
 
+constexpr int kXChunks = 8;    // 64-cell chunks of a row whose loads are issued together
+constexpr int kSegCells = 48;  // cells of a column segment held in registers
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_sat_x(const T* __restrict__ vox, const float* __restrict__ lut, int nx,
                                                 int ny, int nz, float2* __restrict__ sat) {
@@ -45,21 +53,72 @@ __global__ __launch_bounds__(256) void k_sat_x(const T* __restrict__ vox, const 
   const T* src = border_row ? vox : vox + (size_t(z - 1) * ny + (y - 1)) * nx;
   float2* dstrow = sat + size_t(row) * w;
   df carry{0.f, 0.f};
-  for (int base = 0; base < w; base += 64) {
-    const int x = base + lane;
-    float v = 0.f;
-    if (!border_row && x >= 1 && x <= nx) v = lut[src[x - 1]];
-    df acc{v, 0.f};
+  for (int base0 = 0; base0 < w; base0 += 64 * kXChunks) {
+    float vv[kXChunks];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      float oh = __shfl_up(acc.hi, off, 64);
-      float ol = __shfl_up(acc.lo, off, 64);
-      if (lane >= off) acc = df_add(acc, df{oh, ol});
+    for (int c = 0; c < kXChunks; ++c) {
+      const int x = base0 + 64 * c + lane;
+      vv[c] = (!border_row && x >= 1 && x <= nx) ? lut[src[x - 1]] : 0.f;
     }
-    acc = df_add(acc, carry);
-    if (x < w) dstrow[x] = make_float2(acc.hi, acc.lo);
-    carry.hi = __shfl(acc.hi, 63, 64);
-    carry.lo = __shfl(acc.lo, 63, 64);
+#pragma unroll
+    for (int c = 0; c < kXChunks; ++c) {
+      const int base = base0 + 64 * c;
+      if (base >= w) break;
+      const int x = base + lane;
+      df acc{vv[c], 0.f};
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        float oh = __shfl_up(acc.hi, off, 64);
+        float ol = __shfl_up(acc.lo, off, 64);
+        if (lane >= off) acc = df_add(acc, df{oh, ol});
+      }
+      acc = df_add(acc, carry);
+      if (x < w) dstrow[x] = make_float2(acc.hi, acc.lo);
+      carry.hi = __shfl(acc.hi, 63, 64);
+      carry.lo = __shfl(acc.lo, 63, 64);
+    }
+  }
+}
+
+// Segmented prefix along a strided axis: lines enumerated by (i0 in [0, n0), i1 in
+// [0, n1)) with base offset i0 + i1 * stride1, `n_line` cells `stride` apart.  Lane l of
+// every wavefront takes line 64 blockIdx.x + l; wavefront s takes cells [s seg, (s + 1)
+// seg) of it, seg = ceil(n_line / NSEG) <= kSegCells, in registers.
+template <int NSEG>
+__global__ __launch_bounds__(64 * NSEG) void k_sat_seg(float2* __restrict__ sat, int n0, int n1, size_t stride1,
+                                                        int n_line, size_t stride) {
+  __shared__ float2 tot[NSEG][64];
+  const int lane = threadIdx.x & 63, sg = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * 64 + lane, i1 = blockIdx.y;
+  const bool line_ok = i0 < n0 && i1 < n1;
+  const int seg = (n_line + NSEG - 1) / NSEG;
+  const int k0 = sg * seg, nk = line_ok ? min(seg, n_line - k0) : 0;  // cells of this segment (may be <= 0)
+  float2* p = sat + size_t(line_ok ? i0 : 0) + size_t(line_ok ? i1 : 0) * stride1 + size_t(k0) * stride;
+  df v[kSegCells];
+#pragma unroll
+  for (int j = 0; j < kSegCells; ++j) {
+    float2 c = make_float2(0.f, 0.f);
+    if (j < nk) c = p[size_t(j) * stride];
+    v[j] = df{c.x, c.y};
+  }
+#pragma unroll
+  for (int j = 1; j < kSegCells; ++j)
+    if (j < nk) v[j] = df_add(v[j - 1], v[j]);
+  // the segment's total (its last cell), then the carry of the segments before it
+  df last{0.f, 0.f};
+#pragma unroll
+  for (int j = 0; j < kSegCells; ++j)
+    if (j == nk - 1) last = v[j];
+  tot[sg][lane] = make_float2(last.hi, last.lo);
+  __syncthreads();
+  df carry{0.f, 0.f};
+  for (int t = 0; t < sg; ++t) carry = df_add(carry, df{tot[t][lane].x, tot[t][lane].y});
+#pragma unroll
+  for (int j = 0; j < kSegCells; ++j) {
+    if (j < nk) {
+      const df r = df_add(carry, v[j]);
+      p[size_t(j) * stride] = make_float2(r.hi, r.lo);
+    }
   }
 }
 
@@ -156,6 +215,32 @@ static int build_sat_oct(cvr_ctx* c, hipStream_t st) {
   return 0;
 }
 
+// A prefix pass along a strided axis: segmented (the fewest power-of-two segments of at
+// most kSegCells cells) up to 16 segments, else one lane per line.
+static int scan_lines(float2* sat, int n0, int n1, size_t stride1, int n_line, size_t stride, hipStream_t st) {
+  const dim3 g(unsigned((n0 + 63) / 64), unsigned(n1));
+  int nseg = 1;
+  while (nseg * kSegCells < n_line && nseg < 16) nseg *= 2;
+  if (nseg * kSegCells < n_line) {
+    hipLaunchKernelGGL(k_sat_strided, dim3((n0 + 255) / 256, n1), dim3(256), 0, st, sat, n0, n1, stride1, n_line,
+                       stride);
+  } else {
+    auto go = [&](auto ns) {
+      constexpr int NS = decltype(ns)::value;
+      hipLaunchKernelGGL(k_sat_seg<NS>, g, dim3(64 * NS), 0, st, sat, n0, n1, stride1, n_line, stride);
+    };
+    switch (nseg) {
+      case 1: go(std::integral_constant<int, 1>{}); break;
+      case 2: go(std::integral_constant<int, 2>{}); break;
+      case 4: go(std::integral_constant<int, 4>{}); break;
+      case 8: go(std::integral_constant<int, 8>{}); break;
+      default: go(std::integral_constant<int, 16>{}); break;
+    }
+  }
+  CVR_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks) {
   const uint64_t key = (c->vol_version << 32) ^ c->tf_version;
   if (c->d_sat && c->sat_built_for == key) {
@@ -194,14 +279,9 @@ int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks) {
     hipLaunchKernelGGL(k_sat_x<uint16_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
                        static_cast<const uint16_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
   CVR_HIP_CHECK(hipGetLastError());
-  // y: lines over (x, z), stride w
-  hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, d), dim3(256), 0, st, c->d_sat, w, d, size_t(w) * h, h,
-                     size_t(w));
-  CVR_HIP_CHECK(hipGetLastError());
-  // z: lines over (x, y), stride w*h
-  hipLaunchKernelGGL(k_sat_strided, dim3((w + 255) / 256, h), dim3(256), 0, st, c->d_sat, w, h, size_t(w), d,
-                     size_t(w) * h);
-  CVR_HIP_CHECK(hipGetLastError());
+  // y: lines over (x, z), stride w; z: lines over (x, y), stride w*h
+  if (const int rc = scan_lines(c->d_sat, w, d, size_t(w) * h, h, size_t(w), st)) return rc;
+  if (const int rc = scan_lines(c->d_sat, w, h, size_t(w), d, size_t(w) * h, st)) return rc;
   if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[1], st));
   c->sat_built_for = key;
   c->sat_oct_built_for = ~0ull;
