@@ -19,6 +19,7 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 import oracle_lib as O  # noqa: E402
+import surrogate_oracle as SO  # noqa: E402
 from cpp_volume_rendering_amd import native as N  # noqa: E402
 from cpp_volume_rendering_amd import workloads as W  # noqa: E402
 
@@ -38,6 +39,8 @@ def img_stats(img, ref, m, stride):
 
 
 def main():
+    print("# Precision report: the HIP path against the FP64 oracle\n")
+    print("Command (from the repository root, on a GPU box): `python tests/precision_report.py > profiles/precision_report.md`\n")
     print("| output | storage / arithmetic | error vs FP64 oracle |")
     print("|---|---|---|")
     # plain caster, 128^3 u8 at 512^2 (every 4th pixel)
@@ -103,24 +106,44 @@ def main():
         ref, m = O.render_dos(wl.volume, tf, dpyr, O.dos_params(p, wl.grid, light), wl.camera(), 1024, 1024, stride=32)
         row("DOS image 256³ @1024² (128³ pyramid)", f"pyramid {names[prec]}, fp32 sections", img_stats(img, ref, m, 32))
 
-    # surrogate: bf16 MLP against the FP64 network, and the trained network against the exact tracer
-    wl = W.cone_blobs(n=128, seed=4, noise_weight=0.25)
-    r = N.Renderer(N.CONE_NN)
-    r.set_volume(wl.volume)
-    r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
-    r.set_params(n_train=1 << 20)
-    r.prepare()
-    w = r.nn_weights().astype(np.float64)
-    rng = np.random.default_rng(3)
-    X = rng.uniform(0, 1, (4096, 16)).astype(np.float32)
-    y = r.nn_eval(X, path=1)
-    yr = O.mlp_forward(w, X.astype(np.float64))[:, :2]
-    row("surrogate MLP outputs (16-64-64-64-16)", "bf16 operands, fp32 accumulate",
-        f"max-abs {np.abs(y - yr).max():.1e} (FP64 network, same weights)")
-    rep = r.nn_report()
-    row("surrogate occlusion / shadow", "bf16 MLP vs exact cone tracer",
-        f"validation RMSE {rep.val_rmse_occ:.4f} / {rep.val_rmse_sdw:.4f} "
-        f"(network-free estimate {rep.base_rmse_occ:.4f} / {rep.base_rmse_sdw:.4f})")
+    # surrogate at the headline size, 512^3 at 1024^2: the frame against the FP64 surrogate
+    # frame (render_nn over the same trained weights, bf16 rounding points included) per
+    # pyramid storage, the image against the exact tracer, the bf16 MLP against the FP64
+    # network, and the trained network against the exact tracer
+    wl = W.surrogate_blobs_noise()
+    cam, light = wl.camera(), wl.light()
+    for prec in (1, 0, 2):
+        r = N.Renderer(N.CONE_NN)
+        r.set_volume(wl.volume)
+        r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        p = r.params()
+        p.cone.density_precision = prec
+        r.set_params(p)
+        r.prepare()
+        img = r.render(cam, 1024, 1024, light=light)
+        ref, m, _ = SO.nn_oracle_frame(wl, r, cam, light, 1024, 1024, stride=32)
+        row("surrogate image 512³ @1024²", f"pyramid {names[prec]}, bf16 MLP{' (default)' if prec == 1 else ''}",
+            img_stats(img, ref, m, 32) + " (vs the FP64 surrogate frame)")
+        if prec != 1:
+            continue
+        ct = N.Renderer(N.CONE_TRACE)
+        ct.set_volume(wl.volume)
+        ct.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
+        ct.set_params(r.params().cone)
+        whole, rgb, ncov = SO.image_rmse(img, ct.render(cam, 1024, 1024, light=light))
+        row("surrogate image 512³ @1024² vs the exact cone tracer", "pyramid fp16, bf16 MLP",
+            f"image RMSE {whole:.4f} over the frame, {rgb:.4f} over RGB of {ncov} covered pixels (acceptance: 0.01)")
+        w = r.nn_weights().astype(np.float64)
+        rng = np.random.default_rng(3)
+        X = rng.uniform(0, 1, (4096, 16)).astype(np.float32)
+        y = r.nn_eval(X, path=1)
+        yr = O.mlp_forward(w, X.astype(np.float64))[:, :2]
+        row("surrogate MLP outputs (16-64-64-64-16)", "bf16 operands, fp32 accumulate",
+            f"max-abs {np.abs(y - yr).max():.1e} (FP64 network, same weights)")
+        rep = r.nn_report()
+        row("surrogate occlusion / shadow", "bf16 MLP vs exact cone tracer",
+            f"validation RMSE {rep.val_rmse_occ:.4f} / {rep.val_rmse_sdw:.4f} "
+            f"(network-free estimate {rep.base_rmse_occ:.4f} / {rep.base_rmse_sdw:.4f})")
 
 
 if __name__ == "__main__":
