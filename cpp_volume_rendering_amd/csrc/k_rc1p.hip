@@ -16,7 +16,10 @@
 // sample position is recomputed from the sample index each step (no running
 // sum), so the step schedule does not drift in fp32.  Positions do not depend on the
 // compositing, so the march is software-pipelined: the corner-block loads of the next
-// kGroup samples are in flight while the current kGroup are composited.
+// kGroup samples are in flight while the current kGroup are composited, and a group's
+// transfer-function reads (LDS) are all issued before its first exit test (128^3 u8 at
+// 512^2: 0.155 ms one sample at a time, 0.106 ms pipelined, 0.094 ms with the reads
+// hoisted).
 #include "cvr_ctx.hpp"
 #include <type_traits>
 
@@ -64,6 +67,17 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
     // are in flight (two register sets, a and b, alternate).  Returns true when the ray
     // ends inside the group.
     auto group = [&](const RT* rec, int j0) {
+      // the group's densities and transfer-function entries first (no side effects), so
+      // the table reads of later samples need not wait for the earlier samples' exit tests
+      float4 srcs[kGroup];
+#pragma unroll
+      for (int k = 0; k < kGroup; ++k) {
+        const int j = j0 + k;
+        const float s = float(j) * step;
+        const float h = fminf(step, D - s);
+        const f3 p = tp + dir * (s + 0.5f * h);
+        srcs[k] = sample_tf(tfp, ntf, volume_oct_lerp<T>(g, rec[k], p));
+      }
 #pragma unroll
       for (int k = 0; k < kGroup; ++k) {
         const int j = j0 + k;
@@ -71,7 +85,7 @@ __global__ __launch_bounds__(256) void k_rc1p(RayFrame rf, VolGeom g, const floa
         if (!(s < D) || j >= kMaxSamples) return true;
         const float h = fminf(step, D - s);
         const f3 p = tp + dir * (s + 0.5f * h);
-        float4 src = sample_tf(tfp, ntf, volume_oct_lerp<T>(g, rec[k], p));
+        float4 src = srcs[k];
         ++nsamp;
         if (src.w > 0.0f) {
           ++nshade;
