@@ -13,10 +13,13 @@
 // cache is fp32 (the reference stores RG16F), laid out x fastest as float2; at the
 // default size it is 256 KiB and stays in L2, so its 8 taps per sample are cache hits.
 #include "cvr_ctx.hpp"
+#include <type_traits>
 
 namespace cvr {
 
 constexpr int kMaxLightCacheRes = 512;
+constexpr int kLcGroup = 4;  // samples whose volume loads are issued together
+template <class T> struct LcVox { typedef T type; };
 
 // GL_LINEAR of the cache at texel coordinates p * tex_mul - 1/2 (clamped to the edge).
 __device__ __forceinline__ float2 lc_sample(const float2* __restrict__ lc, int rx, int ry, int rz, const float tm[3],
@@ -44,7 +47,9 @@ struct LcK {
   float ks;          // Phong specular weight, 0 without shadows (Kspecular counts with ApplyShadow)
 };
 
-template <typename T>
+// PH: the Blinn-Phong branch, a compile-time variant so that the unrolled sample groups
+// without it stay small.
+template <typename T, bool PH>
 __global__ __launch_bounds__(256) void k_lc_render(RayFrame rf, VolGeom g, LcK L, const float2* __restrict__ lc,
                                                    const float4* __restrict__ tf, int ntf, float step, float et,
                                                    PhongK ph, const float4* __restrict__ grad, float4* __restrict__ out,
@@ -69,38 +74,74 @@ __global__ __launch_bounds__(256) void k_lc_render(RayFrame rf, VolGeom g, LcK L
   if (ray_box(eye, dir, g.G, tn, tf_)) {
     const float D = fabsf(tf_ - tn);
     const f3 tp = eye + dir * tn + mk3(0.5f * g.G[0], 0.5f * g.G[1], 0.5f * g.G[2]);
-    for (int j = 0; j < kMaxSamples; ++j) {
+    typedef typename VolOct<T>::type RT;
+    // sample j sits at the midpoint of [s, s + h], s = j step, h = min(step, D - s); for
+    // j past the end the position is still finite and its clamped block in bounds
+    auto pos = [&](int j) {
       const float s = float(j) * step;
-      if (!(s < D)) break;
-      const float h = fminf(step, D - s);
-      const f3 p = tp + dir * (s + 0.5f * h);
-      float4 src = sample_tf(s_tf, ntf, sample_volume_oct<T>(g, p));
-      ++nsamp;
-      if (src.w > 0.0f) {
-        ++nshade;
-        const float2 I = lc_sample(lc, L.rx, L.ry, L.rz, L.tex_mul, p);
-        if (ph.on) {
-          float dd, sp;
-          if (phong_terms(grad, g, ph, p, eye, dd, sp)) {
-            const float s = I.y * (L.ks * sp);
-            src.x = inv_k * (src.x * I.x * L.ka + I.y * (src.x * L.kd * dd)) + s * ph.spec[0];
-            src.y = inv_k * (src.y * I.x * L.ka + I.y * (src.y * L.kd * dd)) + s * ph.spec[1];
-            src.z = inv_k * (src.z * I.x * L.ka + I.y * (src.z * L.kd * dd)) + s * ph.spec[2];
-          }
-        } else {
-          const float m = inv_k * (I.x * L.ka + I.y * L.kd);
-          src.x *= m;
-          src.y *= m;
-          src.z *= m;
-        }
-        const float a = 1.0f - __expf(-src.w * h);
-        const float wgt = (1.0f - dst.w) * a;
-        dst.x += wgt * src.x;
-        dst.y += wgt * src.y;
-        dst.z += wgt * src.z;
-        dst.w += wgt;
-        if (dst.w > et) break;
+      return tp + dir * (s + 0.5f * fminf(step, D - s));
+    };
+    // Samples in groups of kLcGroup, as the plain march: the corner blocks of the next
+    // group in flight while one is composited (two register sets, a and b), and a group's
+    // transfer-function and cache reads issued before its first exit test.  Returns true
+    // when the ray ends inside the group.
+    auto group = [&](const RT* rec, int j0) {
+      float4 srcs[kLcGroup];
+      float2 Is[kLcGroup];
+#pragma unroll
+      for (int k = 0; k < kLcGroup; ++k) {
+        const f3 p = pos(j0 + k);
+        srcs[k] = sample_tf(s_tf, ntf, volume_oct_lerp<T>(g, rec[k], p));
+        Is[k] = make_float2(0.f, 0.f);
+        if (srcs[k].w > 0.0f) Is[k] = lc_sample(lc, L.rx, L.ry, L.rz, L.tex_mul, p);
       }
+#pragma unroll
+      for (int k = 0; k < kLcGroup; ++k) {
+        const int j = j0 + k;
+        const float s = float(j) * step;
+        if (!(s < D) || j >= kMaxSamples) return true;
+        const float h = fminf(step, D - s);
+        float4 src = srcs[k];
+        ++nsamp;
+        if (src.w > 0.0f) {
+          ++nshade;
+          const float2 I = Is[k];
+          if (PH) {
+            const f3 p = tp + dir * (s + 0.5f * h);
+            float dd, sp;
+            if (phong_terms(grad, g, ph, p, eye, dd, sp)) {
+              const float s2 = I.y * (L.ks * sp);
+              src.x = inv_k * (src.x * I.x * L.ka + I.y * (src.x * L.kd * dd)) + s2 * ph.spec[0];
+              src.y = inv_k * (src.y * I.x * L.ka + I.y * (src.y * L.kd * dd)) + s2 * ph.spec[1];
+              src.z = inv_k * (src.z * I.x * L.ka + I.y * (src.z * L.kd * dd)) + s2 * ph.spec[2];
+            }
+          } else {
+            const float m = inv_k * (I.x * L.ka + I.y * L.kd);
+            src.x *= m;
+            src.y *= m;
+            src.z *= m;
+          }
+          const float a = 1.0f - __expf(-src.w * h);
+          const float wgt = (1.0f - dst.w) * a;
+          dst.x += wgt * src.x;
+          dst.y += wgt * src.y;
+          dst.z += wgt * src.z;
+          dst.w += wgt;
+          if (dst.w > et) return true;
+        }
+      }
+      return false;
+    };
+    RT ra[kLcGroup], rb[kLcGroup];
+#pragma unroll
+    for (int k = 0; k < kLcGroup; ++k) ra[k] = volume_oct_load<T>(g, pos(k));
+    for (int j0 = 0;; j0 += 2 * kLcGroup) {
+#pragma unroll
+      for (int k = 0; k < kLcGroup; ++k) rb[k] = volume_oct_load<T>(g, pos(j0 + kLcGroup + k));
+      if (group(ra, j0)) break;
+#pragma unroll
+      for (int k = 0; k < kLcGroup; ++k) ra[k] = volume_oct_load<T>(g, pos(j0 + 2 * kLcGroup + k));
+      if (group(rb, j0 + kLcGroup)) break;
     }
   }
   out[size_t(py) * rf.W + px] = dst;
@@ -148,12 +189,18 @@ int launch_lc_render(cvr_ctx* c, const RayFrame& rf, int apply_occ, int apply_sd
     return -4;
   }
   unsigned long long* cnt = c->collect_stats ? c->d_counters : nullptr;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_lc_render<uint8_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
+  auto launch = [&](auto t, auto p) {
+    using T = typename decltype(t)::type;
+    constexpr bool PHv = decltype(p)::value;
+    hipLaunchKernelGGL((k_lc_render<T, PHv>), dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
                        c->tf_size, step, et, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
-  else
-    hipLaunchKernelGGL(k_lc_render<uint16_t>, dim3(tiles), dim3(256), shm, st, rf, g, L, c->d_lc, c->d_tf_rgbt,
-                       c->tf_size, step, et, ph, c->d_grad, reinterpret_cast<float4*>(out), cnt);
+  };
+  auto by_phong = [&](auto t) {
+    if (ph.on) launch(t, std::true_type{});
+    else launch(t, std::false_type{});
+  };
+  if (c->dtype == CVR_U8) by_phong(LcVox<uint8_t>{});
+  else by_phong(LcVox<uint16_t>{});
   CVR_HIP_CHECK(hipGetLastError());
   c->lc_valid = true;
   return CVR_OK;
