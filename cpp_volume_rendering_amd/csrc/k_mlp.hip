@@ -245,8 +245,30 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
   const int s = s0 + lane;
   const bool valid = lane < 16 && s < n;
   float f[kNFeat];
+  if (valid) {
+    // a sample's 16 inputs are 64 contiguous bytes: four 16-byte loads
+    const float4* xr = reinterpret_cast<const float4*>(X + size_t(s) * kNFeat);
 #pragma unroll
-  for (int i = 0; i < kNFeat; ++i) f[i] = valid ? X[size_t(s) * kNFeat + i] : 0.f;
+    for (int i = 0; i < kNFeat / 4; ++i) {
+      const float4 v = xr[i];
+      f[4 * i] = v.x;
+      f[4 * i + 1] = v.y;
+      f[4 * i + 2] = v.z;
+      f[4 * i + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < kNFeat; ++i) f[i] = 0.f;
+  }
+  // the targets of the samples whose output rows this lane holds (lanes q = 0), read now
+  // so that their latency does not sit between the forward and the backward
+  const int su = s0 + r;
+  const bool has_t = q == 0 && su < n;
+  float t0 = 0.f, t1 = 0.f;
+  if (has_t) {
+    t0 = Tg[size_t(su) * 2];
+    t1 = Tg[size_t(su) * 2 + 1];
+  }
   // inputs: bf16 rows for the forward operand (as mlp_wave)
   float* sX = sXall[w];
   typedef unsigned int u32x2v_t __attribute__((ext_vector_type(2)));
@@ -287,9 +309,8 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_train(const unsigned c
   f32x4_t g4 = {0.f, 0.f, 0.f, 0.f};
   {
     const f32x4_t y = mfma32(w4[1], a3[1], mfma32(w4[0], a3[0], b4));
-    const int su = s0 + r;
-    if (q == 0 && su < n) {
-      const float e0 = y[0] - Tg[size_t(su) * 2], e1 = y[1] - Tg[size_t(su) * 2 + 1];
+    if (has_t) {
+      const float e0 = y[0] - t0, e1 = y[1] - t1;
       lsum = e0 * e0 + e1 * e1;
       g4[0] = e0 * inv_n;
       g4[1] = e1 * inv_n;
