@@ -363,7 +363,7 @@ def bench_batch(args, N, W, torch, stream):
 def bench_sat(args, N, W, stream):
     """The extinction SAT build of the EBS config (256^3 u16 -> 258^3 two-float cells) on
     its own: one step = one build (a transfer-function upload between steps invalidates
-    the table), device time of the three prefix passes from events around them.
+    the table), device time of the prefix passes from events around them.
     Algorithmic bytes: the voxels read once, the (hi, lo) table written once."""
     wl = W.banded_noise(n=args.size or 256, width=1024, height=1024, seed=2)
     r = N.Renderer(N.EBS)
@@ -390,7 +390,7 @@ def bench_sat(args, N, W, stream):
         "vs_baseline": None, "dtype": "f32 (two-float)", "data": "synthetic (seeded)",
         "config": {"workload": wl.name, "volume": f"{nx}x{ny}x{nz} u16", "sat": f"{nx + 2}x{ny + 2}x{nz + 2} float2",
                    "algorithmic_bytes": algo, "corner_block_copy_ms": round(ms_b, 5),
-                   "kernels": "k_sat_x, k_sat_seg (y), k_sat_seg (z)"},
+                   "kernels": "k_sat_xy (x and y), k_sat_seg (z)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
     }

@@ -7,6 +7,9 @@
 // so the table keeps ~2^-44 relative accuracy while every add is an fp32 add; the
 // cell is stored as float2 (hi, lo).
 //
+// Passes 1 and 2 (x and y) run as one kernel (k_sat_xy) when a row has at most 1024
+// cells: one workgroup per z-slice, the voxels read twice and the table written once.
+// Otherwise:
 // Pass 1 (x): one wavefront per row, a 64-lane shuffle scan per 64-cell chunk
 //   with the chunk carry broadcast from lane 63; the voxels (and their extinction
 //   through the per-value LUT) of up to kXChunks chunks are loaded before the chunks
@@ -76,6 +79,139 @@ __global__ __launch_bounds__(256) void k_sat_x(const T* __restrict__ vox, const 
       if (x < w) dstrow[x] = make_float2(acc.hi, acc.lo);
       carry.hi = __shfl(acc.hi, 63, 64);
       carry.lo = __shfl(acc.lo, 63, 64);
+    }
+  }
+}
+
+// Passes x and y fused, for rows of up to 64 E cells: one workgroup per z-slice of
+// xy_waves<E>() wavefronts, wavefront s taking rows [s R, (s + 1) R), lane l the cells
+// l + 64 k (k < E) of each, so every load and store is coalesced.  Phase 1: every
+// wavefront sums its rows' extinction per column (its segment's column totals,
+// exchanged through LDS).  Phase 2: a lane's running column sums start at the totals of
+// the segments above; each row adds its extinction to them, and the row's prefix along
+// x is a DPP scan of each 64-cell chunk plus the total of the chunks before it (lane
+// 63's value).  The voxels are read twice (the second time from L2; gathering the LUT
+// again measured faster than parking the extinction in the table between the phases),
+// the table written once.  The sums run in fp64, one add per cell and direction where the two-float adds
+// of the separate passes take about ten fp32 operations; a cell stores hi = fp32(v),
+// lo = fp32(v - hi).  Workgroups 0 .. nz - 1 take the interior slices z = 1 .. nz, the
+// last two the zero border slices (stores only), so a grid of nz + 2 <= CUs + 2
+// workgroups leaves no interior slice for a second round.
+template <int E>
+constexpr int xy_waves() {
+  return E <= 6 ? 16 : 8;  // 16 wavefronts hold at most 128 VGPRs each
+}
+
+// A lane's 64-bit value moved by a DPP control (kCtrl) within the rows kRowMask
+// enables; +0 in lanes with no source and in the rows left out.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp0(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b)), kCtrl, kRowMask, 0xf, true));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(b >> 32)), kCtrl, kRowMask, 0xf, true));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+// Inclusive prefix sum across the wavefront: shifts by 1, 2, 4, 8 within each 16-lane
+// row, then lane 15 of rows 0 / 2 into rows 1 / 3 and lane 31 into rows 2 and 3.
+__device__ __forceinline__ double wave_scan(double x) {
+  x += dpp0<0x111, 0xf>(x);  // row_shr:1
+  x += dpp0<0x112, 0xf>(x);  // row_shr:2
+  x += dpp0<0x114, 0xf>(x);  // row_shr:4
+  x += dpp0<0x118, 0xf>(x);  // row_shr:8
+  x += dpp0<0x142, 0xa>(x);  // row_bcast:15
+  x += dpp0<0x143, 0xc>(x);  // row_bcast:31
+  return x;
+}
+// Lane 63's value, in every lane.
+__device__ __forceinline__ double lane63(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(b)), 63));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(b >> 32)), 63));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+template <typename T, int E>
+__global__ __launch_bounds__(64 * xy_waves<E>()) void k_sat_xy(const T* __restrict__ vox,
+                                                                const float* __restrict__ lut, int nx, int ny, int nz,
+                                                                float2* __restrict__ sat) {
+  constexpr int kXyWaves = xy_waves<E>();
+  constexpr int G = E <= 8 ? 4 : 2;  // rows whose loads are issued together
+  __shared__ double tot[kXyWaves][64 * E];
+  const int w = nx + 2, h = ny + 2;
+  const int lane = threadIdx.x & 63, sg = threadIdx.x >> 6;
+  const int R = (h + kXyWaves - 1) / kXyWaves;
+  const int ya = min(h, sg * R), yb = min(h, ya + R);
+  const int bz = int(blockIdx.x);
+  const int z = bz < nz ? bz + 1 : (bz == nz ? 0 : nz + 1);
+  float2* slice = sat + size_t(z) * w * h;
+  if (bz >= nz) {  // a border slice: zeros
+    for (int y = ya; y < yb; ++y)
+#pragma unroll
+      for (int k = 0; k < E; ++k)
+        if (lane + 64 * k < w) slice[size_t(y) * w + lane + 64 * k] = make_float2(0.f, 0.f);
+    return;
+  }
+  // voxel values of rows y0 .. y0 + G - 1 at this lane's cells; -1 on the border and
+  // past the segment (nothing is read there)
+  auto raw = [&](int y0, int (&q)[G][E]) {
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      const int y = y0 + r;
+      const bool in = y < yb && y > 0 && y < h - 1;
+      const T* src = vox + (size_t(z - 1) * ny + (in ? y - 1 : 0)) * nx;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int x = lane + 64 * k;
+        q[r][k] = (in && x >= 1 && x <= nx) ? int(src[x - 1]) : -1;
+      }
+    }
+  };
+  auto ext = [&](const int (&q)[G][E], float (&e)[G][E]) {
+#pragma unroll
+    for (int r = 0; r < G; ++r)
+#pragma unroll
+      for (int k = 0; k < E; ++k) e[r][k] = q[r][k] >= 0 ? lut[q[r][k]] : 0.f;
+  };
+  double c[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) c[k] = 0.0;
+  int qn[G][E];
+  raw(ya, qn);
+  for (int y = ya; y < yb; y += G) {
+    float e[G][E];
+    ext(qn, e);
+    raw(y + G, qn);
+#pragma unroll
+    for (int r = 0; r < G; ++r)
+#pragma unroll
+      for (int k = 0; k < E; ++k) c[k] += double(e[r][k]);
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) tot[sg][lane + 64 * k] = c[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    c[k] = 0.0;
+    for (int t = 0; t < sg; ++t) c[k] += tot[t][lane + 64 * k];
+  }
+  raw(ya, qn);
+  for (int y = ya; y < yb; y += G) {
+    float e[G][E];
+    ext(qn, e);
+    raw(y + G, qn);  // the next rows' voxels in flight while these rows are scanned
+#pragma unroll
+    for (int r = 0; r < G; ++r) {
+      if (y + r >= yb) break;
+      float2* row = slice + size_t(y + r) * w;
+      double carry = 0.0;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        c[k] += double(e[r][k]);
+        const double v = wave_scan(c[k]) + carry;
+        const float hi = float(v);
+        if (lane + 64 * k < w) row[lane + 64 * k] = make_float2(hi, float(v - double(hi)));
+        if (k + 1 < E) carry = lane63(v);
+      }
     }
   }
 }
@@ -178,22 +314,20 @@ static void build_ext_lut(const cvr_ctx* c, std::vector<float>& lut) {
 // z..z+1), the +1 neighbours clamped to the grid, as 4 float4 (two (hi, lo) cells each):
 // planes z and z+1, each (x, y), (x+1, y) then (x, y+1), (x+1, y+1).  64 bytes, one
 // cache line: a GL_LINEAR tap of the SAT is one record instead of 8 scattered reads.
+// Four lanes per record, lane p writing float4 p (bit 0: y + 1, bit 1: z + 1), so a
+// wavefront's store covers 16 whole records (1 KiB contiguous) instead of one quarter of
+// each of 64 records.
 __global__ __launch_bounds__(256) void k_sat_oct(const float2* __restrict__ sat, float4* __restrict__ oct, int w, int h,
                                                  int d) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-  const int z = blockIdx.z;
+  const int x = blockIdx.x * 64 + (threadIdx.x >> 2), part = threadIdx.x & 3;
+  const int y = blockIdx.y, z = blockIdx.z;
   if (x >= w || y >= h || z >= d) return;
-  const int x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1), z1 = min(z + 1, d - 1);
-  const size_t sy = size_t(w), sz = size_t(w) * h;
-  const size_t r00 = z * sz + y * sy, r01 = z * sz + y1 * sy, r10 = z1 * sz + y * sy, r11 = z1 * sz + y1 * sy;
-  float4* o = oct + 4 * (r00 + x);
-  const float2 a = sat[r00 + x], b = sat[r00 + x1], c2 = sat[r01 + x], e = sat[r01 + x1];
-  const float2 f = sat[r10 + x], g = sat[r10 + x1], i = sat[r11 + x], j = sat[r11 + x1];
-  o[0] = make_float4(a.x, a.y, b.x, b.y);
-  o[1] = make_float4(c2.x, c2.y, e.x, e.y);
-  o[2] = make_float4(f.x, f.y, g.x, g.y);
-  o[3] = make_float4(i.x, i.y, j.x, j.y);
+  const int x1 = min(x + 1, w - 1);
+  const int yy = (part & 1) ? min(y + 1, h - 1) : y;
+  const int zz = (part & 2) ? min(z + 1, d - 1) : z;
+  const size_t r = (size_t(zz) * h + yy) * w;
+  const float2 a = sat[r + x], b = sat[r + x1];
+  oct[4 * ((size_t(z) * h + y) * w + x) + part] = make_float4(a.x, a.y, b.x, b.y);
 }
 
 // Corner blocks of a built SAT (64 bytes per cell), for the marches that read them.
@@ -207,8 +341,8 @@ static int build_sat_oct(cvr_ctx* c, hipStream_t st) {
       return -5;
     }
   }
-  if (d > 65535) { set_error("summed-area table: depth above 65535"); return -4; }
-  hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned((h + 3) / 4), unsigned(d)), dim3(256), 0, st,
+  if (d > 65535 || h > 65535) { set_error("summed-area table: height or depth above 65535"); return -4; }
+  hipLaunchKernelGGL(k_sat_oct, dim3(unsigned((w + 63) / 64), unsigned(h), unsigned(d)), dim3(256), 0, st,
                      c->d_sat, static_cast<float4*>(c->d_sat_oct), w, h, d);
   CVR_HIP_CHECK(hipGetLastError());
   c->sat_oct_built_for = c->sat_built_for;
@@ -241,6 +375,34 @@ static int scan_lines(float2* sat, int n0, int n1, size_t stride1, int n_line, s
   return 0;
 }
 
+// Passes x and y in one launch (k_sat_xy) when a row fits 16 cells per lane; false
+// otherwise (the caller runs them as two passes).
+static bool launch_sat_xy(cvr_ctx* c, hipStream_t st) {
+  const int w = c->nx + 2, d = c->nz + 2;
+  const int need = (w + 63) / 64;
+  bool launched = false;
+  auto go = [&](auto ecst) {
+    constexpr int E = decltype(ecst)::value;
+    if (launched || need > E) return;
+    launched = true;
+    if (c->dtype == CVR_U8)
+      hipLaunchKernelGGL((k_sat_xy<uint8_t, E>), dim3(unsigned(d)), dim3(64 * xy_waves<E>()), 0, st,
+                         static_cast<const uint8_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+    else
+      hipLaunchKernelGGL((k_sat_xy<uint16_t, E>), dim3(unsigned(d)), dim3(64 * xy_waves<E>()), 0, st,
+                         static_cast<const uint16_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+  };
+  go(std::integral_constant<int, 2>{});
+  go(std::integral_constant<int, 3>{});
+  go(std::integral_constant<int, 4>{});
+  go(std::integral_constant<int, 5>{});
+  go(std::integral_constant<int, 6>{});
+  go(std::integral_constant<int, 8>{});
+  go(std::integral_constant<int, 12>{});
+  go(std::integral_constant<int, 16>{});
+  return launched;
+}
+
 int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks) {
   const uint64_t key = (c->vol_version << 32) ^ c->tf_version;
   if (c->d_sat && c->sat_built_for == key) {
@@ -271,16 +433,21 @@ int build_sat(cvr_ctx* c, hipStream_t st, bool with_blocks) {
       if (!*e) CVR_HIP_CHECK(hipEventCreate(e));
     CVR_HIP_CHECK(hipEventRecord(c->ev_sat[0], st));
   }
-  const int rows = h * d;
-  if (c->dtype == CVR_U8)
-    hipLaunchKernelGGL(k_sat_x<uint8_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
-                       static_cast<const uint8_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
-  else
-    hipLaunchKernelGGL(k_sat_x<uint16_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
-                       static_cast<const uint16_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
-  CVR_HIP_CHECK(hipGetLastError());
-  // y: lines over (x, z), stride w; z: lines over (x, y), stride w*h
-  if (const int rc = scan_lines(c->d_sat, w, d, size_t(w) * h, h, size_t(w), st)) return rc;
+  if (launch_sat_xy(c, st)) {
+    CVR_HIP_CHECK(hipGetLastError());
+  } else {
+    const int rows = h * d;
+    if (c->dtype == CVR_U8)
+      hipLaunchKernelGGL(k_sat_x<uint8_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
+                         static_cast<const uint8_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+    else
+      hipLaunchKernelGGL(k_sat_x<uint16_t>, dim3((rows + 3) / 4), dim3(256), 0, st,
+                         static_cast<const uint16_t*>(c->d_vol), c->d_ext_lut, c->nx, c->ny, c->nz, c->d_sat);
+    CVR_HIP_CHECK(hipGetLastError());
+    // y: lines over (x, z), stride w
+    if (const int rc = scan_lines(c->d_sat, w, d, size_t(w) * h, h, size_t(w), st)) return rc;
+  }
+  // z: lines over (x, y), stride w*h
   if (const int rc = scan_lines(c->d_sat, w, h, size_t(w), d, size_t(w) * h, st)) return rc;
   if (timed) CVR_HIP_CHECK(hipEventRecord(c->ev_sat[1], st));
   c->sat_built_for = key;

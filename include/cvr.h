@@ -378,7 +378,7 @@ typedef struct cvr_stats {
   uint64_t fallback_samples; /* surrogate: samples the exact tracer shaded */
   double last_ms;            /* device time of the last render's main kernel */
   uint64_t box_sums;         /* CVR_EBS: SAT box sums evaluated (8 trilinear SAT reads each) */
-  /* CVR_EBS with stats on: device time of the last SAT build, its three prefix passes
+  /* CVR_EBS with stats on: device time of the last SAT build, its prefix passes
    * and the corner-block copy the render kernels read (events around the kernels). */
   double sat_scan_ms;
   double sat_blocks_ms;

@@ -17,10 +17,12 @@ def sat_rel_err(g, r):
 
 
 @pytest.mark.parametrize("shape,dtype", [((19, 13, 11), N.U8), ((40, 33, 35), N.U16), ((64, 64, 64), N.U16),
-                                         # long lines along each axis: rows beyond one batch of row chunks,
-                                         # columns of 16 segments, and columns beyond them (one lane per column)
+                                         # long lines along each axis: rows of 13-16 cells per lane in the fused
+                                         # x/y pass, rows past it (the separate x pass, beyond one batch of row
+                                         # chunks), columns of 16 segments, and columns beyond them (one lane
+                                         # per column)
                                          ((1100, 5, 6), N.U8), ((6, 500, 5), N.U16), ((5, 7, 800), N.U8),
-                                         ((7, 900, 4), N.U16)])
+                                         ((4, 6, 1100), N.U16), ((7, 900, 4), N.U16)])
 def test_sat_matches_fp64(shape, dtype):
     if dtype == N.U8:
         vol = N.synth_blobs(shape, dtype, n_blobs=4, seed=8)
