@@ -421,11 +421,13 @@ def main():
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(device=dev)
-    if args.workload == "batch":
-        bench_batch(args, N, W, torch, stream)
-        return
-    if args.workload == "sat":
-        bench_sat(args, N, W, stream)
+    if args.workload in ("batch", "sat"):
+        if args.workload == "batch":
+            bench_batch(args, N, W, torch, stream)
+        else:
+            bench_sat(args, N, W, stream)
+        if args.calib_copy:
+            calib_copy(torch)
         return
     peaks = gather_peaks(N, stream)
     log(f"gather peaks (GB/s of 32-byte reads): {peaks}")
