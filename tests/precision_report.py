@@ -3,7 +3,7 @@ precision of the sampled densities (run on a GPU box: python tests/precision_rep
 
 Images: max-abs per RGBA channel over non-straddle pixels, RMSE, pixels over 2e-3, on
 every `stride`-th pixel of the full-size configurations.  SAT: relative error of the
-two-float fp32 table.  Pyramid: relative error of the fp32 levels.  MLP: bf16-MFMA
+stored (hi, lo) table.  Pyramid: relative error of the fp32 levels.  MLP: bf16-MFMA
 inference against the FP64 network on the same weights.  Surrogate: validation RMSE of
 the trained network against the exact tracer.  Prints a markdown table.
 """
@@ -63,7 +63,7 @@ def main():
     tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
     sref = O.sat(wl.volume, tf)
     rel = np.abs(sat - sref) / np.maximum(np.abs(sref), 1e-30)
-    row("SAT 258³", "fp32 two-float (hi, lo) accumulation", f"max relative {rel[sref > 0].max():.1e}")
+    row("SAT 258³", "x, y sums in fp64, z in fp32 two-float; stored (hi, lo)", f"max relative {rel[sref > 0].max():.1e}")
     light = wl.light()
     img = r.render(wl.camera(), 1024, 1024, light=light)
     prm = O.ebs_params(r.params(), light)
