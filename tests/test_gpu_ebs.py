@@ -16,13 +16,21 @@ def sat_rel_err(g, r):
     return float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-30 + 1e-12 * np.abs(r).max())))
 
 
+# shapes are (nx, ny, nz); rows of w = nx + 2 cells go through the fused x/y pass at
+# ceil(w / 64) rounded up to 2, 3, 4, 5, 6, 8, 12 or 16 cells per lane, wider ones through the
+# separate x pass; z columns of d = nz + 2 cells through up to 16 register segments of 48
+# cells, longer ones one lane per column
 @pytest.mark.parametrize("shape,dtype", [((19, 13, 11), N.U8), ((40, 33, 35), N.U16), ((64, 64, 64), N.U16),
-                                         # long lines along each axis: rows of 13-16 cells per lane in the fused
-                                         # x/y pass, rows past it (the separate x pass, beyond one batch of row
-                                         # chunks), columns of 16 segments, and columns beyond them (one lane
-                                         # per column)
-                                         ((1100, 5, 6), N.U8), ((6, 500, 5), N.U16), ((5, 7, 800), N.U8),
-                                         ((4, 6, 1100), N.U16), ((7, 900, 4), N.U16)])
+                                         # rows past 1024 cells (the separate x pass, beyond one batch of
+                                         # row chunks; then y columns of one segment, or one lane each past
+                                         # 16 segments) and long y columns in the fused pass
+                                         ((1100, 5, 6), N.U8), ((1030, 780, 2), N.U8), ((6, 500, 5), N.U16),
+                                         ((7, 900, 4), N.U16),
+                                         # z columns of 16 segments, and beyond them
+                                         ((5, 4, 700), N.U8), ((5, 7, 800), N.U8),
+                                         # 3, 4, 6, 8, 12 and 16 cells per lane in the fused pass
+                                         ((150, 9, 3), N.U16), ((200, 5, 4), N.U8), ((350, 4, 3), N.U16),
+                                         ((400, 6, 2), N.U8), ((700, 3, 3), N.U16), ((900, 3, 2), N.U8)])
 def test_sat_matches_fp64(shape, dtype):
     if dtype == N.U8:
         vol = N.synth_blobs(shape, dtype, n_blobs=4, seed=8)
