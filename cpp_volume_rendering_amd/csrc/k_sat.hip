@@ -85,18 +85,19 @@ __global__ __launch_bounds__(256) void k_sat_x(const T* __restrict__ vox, const 
 
 // Passes x and y fused, for rows of up to 64 E cells: one workgroup per z-slice of
 // xy_waves<E>() wavefronts, wavefront s taking rows [s R, (s + 1) R), lane l the cells
-// l + 64 k (k < E) of each, so every load and store is coalesced.  Phase 1: every
-// wavefront sums its rows' extinction per column (its segment's column totals,
-// exchanged through LDS).  Phase 2: a lane's running column sums start at the totals of
-// the segments above; each row adds its extinction to them, and the row's prefix along
-// x is a DPP scan of each 64-cell chunk plus the total of the chunks before it (lane
-// 63's value).  The voxels are read twice (the second time from L2; gathering the LUT
-// again measured faster than parking the extinction in the table between the phases),
-// the table written once.  The sums run in fp64, one add per cell and direction where the two-float adds
-// of the separate passes take about ten fp32 operations; a cell stores hi = fp32(v),
-// lo = fp32(v - hi).  Workgroups 0 .. nz - 1 take the interior slices z = 1 .. nz, the
-// last two the zero border slices (stores only), so a grid of nz + 2 <= CUs + 2
-// workgroups leaves no interior slice for a second round.
+// l + 64 k (k < E) of each, so every load and store is coalesced.
+//  - Phase 1: every wavefront sums its rows' extinction per column (its segment's
+//    column totals, exchanged through LDS).
+//  - Phase 2: a lane's running column sums start at the totals of the segments above;
+//    each row adds its extinction to them, and the row's prefix along x is a DPP scan of
+//    each 64-cell chunk plus the total of the chunks before it (lane 63's value).
+// The voxels are read twice, the second time from L2 (gathering the LUT again measured
+// faster than parking the extinction in the table between the phases); the table is
+// written once.  The sums run in fp64: one add per cell and direction, where a two-float
+// add takes about ten fp32 operations.  A cell stores hi = fp32(v), lo = fp32(v - hi).
+// Workgroups 0 .. nz - 1 take the interior slices z = 1 .. nz and the last two the zero
+// border slices (stores only), so a grid of nz + 2 <= CUs + 2 workgroups leaves no
+// interior slice for a second round.
 template <int E>
 constexpr int xy_waves() {
   return E <= 6 ? 16 : 8;  // 16 wavefronts hold at most 128 VGPRs each
