@@ -197,7 +197,7 @@ def host_cpu():
 
 
 def median_of_5(run, arg):
-    """One warm-up call, then the median wall time of 5 calls (BASELINE.md's protocol)."""
+    """One warm-up call, then the median wall time of 5 calls."""
     run(arg)
     ts = []
     for _ in range(5):

@@ -1,5 +1,5 @@
 """Precision report: each output of the HIP path against the FP64 oracle, per storage
-precision of the sampled densities (run on a GPU box: python tests/precision_report.py).
+precision of the sampled densities (run on an MI355X: python tests/precision_report.py).
 
 Images: max-abs per RGBA channel over non-straddle pixels, RMSE, pixels over 2e-3, on
 every `stride`-th pixel of the full-size configurations.  SAT: relative error of the
@@ -40,7 +40,7 @@ def img_stats(img, ref, m, stride):
 
 def main():
     print("# Precision report: the HIP path against the FP64 oracle\n")
-    print("Command (from the repository root, on a GPU box): `python tests/precision_report.py > profiles/precision_report.md`\n")
+    print("Command (from the repository root, on an MI355X): `python tests/precision_report.py > profiles/precision_report.md`\n")
     print("| output | storage / arithmetic | error vs FP64 oracle |")
     print("|---|---|---|")
     # plain caster, 128^3 u8 at 512^2 (every 4th pixel)
