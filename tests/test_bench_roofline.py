@@ -59,3 +59,19 @@ def test_gather_roofline_level_and_step_up():
     assert r["bound"] == "l2" and r["footprint_level"] == "mall" and r["frac"] <= 1.0
     r = bench.gather_roofline(9000.0, 1 << 20, peaks, None, 1.0)
     assert r["bound"] == "l1" and r["frac"] <= 1.0
+
+
+def test_more_than_one_gpu_is_refused():
+    """bench.py --gpus N with N > 1 (or a multi-rank launch) exits with a message before it
+    touches a device: the build runs on one GPU."""
+    import json
+    import os
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(bench.__file__).resolve().parent
+    for args, env in ((["--gpus", "2"], {}), (["--gpus", "1"], {"WORLD_SIZE": "2"})):
+        r = subprocess.run([sys.executable, str(root / "bench.py"), *args], capture_output=True, text=True,
+                           timeout=120, env={**os.environ, **env})
+        assert r.returncode == 2, (r.returncode, r.stderr[-300:])
+        assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
