@@ -76,12 +76,13 @@ def test_register_path_equals_training_path(nn_handle):
     assert np.abs(a - b).max() <= 2e-3 * (1 + np.abs(a).max())
 
 
-def test_gradient_matches_fp64(nn_handle):
+@pytest.mark.parametrize("n", [1024, 1000, 130])  # whole 64-sample chunks, and a last chunk of 40 or 2
+def test_gradient_matches_fp64(nn_handle, n):
     w = random_net(5, 0.2)
     nn_handle.nn_set_weights(w)
     rng = np.random.default_rng(6)
-    X = rng.uniform(0, 1, (1024, 16)).astype(np.float32)
-    T = rng.uniform(0, 1, (1024, 2)).astype(np.float32)
+    X = rng.uniform(0, 1, (n, 16)).astype(np.float32)
+    T = rng.uniform(0, 1, (n, 2)).astype(np.float32)
     loss, g = nn_handle.nn_gradient(X, T)
     rl, rg = O.mlp_loss_grad(w.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
     assert loss == pytest.approx(rl, rel=3e-2)
