@@ -236,6 +236,18 @@ def test_surrogate_frame_matches_fp64_oracle():
     assert_frame_parity(img, ref64, m64, 1e-2, 1, 0.005)
 
 
+def test_surrogate_frame_camera_inside_matches_fp64_oracle():
+    """A camera inside the volume box (rays start at the eye, not at the box entry) and a
+    wide field of view: the surrogate frame against the FP64 surrogate frame."""
+    wl, r, p = train_small()
+    cam = N.Camera((3.0, -2.0, 5.0), (0.0, 0.0, -40.0), (0.0, 1.0, 0.0), 60.0, 1.0)
+    light = wl.light()
+    img = r.render(cam, 64, 64, light=light)
+    ref, m, _ = SO.nn_oracle_frame(wl, r, cam, light, 64, 64)
+    assert ref[..., 3].max() > 0.2
+    assert_frame_parity(img, ref, m, 2e-3, 1, 0.005)
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_surrogate_frame_16bit_pyramid_matches_fp64_oracle(prec):
     """fp16 / bf16 pyramid storage: the oracle reads the same rounded values."""
