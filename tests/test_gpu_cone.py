@@ -105,6 +105,21 @@ def test_cone_render_small_own_pyramid():
     assert ref[..., 3].max() > 0.2
 
 
+def test_cone_render_camera_inside_ragged_frame():
+    """A camera inside the volume box with a wide field of view, on a frame whose sides
+    are not multiples of the tile size."""
+    wl = W.cone_blobs(n=40, width=61, height=47, seed=16)
+    r, p = make(wl)
+    light = wl.light()
+    cam = N.Camera((3.0, -2.0, 5.0), (0.0, 0.0, -40.0), (0.0, 1.0, 0.0), 60.0, 61 / 47)
+    img = r.render(cam, 61, 47, light=light)
+    tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
+    pyr = O.OPyr.build(wl.volume, tf, nlev=6)
+    ref, m = O.render_cone(wl.volume, tf, pyr, O.cone_params(wl.grid, light), cam, 61, 47)
+    assert_parity(img, ref, m)
+    assert ref[..., 3].max() > 0.2
+
+
 def test_cone_render_full_config_sparse():
     """256^3 u8 with 64 blobs at 1024^2, 8 + 1 cones x 64 steps, 6-level pyramid; the
     oracle renders every 32nd pixel on the GPU's pyramid (checked separately above)."""

@@ -68,6 +68,21 @@ def test_dos_render_small_own_pyramid(shadow_type, apply_occ, packing):
     assert ref[..., 3].max() > 0.2
 
 
+def test_dos_render_camera_inside_ragged_frame():
+    """A camera inside the volume box with a wide field of view (occlusion cones towards
+    an eye among the samples), on a frame whose sides are not multiples of the tile size."""
+    wl = W.cone_blobs(n=40, width=61, height=47, seed=17)
+    r, p = make(N.DOS, wl, apply_occlusion=1, apply_shadow=1, shadow_type=0, base_res=(32, 40, 24))
+    light = wl.light()
+    cam = N.Camera((3.0, -2.0, 5.0), (0.0, 0.0, -40.0), (0.0, 1.0, 0.0), 60.0, 61 / 47)
+    img = r.render(cam, 61, 47, light=light)
+    tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
+    pyr = O.OPyr.build(wl.volume, tf, base=(32, 40, 24))
+    ref, m = O.render_dos(wl.volume, tf, pyr, O.dos_params(p, wl.grid, light), cam, 61, 47)
+    assert_parity(img, ref, m)
+    assert ref[..., 3].max() > 0.2
+
+
 def test_dos_render_anisotropic_u16():
     vol = N.synth_noise((36, 28, 20), N.U16, octaves=3, seed=7, bands=[0.3, 0.6])
     rgb, a = W.seeded_tf(17, 5, max_density=4095)  # a 12-bit table over the 16-bit range

@@ -62,13 +62,13 @@ def test_sat_full_256_u16():
     assert g[-1, -1, -1] == pytest.approx(O.sat_input(wl.volume, O.OTF(wl.rgb_iso, wl.alpha_iso)).sum(), rel=1e-9)
 
 
-def ebs_pair(wl, w, h, stride=1, **params):
+def ebs_pair(wl, w, h, stride=1, cam=None, **params):
     r = N.Renderer(N.EBS)
     r.set_volume(wl.volume, wl.scale)
     r.set_transfer_function(wl.rgb_iso, wl.alpha_iso)
     p = r.set_params(**params)
     light = wl.light()
-    cam = wl.camera()
+    cam = cam or wl.camera()
     img = r.render(cam, w, h, light=light)
     tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
     sat = O.sat(wl.volume, tf)
@@ -98,6 +98,17 @@ def test_ebs_small(shadow_type, sat_layout):
     img, ref, m = ebs_pair(wl, 64, 64, shadow_type=shadow_type, sat_layout=sat_layout)
     assert_parity(img, ref, m)
     assert ref[..., :3].max() > 0.05
+
+
+@pytest.mark.parametrize("shadow_type", [1, 0])
+def test_ebs_camera_inside_ragged_frame(shadow_type):
+    """A camera inside the volume box with a wide field of view, on a frame whose sides
+    are not multiples of the tile size."""
+    wl = W.banded_noise(n=48, width=61, height=47, seed=14)
+    cam = N.Camera((3.0, -2.0, 5.0), (0.0, 0.0, -40.0), (0.0, 1.0, 0.0), 60.0, 61 / 47)
+    img, ref, m = ebs_pair(wl, 61, 47, cam=cam, shadow_type=shadow_type)
+    assert_parity(img, ref, m)
+    assert ref[..., 3].max() > 0.2
 
 
 def test_ebs_blobs_ao_only_and_shadow_only():
