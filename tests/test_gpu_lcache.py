@@ -63,6 +63,20 @@ def test_ebs_light_cache(shadow_type, res):
     assert full[..., 3].max() > 0.3
 
 
+def test_ebs_light_cache_camera_inside_ragged_frame():
+    """The march over the GPU's own cache with the camera inside the volume box and a
+    frame whose sides are not multiples of the tile size."""
+    wl = W.banded_noise(n=48, width=61, height=47, seed=18)
+    r, p = with_cache(N.EBS, wl, (24, 20, 16), shadow_type=1)
+    light = wl.light()
+    cam = N.Camera((3.0, -2.0, 5.0), (0.0, 0.0, -40.0), (0.0, 1.0, 0.0), 60.0, 61 / 47)
+    img = r.render(cam, 61, 47, light=light)
+    tf = O.OTF(wl.rgb_iso, wl.alpha_iso)
+    own, m = O.render_lc(wl.volume, tf, r.light_cache(), 1, 1, p.ka, p.kd, cam, 61, 47, scale=wl.scale)
+    parity(img, own, m)
+    assert own[..., 3].max() > 0.2
+
+
 @pytest.mark.parametrize("shadow_type,packing", [(2, 1), (0, 2)])
 def test_dos_light_cache(shadow_type, packing):
     wl = W.cone_blobs(n=40, width=64, height=48, seed=15)
